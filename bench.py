@@ -1,0 +1,164 @@
+"""PENNANT hydro cycle benchmark on one MI355X.
+
+Workload (default): Sedov blast on a 1440 x 1440 rect mesh (2,073,600 zones,
+8,294,400 sides), plain FP64, fused cycle.  A step is one hydro cycle over
+the whole mesh; the state is resident in HBM before the timed region starts.
+
+Prints ONE JSON line on stdout:
+  value       zone-cycles/s = zones * cycles / wall (HIP events on the
+              library's stream around exactly --steps cycles)
+  roofline    dominant kernel: algorithmic bytes per launch / its average
+              duration (HIP events between the launches in the same timed
+              region) against MI355X's 8 TB/s
+  cpu_baseline  the CPU oracle (a serial C++ restatement of the reference's
+              CPU task bodies, and its OpenMP variant) on a bounded sample of
+              the same workload, on this host
+Diagnostics go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "pennant-legion_amd", ROOT / "tests"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0
+BASELINE_METRIC = "zone-cycles/s on 1 MI355X (learned+mixed vs plain FP64) at \u22641% L1 density err"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_deck(name: str, decks):
+    big = 10 ** 8   # cstop large enough that the timed cycles never stop early
+    if name == "sedov1440":
+        return decks.sedov_1440(cstop=big), "sedov-rect-1440x1440"
+    if name == "noh1024":
+        return decks.noh_pie_1024(cstop=big), "noh-pie-1024x1024"
+    if name == "leblanc4096":
+        return decks.leblanc_4096x256(cstop=big), "leblanc-rect-4096x256"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def kernel_bytes(S: int, Z: int, P: int):
+    """Algorithmic (compulsory) HBM bytes per launch of each fused kernel:
+    every distinct input element read once, every output written once
+    (int32 maps, FP64 values)."""
+    return {
+        # mapsp1 4, smf 8, cf.x/cf.y/cmass/sfw.x/sfw.y 40 per side; zm zr ze
+        # zwrate zvol 40 + zp 8 per zone; px, pu 32 per point
+        "zone_pred": 52 * S + 48 * Z + 32 * P,
+        # pcside 4, cmass 8, cf 16 per corner; pcbeg 4, px/pu 32 in + 32 out,
+        # bc bits 1, vs 16 + pxp.x 8 out per point
+        "point": 28 * S + 93 * P,
+        # mapsp1 4, sfw 16 per side; zetot zvol zm zp 32 in, zwrate zetot ze
+        # zr zvol 40 out per zone; px 16, vs 16, pxp.x 8 per point
+        "zone_corr": 20 * S + 72 * Z + 40 * P,
+    }
+
+
+def cpu_baseline(deck_text: str, numz: int, budget_s: float):
+    """Times the oracle (serial, then OpenMP) on whole cycles of the same
+    workload after setup, for about budget_s seconds each."""
+    from oracle_lib import Oracle, lib
+    out = {}
+    for omp in (False, True):
+        o = Oracle(deck_text, omp=omp)
+        t0 = time.perf_counter()
+        o.run(1)
+        one = time.perf_counter() - t0
+        n = max(1, min(50, int(budget_s / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        o.run(n)
+        dt = time.perf_counter() - t0
+        out["omp" if omp else "serial"] = (numz * n / dt, n, dt)
+        del o
+    threads = lib().ora_omp_threads()
+    return out, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="sedov1440")
+    ap.add_argument("--mode", default="fused", choices=["fused", "tasks"])
+    ap.add_argument("--cpu-budget", type=float, default=8.0,
+                    help="seconds of CPU time per oracle variant (0 disables)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs "
+              "on a single device; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import pennant_amd as pa
+    from pennant_amd import decks
+
+    text, wname = workload_deck(args.workload, decks)
+    mode = pa.MODE_FUSED if args.mode == "fused" else pa.MODE_TASKS
+    t0 = time.perf_counter()
+    pb = pa.Problem(text)
+    log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
+    if args.warmup > 0:
+        pb.run(args.warmup, mode)
+    total_ms, phase_ms = pb.time_cycles(args.steps, mode, phases=(mode == pa.MODE_FUSED))
+    sc = pb.scalars()
+    assert sc["cycle"] == args.warmup + args.steps, sc
+    ms_per_step = total_ms / args.steps
+    value = pb.numz * args.steps / (total_ms * 1e-3)
+    log(f"{args.mode}: {args.steps} cycles in {total_ms:.3f} ms -> {ms_per_step * 1e3:.1f} us/cycle, "
+        f"{value:.4e} zone-cycles/s; t={sc['time']:.6e}")
+
+    roofline = None
+    if phase_ms is not None:
+        kb = kernel_bytes(pb.nums, pb.numz, pb.nump)
+        names = ("zone_pred", "point", "zone_corr", "finalize")
+        per = {n: phase_ms[i] / args.steps for i, n in enumerate(names)}
+        for n in names:
+            gbs = kb[n] / (per[n] * 1e-3) / 1e9 if n in kb else 0.0
+            log(f"  {n:10s} {per[n] * 1e3:8.1f} us/launch" +
+                (f"  {kb[n] / 1e6:8.1f} MB  {gbs:8.1f} GB/s" if n in kb else ""))
+        dom = max(("zone_pred", "point", "zone_corr"), key=lambda n: per[n])
+        achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None,
+                    "alg_bytes_per_launch": kb[dom], "avg_launch_us": round(per[dom] * 1e3, 2),
+                    "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1)}
+
+    cpu = None
+    if args.cpu_budget > 0:
+        res, threads = cpu_baseline(text, pb.numz, args.cpu_budget)
+        v_omp, n_omp, t_omp = res["omp"]
+        v_ser, n_ser, t_ser = res["serial"]
+        log(f"cpu oracle serial {v_ser:.3e} zone-cycles/s ({n_ser} cycles, {t_ser:.1f}s); "
+            f"omp x{threads} {v_omp:.3e} ({n_omp} cycles, {t_omp:.1f}s)")
+        cpu = {"value": round(v_omp, 1), "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+               "sample": f"{n_omp} cycles of {wname} after setup (OpenMP variant); serial "
+                         f"variant {v_ser:.4g} zone-cycles/s on 1 core over {n_ser} cycles",
+               "serial_value": round(v_ser, 1)}
+    pb.close()
+
+    line = {
+        "metric": BASELINE_METRIC,
+        "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wname, "form": f"plain-fp64-{args.mode}", "zones": pb.numz,
+                   "sides": pb.nums, "points": pb.nump},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

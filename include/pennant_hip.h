@@ -1,0 +1,360 @@
+/*
+ * pennant_hip.h -- C ABI of the MI355X (gfx950) PENNANT hydro path.
+ *
+ * Two layers:
+ *
+ *  1. Task entry points, pennant_hip_<task>(): one per Legion task ID on the
+ *     per-cycle hydro path.  Each replaces the TOC_PROC (CUDA) variant of that
+ *     task ID (see the citation on each declaration) and takes exactly the
+ *     fields of the task's region requirements, as device pointers, in the
+ *     reference's positional order.  A Legion HIP task wrapper unpacks its
+ *     accessors with acc.ptr(rect) and forwards them.
+ *
+ *  2. Driver entry points, pennant_*(): a whole-problem driver (deck parsing,
+ *     mesh generation, initialisation, the cycle loop with dt kept on the
+ *     device) mirroring Driver/Hydro (src/Driver.cc, src/Hydro.cc).  It runs
+ *     the cycle either through the task entry points (one launch per task, the
+ *     reference's structure) or through the fused cycle kernels.
+ *
+ * Conventions for every task entry point:
+ *   - all array arguments are device pointers; vector (double2) fields are
+ *     passed as a pair of SoA arrays (pennant_vec / pennant_cvec);
+ *   - scalar arguments that the reference passes in task->args are passed by
+ *     value; dt (task->futures[0] in the reference) is a device pointer to one
+ *     double, so a captured hipGraph can replay with a new dt;
+ *   - launches go on `stream`; nothing is allocated, nothing synchronises the
+ *     host; every result is deterministic (zone sums run over the zone's sides
+ *     in ascending side order and point sums over a point-corner CSR sorted by
+ *     side, which reproduces the reference's serial summation order);
+ *   - return value: 0, or a hipError_t code (or PENNANT_EINVAL for arguments
+ *     the kernels cannot accept).  Nothing calls exit().
+ */
+#ifndef PENNANT_HIP_H_
+#define PENNANT_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PENNANT_EINVAL (-22)
+#define PENNANT_EBADSIDES (-100)
+
+/* Mesh connectivity on the device.  Counts follow src/GenMesh.cc:469-526;
+ * maps follow src/Mesh.hh:33-105 (MAPSP1, MAPSP2, MAPSZ, MAPSS3, MAPSS4,
+ * ZNUMP) stored as int32.  Sides of a zone are contiguous
+ * (zsbeg[z]..zsbeg[z+1]-1).  pcbeg/pcside is the point-corner CSR: for point
+ * p, pcside[pcbeg[p]..pcbeg[p+1]-1] lists, in ascending order, the sides s
+ * with mapsp1[s] == p (corner s of the reference, src/Hydro.cc:1100-1116). */
+typedef struct pennant_mesh {
+    int32_t nump, numz, nums;
+    int32_t maxznump;          /* largest znump; at most PENNANT_MAX_ZSIDES */
+    const int32_t* mapsp1;
+    const int32_t* mapsp2;
+    const int32_t* mapsz;
+    const int32_t* mapss3;
+    const int32_t* mapss4;
+    const int32_t* znump;
+    const int32_t* zsbeg;      /* numz + 1 entries */
+    const int32_t* pcbeg;      /* nump + 1 entries */
+    const int32_t* pcside;     /* nums entries */
+} pennant_mesh;
+
+#define PENNANT_MAX_ZSIDES 8
+
+typedef struct { double* x; double* y; } pennant_vec;
+typedef struct { const double* x; const double* y; } pennant_cvec;
+
+/* Bytes of device workspace the reduction tasks (calc_vols, calc_dt_new,
+ * calc_dvol) need for their per-block partials. */
+size_t pennant_hip_reduce_workspace_bytes(void);
+
+/* --- Mesh (src/Mesh.hh:107-122 task IDs) ------------------------------- */
+
+/* TID_CALCCTRS: ex = (px1+px2)/2 per side; zx = sum px1*(1/n) per zone.
+ * Replaces gpu_calc_ctrs, src/Mesh.cu:45-115 (CPU body src/Mesh.cc:797-837). */
+int pennant_hip_calc_ctrs(const pennant_mesh* m, pennant_cvec px,
+                          pennant_vec ex, pennant_vec zx, hipStream_t stream);
+
+/* TID_CALCVOLS: sarea, svol per side; zarea, zvol per zone; *numsbad (device
+ * int) receives the count of sides with svol <= 0 (the task's SumOp<int>
+ * future; an integer sum, exact in any order).  `workspace` is not used and
+ * may be NULL.  Replaces gpu_calc_vols, src/Mesh.cu:117-210
+ * (src/Mesh.cc:928-989). */
+int pennant_hip_calc_vols(const pennant_mesh* m, pennant_cvec px, pennant_cvec zx,
+                          double* sarea, double* svol, double* zarea, double* zvol,
+                          int32_t* numsbad, void* workspace, hipStream_t stream);
+
+/* TID_CALCSIDEFRACS (init only): smf = sarea / zarea[z].
+ * src/Mesh.cc:1085-1101. */
+int pennant_hip_calc_side_fracs(const pennant_mesh* m, const double* sarea,
+                                const double* zarea, double* smf, hipStream_t stream);
+
+/* TID_CALCSURFVECS: ssurf = rotateCCW(ex - zx[z]).
+ * Replaces src/Mesh.cu:266-305 (src/Mesh.cc:1104-1123). */
+int pennant_hip_calc_surf_vecs(const pennant_mesh* m, pennant_cvec ex, pennant_cvec zx,
+                               pennant_vec ssurf, hipStream_t stream);
+
+/* TID_CALCEDGELEN: elen = |px2 - px1|.
+ * Replaces src/Mesh.cu:212-264 (src/Mesh.cc:1151-1179). */
+int pennant_hip_calc_edge_len(const pennant_mesh* m, pennant_cvec px, double* elen,
+                              hipStream_t stream);
+
+/* TID_CALCCHARLEN: zdl = min(1e99, min_s fac*sarea/elen), fac = 3 for
+ * triangles else 4.  Replaces src/Mesh.cu:307-372 (src/Mesh.cc:1216-1244). */
+int pennant_hip_calc_char_len(const pennant_mesh* m, const double* sarea,
+                              const double* elen, double* zdl, hipStream_t stream);
+
+/* --- Hydro (src/Hydro.hh:35-51 task IDs) ------------------------------- */
+
+/* TID_ADVPOSHALF: pxp = px0 + (dt/2)*pu0.
+ * Replaces gpu_adv_pos_half, src/Hydro.cu:82-122 (src/Hydro.cc:986-1006). */
+int pennant_hip_adv_pos_half(const pennant_mesh* m, pennant_cvec px0, pennant_cvec pu0,
+                             pennant_vec pxp, const double* dt, hipStream_t stream);
+
+/* TID_CALCRHO: zr = zm / zvol.
+ * Replaces src/Hydro.cu:124-163 (src/Hydro.cc:1034-1054). */
+int pennant_hip_calc_rho(const pennant_mesh* m, const double* zm, const double* zvol,
+                         double* zr, hipStream_t stream);
+
+/* TID_CALCCRNRMASS: pmaswt[p] += zr*zarea*0.5*(smf[s]+smf[ss3[s]]) over the
+ * point's corners (ascending s).  Replaces gpu_calc_crnr_mass,
+ * src/Hydro.cu:165-224 (src/Hydro.cc:1083-1117). */
+int pennant_hip_calc_crnr_mass(const pennant_mesh* m, const double* smf,
+                               const double* zr, const double* zarea,
+                               double* pmaswt, hipStream_t stream);
+
+/* TID_SUMCRNRFORCE: pf[p] += (sfp+sfq+sft)[s] - (sfp+sfq+sft)[ss3[s]] over
+ * the point's corners (ascending s).  Replaces gpu_sum_crnr_force,
+ * src/Hydro.cu:226-283 (src/Hydro.cc:1159-1193). */
+int pennant_hip_sum_crnr_force(const pennant_mesh* m, pennant_cvec sfp, pennant_cvec sfq,
+                               pennant_cvec sft, pennant_vec pf, hipStream_t stream);
+
+/* TID_CALCACCEL: pa = pf / max(pmaswt, 1e-99).
+ * Replaces src/Hydro.cu:285-323 (src/Hydro.cc:1235-1254). */
+int pennant_hip_calc_accel(const pennant_mesh* m, pennant_cvec pf, const double* pmaswt,
+                           pennant_vec pa, hipStream_t stream);
+
+/* TID_ADVPOSFULL: pu = pu0 + dt*pa; px = px0 + (dt*0.5)*(pu0+pu).
+ * Replaces src/Hydro.cu:325-371 (src/Hydro.cc:1281-1306). */
+int pennant_hip_adv_pos_full(const pennant_mesh* m, pennant_cvec px0, pennant_cvec pu0,
+                             pennant_cvec pa, pennant_vec px, pennant_vec pu,
+                             const double* dt, hipStream_t stream);
+
+/* TID_CALCWORK: zw = sum dwork (from 0), zetot += dwork per side (ascending),
+ * dwork = -(dt/2)*(dot(sfp+sfq, pu0[p1]+pu[p1])*pxp[p1].x
+ *                  + dot(-(sfp+sfq), pu0[p2]+pu[p2])*pxp[p2].x).
+ * Replaces gpu_calc_work, src/Hydro.cu:373-473 (src/Hydro.cc:1339-1406). */
+int pennant_hip_calc_work(const pennant_mesh* m, pennant_cvec sfp, pennant_cvec sfq,
+                          pennant_cvec pu, pennant_cvec pu0, pennant_cvec pxp,
+                          double* zw, double* zetot, const double* dt, hipStream_t stream);
+
+/* TID_CALCWORKRATE: zwrate = (zw + zp*(zvol-zvol0)) * (1/dt).
+ * Replaces src/Hydro.cu:475-522 (src/Hydro.cc:1483-1510). */
+int pennant_hip_calc_work_rate(const pennant_mesh* m, const double* zvol0,
+                               const double* zvol, const double* zw, const double* zp,
+                               double* zwrate, const double* dt, hipStream_t stream);
+
+/* TID_CALCENERGY: ze = zetot / (zm + 1e-99).
+ * Replaces src/Hydro.cu:524-561 (src/Hydro.cc:1545-1564). */
+int pennant_hip_calc_energy(const pennant_mesh* m, const double* zetot, const double* zm,
+                            double* ze, hipStream_t stream);
+
+/* TID_CALCDTNEW: *dtnew = min(1e99, min_z zdl*cfl/max(zdu, max(zss, 1e-99))).
+ * Replaces gpu_calc_dt_new + reduce_double<MinOp>, src/Hydro.cu:563-626 and
+ * src/CudaHelp.hh:14-41 (src/Hydro.cc:1591-1619). */
+int pennant_hip_calc_dt_new(const pennant_mesh* m, const double* zdl, const double* zdu,
+                            const double* zss, double cfl, double* dtnew,
+                            void* workspace, hipStream_t stream);
+
+/* TID_CALCDVOL: *dvovmax = max(1e-99, max_z |(zvol-zvol0)/zvol0|).
+ * Replaces gpu_calc_dvol, src/Hydro.cu:628-681 (src/Hydro.cc:1622-1644). */
+int pennant_hip_calc_dvol(const pennant_mesh* m, const double* zvol, const double* zvol0,
+                          double* dvovmax, void* workspace, hipStream_t stream);
+
+/* TID_CALCDT: *dtrec = min(1e99, *dtnew, *dtlast*cflv / *dvovmax) on the
+ * device (a one-thread kernel; the reference runs it as a host task,
+ * src/Hydro.cc:1647-1667). */
+int pennant_hip_calc_dt(const double* dtlast, const double* dtnew, const double* dvovmax,
+                        double cflv, double* dtrec, hipStream_t stream);
+
+/* --- PolyGas (src/PolyGas.hh:23-26), TTS (src/TTS.hh:23-25) ------------ */
+
+/* TID_CALCSTATEHALF: EOS at the half step (reads ZR, ZVOLP, ZVOL0, ZE,
+ * ZWRATE, ZM; writes ZP, ZSS).  Replaces gpu_state_half,
+ * src/PolyGas.cu:27-101 (src/PolyGas.cc:63-108). */
+int pennant_hip_calc_state_half(const pennant_mesh* m, const double* zr,
+                                const double* zvolp, const double* zvol0,
+                                const double* ze, const double* zwrate, const double* zm,
+                                double* zp, double* zss, double gamma, double ssmin,
+                                const double* dt, hipStream_t stream);
+
+/* TID_CALCFORCEPGAS: sf = -zp[z] * ssurf.
+ * Replaces gpu_calc_force_pgas, src/PolyGas.cu:103-142 (src/PolyGas.cc:164-185). */
+int pennant_hip_calc_force_pgas(const pennant_mesh* m, pennant_cvec ssurf,
+                                const double* zp, pennant_vec sf, hipStream_t stream);
+
+/* TID_CALCFORCETTS: sf = -alfa*max(zss,ssmin)^2*(zr*smf*zarea/sarea - zr)*ssurf.
+ * Replaces gpu_calc_force_tts, src/TTS.cu:22-95 (src/TTS.cc:55-104). */
+int pennant_hip_calc_force_tts(const pennant_mesh* m, const double* sarea,
+                               const double* smf, pennant_cvec ssurf, const double* zarea,
+                               const double* zr, const double* zss, pennant_vec sf,
+                               double alfa, double ssmin, hipStream_t stream);
+
+/* --- QCS (src/QCS.hh:25-30) --------------------------------------------- */
+
+/* TID_SETCORNERDIV: zuc, carea, ccos, cdiv, cevol, cdu.
+ * Replaces gpu_zone_centered_velocity + gpu_corner_divergence,
+ * src/QCS.cu:42-228 (src/QCS.cc:116-242). */
+int pennant_hip_set_corner_div(const pennant_mesh* m, pennant_cvec ex, const double* elen,
+                               pennant_cvec zx, pennant_cvec px, pennant_cvec pu,
+                               pennant_vec zuc, double* carea, double* ccos, double* cdiv,
+                               double* cevol, double* cdu, hipStream_t stream);
+
+/* TID_SETQCNFORCE: crmu, cqe1, cqe2.
+ * Replaces gpu_compute_crmu + gpu_compute_cqe, src/QCS.cu:230-355
+ * (src/QCS.cc:246-327). */
+int pennant_hip_set_qcn_force(const pennant_mesh* m, const double* elen, const double* cdiv,
+                              const double* cdu, const double* cevol, const double* zrp,
+                              const double* zss, pennant_cvec pu, double* crmu,
+                              pennant_vec cqe1, pennant_vec cqe2, double qgamma,
+                              double q1, double q2, hipStream_t stream);
+
+/* TID_SETFORCEQCS: cw, clamped ccos (read-write), sfq.
+ * Replaces gpu_set_force1 + gpu_set_force2, src/QCS.cu:357-440
+ * (src/QCS.cc:331-380). */
+int pennant_hip_set_force_qcs(const pennant_mesh* m, const double* carea,
+                              pennant_cvec cqe1, pennant_cvec cqe2, const double* elen,
+                              double* ccos, double* cw, pennant_vec sfq, hipStream_t stream);
+
+/* TID_SETVELDIFF: ztmp = max(0, max_s |dot(du,dx)|/elen); zdu = q1*zss + 2*q2*ztmp.
+ * Replaces gpu_set_vel_diff1 + gpu_set_vel_diff2, src/QCS.cu:442-549
+ * (src/QCS.cc:384-448). */
+int pennant_hip_set_vel_diff(const pennant_mesh* m, const double* elen, const double* zss,
+                             pennant_cvec px, pennant_cvec pu, double* ztmp, double* zdu,
+                             double q1, double q2, hipStream_t stream);
+
+/* --- HydroBC (src/HydroBC.hh:30-35) ------------------------------------- */
+
+/* TID_APPLYFIXEDBC: for the numb listed points, pu0 and pf are projected onto
+ * the plane with unit normal vfix: v -= dot(v, vfix)*vfix.
+ * Replaces gpu_apply_fixed_bc, src/HydroBC.cu:22-80 (src/HydroBC.cc:150-183). */
+int pennant_hip_apply_fixed_bc(const int32_t* mapbp, int32_t numb, double vfix_x,
+                               double vfix_y, pennant_vec pf, pennant_vec pu0,
+                               hipStream_t stream);
+
+/* --- Driver (src/Driver.hh:21-25) --------------------------------------- */
+
+/* TID_CALCGLOBALDT on the device (src/Driver.cc:213-245):
+ * dt = dtmax; cycle 0: min(dt, dtinit), else min(dt, dtfac * *dtlast);
+ * min(dt, tstop - *time); cycle > 0: min(dt, *dtrec). */
+int pennant_hip_calc_global_dt(const double* dtlast, const double* dtrec,
+                               const double* time, int32_t cycle, double dtmax,
+                               double dtinit, double dtfac, double tstop, double* dt,
+                               hipStream_t stream);
+
+/* ======================================================================== */
+/* Driver API (whole problem on one device).                                */
+
+typedef struct pennant_problem pennant_problem;
+
+enum pennant_mode {
+    PENNANT_MODE_TASKS = 0,   /* one launch per task ID, src/Hydro.cc:342-983 order */
+    PENNANT_MODE_FUSED = 1    /* fused cycle kernels (same arithmetic, fewer HBM passes) */
+};
+
+/* Parses a .pnt deck (src/InputFile.cc:24-104 syntax), generates the mesh
+ * (rect or pie, one piece), initialises it (src/Mesh.cc:616-623,
+ * src/Hydro.cc:241-339) and uploads everything to the current device.
+ * Returns NULL on error with a message in errbuf (the reference's fatal input
+ * errors, src/GenMesh.cc:89-127, src/Driver.cc:80-83, src/Mesh.cc:220-223,
+ * src/InputFile.cc:47-49, become this error return). */
+pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen);
+void pennant_destroy(pennant_problem* pb);
+
+/* Runs up to ncycles cycles (stopping at cstop or tstop) with the given mode,
+ * asynchronously on the problem's stream.  Returns 0 or a hipError_t. */
+int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode);
+
+/* Waits for the stream; returns 0, PENNANT_EBADSIDES (message in
+ * pennant_error) or a hipError_t. */
+int pennant_sync(pennant_problem* pb);
+
+/* Convenience: launch_cycles + sync, returns the number of cycles run or a
+ * negative error code. */
+int pennant_run(pennant_problem* pb, int ncycles, int mode);
+
+const char* pennant_error(pennant_problem* pb);
+
+/* Counts: out[0]=nump, out[1]=numz, out[2]=nums, out[3]=number of BC lists. */
+void pennant_sizes(pennant_problem* pb, int64_t* out4);
+
+/* Device control block, copied to the host (synchronises):
+ * out[0]=time, out[1]=dt (of the last cycle run), out[2]=dtrec, out[3]=cycle,
+ * out[4]=dtnew, out[5]=dvovmax, out[6]=numsbad_pred, out[7]=numsbad_corr,
+ * out[8]=next dt (calcGlobalDt for the next cycle), out[9]=done flag. */
+int pennant_scalars(pennant_problem* pb, double* out10);
+int pennant_set_scalars(pennant_problem* pb, const double* in10);
+
+/* Field access by reference field name ("px", "pu0", "zr", "sfq", ...;
+ * lower-case FID names, plus "exp" for EXP).  ncomp is 1 or 2; data is
+ * interleaved (x0,y0,x1,y1,...) for vector fields.  Synchronous. */
+int pennant_field_info(pennant_problem* pb, const char* name, int64_t* count, int* ncomp);
+int pennant_get_field(pennant_problem* pb, const char* name, double* host_out);
+int pennant_set_field(pennant_problem* pb, const char* name, const double* host_in);
+/* device pointer of one component (0 = x or scalar, 1 = y) */
+double* pennant_field_device(pennant_problem* pb, const char* name, int comp);
+
+/* Maps as int32 host copies: "mapsp1", "mapsp2", "mapsz", "mapss3", "mapss4",
+ * "znump", "zsbeg", "pcbeg", "pcside". */
+int pennant_get_map(pennant_problem* pb, const char* name, int32_t* host_out, int64_t* count);
+
+/* The device mesh descriptor (device pointers) and the problem's stream, for
+ * callers that drive the task entry points themselves. */
+const pennant_mesh* pennant_mesh_desc(pennant_problem* pb);
+hipStream_t pennant_stream(pennant_problem* pb);
+
+/* One doCycle step in task mode (steps numbered as in src/Hydro.cc:342-983:
+ * 0 copy zvol0, 1 point init + advPosHalf, 2 calcCtrs pred, 3 calcVols pred,
+ * 4 calcSurfVecs, 5 calcEdgeLen, 6 calcCharLen, 7 calcRho pred,
+ * 8 calcCrnrMass, 9 calcStateHalf, 10 calcForcePGas, 11 calcForceTTS,
+ * 12 setCornerDiv, 13 setQCnForce, 14 setForceQCS, 15 setVelDiff,
+ * 16 sumCrnrForce, 17 applyFixedBC (all lists), 18 checkBadSides pred,
+ * 19 calcAccel + advPosFull, 20 calcCtrs corr, 21 calcVols corr, 22 calcWork,
+ * 23 calcWorkRate, 24 calcEnergy, 25 calcRho corr, 26 calcDtNew, 27 calcDvol,
+ * 28 calcDt, 29 checkBadSides corr), using the control block's dt.
+ * begin_cycle runs calcGlobalDt into the control block; end_cycle advances
+ * time and cycle.  Synchronous; for step-by-step parity checks. */
+int pennant_begin_cycle(pennant_problem* pb);
+int pennant_step(pennant_problem* pb, int step);
+int pennant_end_cycle(pennant_problem* pb);
+
+/* Runs ncycles cycles in `mode` between HIP events recorded on the problem's
+ * stream, then synchronises.  *total_ms receives the elapsed time.  In fused
+ * mode, phase_ms (4 entries, may be NULL) receives the summed durations of
+ * the zone-predictor, point, zone-corrector and finalize kernels, measured
+ * by events recorded between their launches in every cycle.  Returns 0,
+ * PENNANT_EBADSIDES or a hipError_t. */
+int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
+                        double* phase_ms);
+
+/* Writes <path> in the reference's .xy format (src/WriteXY.cc:28-53). */
+int pennant_write_xy(pennant_problem* pb, const char* path);
+
+/* Host-only view of the deck and generated mesh (no device needed): the same
+ * parsing, error messages and numbering pennant_create uses.  Names for
+ * pennant_host_mesh_get: the int32 maps of pennant_get_map, and "px", "py"
+ * (double point coordinates). */
+typedef struct pennant_host_mesh pennant_host_mesh;
+pennant_host_mesh* pennant_host_mesh_create(const char* deck_text, char* errbuf, int errlen);
+int pennant_host_mesh_get(pennant_host_mesh* hm, const char* name, void* host_out, int64_t* count);
+void pennant_host_mesh_destroy(pennant_host_mesh* hm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENNANT_HIP_H_ */

@@ -70,7 +70,11 @@ struct Deck {
     bool parse(const char* text) {
         std::istringstream in(text);
         std::string line;
-        while (std::getline(in, line)) {
+        while (true) {
+            // getline + eof test as src/InputFile.cc:35-38: a last line with
+            // no terminating newline is not read
+            std::getline(in, line);
+            if (in.eof()) break;
             std::istringstream ls(line);
             std::string key;
             ls >> key;
@@ -202,7 +206,13 @@ bool gen_mesh(Ora& o) {
             const int64_t i = (p - 1) % npx, j = (p - 1) / npx + 1;
             const double th = dth * (double)(nzx - i);
             const double r = dr * (double)j;
-            o.px[p] = mk(r * std::cos(th), r * std::sin(th));
+            // src/GenMesh.cc:866-867 evaluates cos(th) and sin(th); the
+            // reference's g++ -O2 build (DEBUG=0, Makefile:29) fuses the pair
+            // into one glibc sincos call, whose results differ from separate
+            // sin/cos calls in the last bit for some angles.
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            o.px[p] = mk(r * cs, r * sn);
         }
         // src/GenMesh.cc:936-975
         for (int64_t z = 0; z < o.numz; ++z) o.znump[z] = (z / nzx == 0) ? 3 : 4;

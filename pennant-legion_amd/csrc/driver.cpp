@@ -1,0 +1,677 @@
+// Whole-problem driver: the host side above the task entry points.
+//
+// Mirrors the reference's host structure -- InputFile (src/InputFile.cc),
+// GenMesh one-piece numbering (src/GenMesh.cc), Mesh::init geometry
+// (src/Mesh.cc:616-623), HydroBC point lists (src/HydroBC.cc:63-144),
+// Hydro::init (src/Hydro.cc:241-339) and Driver::run's cycle loop
+// (src/Driver.cc:102-187) -- but keeps every field resident in HBM as SoA
+// arrays and keeps dt, time and the stop test on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+#include "problem.h"
+
+using namespace pn;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// .pnt deck, src/InputFile.cc:24-104.  Like the reference's getline/eof loop,
+// a final line with no terminating newline is not read.
+struct Deck {
+    std::map<std::string, std::string> kv;
+    std::string err;
+    bool parse(const char* text) {
+        const std::string all(text);
+        size_t pos = 0;
+        while (true) {
+            const size_t nl = all.find('\n', pos);
+            if (nl == std::string::npos) break;
+            std::istringstream ls(all.substr(pos, nl - pos));
+            pos = nl + 1;
+            std::string key;
+            ls >> key;
+            if (key.empty() || key[0] == '#') continue;
+            if (kv.count(key)) { err = "Duplicate key " + key + " in input file"; return false; }
+            std::string val;
+            std::getline(ls, val);
+            kv[key] = val;
+        }
+        return true;
+    }
+    template <class T> T get(const std::string& k, T d) const {
+        auto it = kv.find(k);
+        if (it == kv.end()) return d;
+        std::istringstream s(it->second);
+        T v = T();
+        s >> v;
+        return v;
+    }
+    std::vector<double> list(const std::string& k) const {
+        std::vector<double> out;
+        auto it = kv.find(k);
+        if (it == kv.end()) return out;
+        std::istringstream s(it->second);
+        double v;
+        while (s >> v) out.push_back(v);
+        return out;
+    }
+};
+
+bool read_params(const char* text, Params& q, std::string& err) {
+    Deck d;
+    if (!d.parse(text)) { err = d.err; return false; }
+    // Driver ctor, src/Driver.cc:78-87
+    q.cstop = d.get<int>("cstop", 999999);
+    q.tstop = d.get<double>("tstop", 1.e99);
+    if (q.cstop == 999999 && q.tstop == 1.e99) { err = "Must specify either cstop or tstop"; return false; }
+    q.dtmax = d.get<double>("dtmax", 1.e99);
+    q.dtinit = d.get<double>("dtinit", 1.e99);
+    q.dtfac = d.get<double>("dtfac", 1.2);
+    q.dtreport = d.get<int>("dtreport", 10);
+    // Mesh ctor, src/Mesh.cc:216-223
+    q.subregion = d.list("subregion");
+    if (!q.subregion.empty() && q.subregion.size() != 4) { err = "subregion must have 4 entries"; return false; }
+    // GenMesh ctor, src/GenMesh.cc:87-129
+    q.meshtype = d.get<std::string>("meshtype", "");
+    if (q.meshtype.empty()) { err = "must specify meshtype"; return false; }
+    if (q.meshtype != "pie" && q.meshtype != "rect" && q.meshtype != "hex") { err = "invalid meshtype " + q.meshtype; return false; }
+    const std::vector<double> mp = d.list("meshparams");
+    if (mp.empty()) { err = "must specify meshparams"; return false; }
+    if (mp.size() > 4) { err = "meshparams must have <= 4 values"; return false; }
+    q.nzx = (long)mp[0];
+    q.nzy = mp.size() >= 2 ? (long)mp[1] : q.nzx;
+    if (q.meshtype != "pie") q.lenx = mp.size() >= 3 ? mp[2] : 1.0;
+    else q.lenx = (mp.size() >= 3 ? mp[2] : 90.0) * M_PI / 180.0;
+    q.leny = mp.size() >= 4 ? mp[3] : 1.0;
+    if (q.nzx <= 0 || q.nzy <= 0 || q.lenx <= 0. || q.leny <= 0.) { err = "meshparams values must be positive"; return false; }
+    if (q.meshtype == "pie" && q.lenx >= 2. * M_PI) { err = "meshparams theta must be < 360"; return false; }
+    if (q.meshtype == "hex") { err = "meshtype hex is not supported (its generator is assert(false) in the reference, src/GenMesh.cc:875-896)"; return false; }
+    // Hydro ctor, src/Hydro.cc:206-214; PolyGas src/PolyGas.cc:58-59;
+    // TTS src/TTS.cc:46-47 (same ssmin key); QCS src/QCS.cc:84-86
+    q.cfl = d.get<double>("cfl", 0.6);
+    q.cflv = d.get<double>("cflv", 0.1);
+    q.rinit = d.get<double>("rinit", 1.);
+    q.einit = d.get<double>("einit", 0.);
+    q.rinitsub = d.get<double>("rinitsub", 1.);
+    q.einitsub = d.get<double>("einitsub", 0.);
+    q.uinitradial = d.get<double>("uinitradial", 0.);
+    q.bcx = d.list("bcx");
+    q.bcy = d.list("bcy");
+    q.gamma = d.get<double>("gamma", 5. / 3.);
+    q.ssmin = d.get<double>("ssmin", 0.);
+    q.alfa = d.get<double>("alfa", 0.5);
+    q.qgamma = d.get<double>("qgamma", 5. / 3.);
+    q.q1 = d.get<double>("q1", 0.);
+    q.q2 = d.get<double>("q2", 2.);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// One-piece mesh.  A zone's sides are generated as a cyclic run, so the side
+// maps are built zone by zone: side k of zone z joins corner k to corner
+// k+1 of the zone's CCW corner list.  The resulting numbering is the
+// reference's (src/GenMesh.cc:987-1282 with numpcx = numpcy = 1).
+
+void build_mesh(const Params& q, HostMesh& h) {
+    const long nzx = q.nzx, nzy = q.nzy, npx = nzx + 1;
+    const bool pie = (q.meshtype == "pie");
+    h.numz = (int)(nzx * nzy);
+    h.nump = pie ? (int)(npx * nzy + 1) : (int)(npx * (nzy + 1));
+    h.px.resize(h.nump);
+    h.py.resize(h.nump);
+    if (!pie) {
+        const double dx = q.lenx / (double)nzx, dy = q.leny / (double)nzy;
+        for (long j = 0; j <= nzy; ++j)
+            for (long i = 0; i <= nzx; ++i) {
+                h.px[j * npx + i] = dx * double(i);
+                h.py[j * npx + i] = dy * double(j);
+            }
+    } else {
+        const double dth = q.lenx / (double)nzx, dr = q.leny / (double)nzy;
+        h.px[0] = 0.;
+        h.py[0] = 0.;
+        for (long j = 1; j <= nzy; ++j)
+            for (long i = 0; i <= nzx; ++i) {
+                const double th = dth * (double)(nzx - i);
+                const double r = dr * (double)j;
+                // one glibc sincos call, as the reference's g++ -O2 build
+                // evaluates cos(th), sin(th) (src/GenMesh.cc:866-867)
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                h.px[(j - 1) * npx + i + 1] = r * cs;
+                h.py[(j - 1) * npx + i + 1] = r * sn;
+            }
+    }
+    h.znump.resize(h.numz);
+    h.zsbeg.resize(h.numz + 1);
+    std::vector<int> corners;   // CCW corner list of every zone, concatenated
+    corners.reserve(4 * (size_t)h.numz);
+    for (long zy = 0; zy < nzy; ++zy)
+        for (long zxi = 0; zxi < nzx; ++zxi) {
+            const long z = zy * nzx + zxi;
+            h.zsbeg[z] = (int)corners.size();
+            if (!pie) {
+                const int p0 = (int)(zy * npx + zxi);
+                corners.insert(corners.end(), {p0, p0 + 1, p0 + 1 + (int)npx, p0 + (int)npx});
+            } else if (zy == 0) {
+                const int p0 = (int)(zxi + 1);
+                corners.insert(corners.end(), {0, p0 + 1, p0});
+            } else {
+                const int p0 = (int)((zy - 1) * npx + zxi + 1);
+                corners.insert(corners.end(), {p0, p0 + 1, p0 + (int)npx + 1, p0 + (int)npx});
+            }
+            h.znump[z] = (int)corners.size() - h.zsbeg[z];
+        }
+    h.zsbeg[h.numz] = (int)corners.size();
+    h.nums = (int)corners.size();
+    h.mapsp1.resize(h.nums); h.mapsp2.resize(h.nums); h.mapsz.resize(h.nums);
+    h.mapss3.resize(h.nums); h.mapss4.resize(h.nums);
+    h.maxznump = 0;
+    for (int z = 0; z < h.numz; ++z) {
+        const int b = h.zsbeg[z], n = h.znump[z];
+        if (n > h.maxznump) h.maxznump = n;
+        for (int k = 0; k < n; ++k) {
+            const int s = b + k;
+            h.mapsp1[s] = corners[s];
+            h.mapsp2[s] = corners[b + (k + 1) % n];
+            h.mapsz[s] = z;
+            h.mapss3[s] = b + (k + n - 1) % n;
+            h.mapss4[s] = b + (k + 1) % n;
+        }
+    }
+    h.cyclic = true;
+    // point-corner CSR: sides grouped by mapsp1, ascending side order
+    h.pcbeg.assign(h.nump + 1, 0);
+    for (int s = 0; s < h.nums; ++s) h.pcbeg[h.mapsp1[s] + 1]++;
+    for (int p = 0; p < h.nump; ++p) h.pcbeg[p + 1] += h.pcbeg[p];
+    h.pcside.resize(h.nums);
+    std::vector<int> fill(h.pcbeg.begin(), h.pcbeg.end() - 1);
+    for (int s = 0; s < h.nums; ++s) h.pcside[fill[h.mapsp1[s]]++] = s;
+}
+
+template <class T> T* dalloc(pennant_problem* pb, size_t n) {
+    void* p = nullptr;
+    const size_t bytes = n * sizeof(T) > 0 ? n * sizeof(T) : sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, bytes) != hipSuccess) { (void)hipFree(p); return nullptr; }
+    pb->allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+template <class T> T* upload(pennant_problem* pb, const std::vector<T>& v) {
+    T* d = dalloc<T>(pb, v.size());
+    if (d && !v.empty() && hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return nullptr;
+    return d;
+}
+
+// Reference field catalogue (src/Mesh.cc:262-423): name, entity, components.
+struct FieldSpec { const char* name; int kind; int ncomp; };
+const FieldSpec kFields[] = {
+    {"px", 0, 2}, {"pxp", 0, 2}, {"px0", 0, 2}, {"pu", 0, 2}, {"pu0", 0, 2}, {"pf", 0, 2},
+    {"pap", 0, 2}, {"pmaswt", 0, 1},
+    {"zx", 1, 2}, {"zxp", 1, 2}, {"zuc", 1, 2}, {"zarea", 1, 1}, {"zvol", 1, 1},
+    {"zareap", 1, 1}, {"zvolp", 1, 1}, {"zvol0", 1, 1}, {"zdl", 1, 1}, {"zm", 1, 1},
+    {"zr", 1, 1}, {"zrp", 1, 1}, {"ze", 1, 1}, {"zetot", 1, 1}, {"zw", 1, 1},
+    {"zwrate", 1, 1}, {"zp", 1, 1}, {"zss", 1, 1}, {"zdu", 1, 1}, {"ztmp", 1, 1},
+    {"ex", 2, 2}, {"exp", 2, 2}, {"ssurfp", 2, 2}, {"sfp", 2, 2}, {"sfq", 2, 2}, {"sft", 2, 2},
+    {"cqe1", 2, 2}, {"cqe2", 2, 2}, {"sarea", 2, 1}, {"svol", 2, 1}, {"sareap", 2, 1},
+    {"svolp", 2, 1}, {"elen", 2, 1}, {"smf", 2, 1}, {"carea", 2, 1}, {"cevol", 2, 1},
+    {"cdu", 2, 1}, {"cdiv", 2, 1}, {"ccos", 2, 1}, {"crmu", 2, 1}, {"cw", 2, 1},
+};
+
+int64_t count_of(const pennant_problem* pb, int kind) {
+    return kind == 0 ? pb->dm.nump : (kind == 1 ? pb->dm.numz : pb->dm.nums);
+}
+
+#define CK(x) do { int _e = (int)(x); if (_e) return _e; } while (0)
+
+// One doCycle step through the task entry points; steps as in the header.
+int task_step(pennant_problem* pb, int st) {
+    const Params& q = pb->prm;
+    const pennant_mesh* m = &pb->dm;
+    hipStream_t s = pb->stream;
+    const double* dt = &pb->ctl->dt;
+    const size_t P = (size_t)m->nump * sizeof(double);
+    switch (st) {
+    case 0:  // copy ZVOL -> ZVOL0, src/Hydro.cc:361-368
+        CK(hipMemcpyAsync(pb->f("zvol0"), pb->f("zvol"), (size_t)m->numz * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+        return 0;
+    case 1:  // copies, fills and advPosHalf, src/Hydro.cc:389-437
+        CK(hipMemcpyAsync(pb->f("px0", 0), pb->f("px", 0), P, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(pb->f("px0", 1), pb->f("px", 1), P, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(pb->f("pu0", 0), pb->f("pu", 0), P, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(pb->f("pu0", 1), pb->f("pu", 1), P, hipMemcpyDeviceToDevice, s));
+        CK(hipMemsetAsync(pb->f("pmaswt"), 0, P, s));
+        CK(hipMemsetAsync(pb->f("pf", 0), 0, P, s));
+        CK(hipMemsetAsync(pb->f("pf", 1), 0, P, s));
+        return pennant_hip_adv_pos_half(m, pb->cv("px0"), pb->cv("pu0"), pb->vv("pxp"), dt, s);
+    case 2: return pennant_hip_calc_ctrs(m, pb->cv("pxp"), pb->vv("exp"), pb->vv("zxp"), s);
+    case 3:
+        return pennant_hip_calc_vols(m, pb->cv("pxp"), pb->cv("zxp"), pb->f("sareap"),
+                                     pb->f("svolp"), pb->f("zareap"), pb->f("zvolp"),
+                                     &pb->ctl->numsbad_pred, pb->ws, s);
+    case 4: return pennant_hip_calc_surf_vecs(m, pb->cv("exp"), pb->cv("zxp"), pb->vv("ssurfp"), s);
+    case 5: return pennant_hip_calc_edge_len(m, pb->cv("pxp"), pb->f("elen"), s);
+    case 6: return pennant_hip_calc_char_len(m, pb->f("sareap"), pb->f("elen"), pb->f("zdl"), s);
+    case 7: return pennant_hip_calc_rho(m, pb->f("zm"), pb->f("zvolp"), pb->f("zrp"), s);
+    case 8:
+        return pennant_hip_calc_crnr_mass(m, pb->f("smf"), pb->f("zrp"), pb->f("zareap"),
+                                          pb->f("pmaswt"), s);
+    case 9:
+        return pennant_hip_calc_state_half(m, pb->f("zr"), pb->f("zvolp"), pb->f("zvol0"),
+                                           pb->f("ze"), pb->f("zwrate"), pb->f("zm"),
+                                           pb->f("zp"), pb->f("zss"), q.gamma, q.ssmin, dt, s);
+    case 10: return pennant_hip_calc_force_pgas(m, pb->cv("ssurfp"), pb->f("zp"), pb->vv("sfp"), s);
+    case 11:
+        return pennant_hip_calc_force_tts(m, pb->f("sareap"), pb->f("smf"), pb->cv("ssurfp"),
+                                          pb->f("zareap"), pb->f("zrp"), pb->f("zss"),
+                                          pb->vv("sft"), q.alfa, q.ssmin, s);
+    case 12:
+        return pennant_hip_set_corner_div(m, pb->cv("exp"), pb->f("elen"), pb->cv("zxp"),
+                                          pb->cv("pxp"), pb->cv("pu0"), pb->vv("zuc"),
+                                          pb->f("carea"), pb->f("ccos"), pb->f("cdiv"),
+                                          pb->f("cevol"), pb->f("cdu"), s);
+    case 13:
+        return pennant_hip_set_qcn_force(m, pb->f("elen"), pb->f("cdiv"), pb->f("cdu"),
+                                         pb->f("cevol"), pb->f("zrp"), pb->f("zss"),
+                                         pb->cv("pu0"), pb->f("crmu"), pb->vv("cqe1"),
+                                         pb->vv("cqe2"), q.qgamma, q.q1, q.q2, s);
+    case 14:
+        return pennant_hip_set_force_qcs(m, pb->f("carea"), pb->cv("cqe1"), pb->cv("cqe2"),
+                                         pb->f("elen"), pb->f("ccos"), pb->f("cw"),
+                                         pb->vv("sfq"), s);
+    case 15:
+        return pennant_hip_set_vel_diff(m, pb->f("elen"), pb->f("zss"), pb->cv("pxp"),
+                                        pb->cv("pu0"), pb->f("ztmp"), pb->f("zdu"), q.q1, q.q2, s);
+    case 16:
+        return pennant_hip_sum_crnr_force(m, pb->cv("sfp"), pb->cv("sfq"), pb->cv("sft"),
+                                          pb->vv("pf"), s);
+    case 17:  // one launch per BC list, bcx lists then bcy lists (src/Hydro.cc:773-798)
+        for (const BCList& b : pb->bcs)
+            CK(pennant_hip_apply_fixed_bc(b.d_pts, b.numb, b.vx, b.vy, pb->vv("pf"),
+                                          pb->vv("pu0"), s));
+        return 0;
+    case 18: return 0;  // bad-side check: done on the device by the end-of-cycle kernel
+    case 19:
+        CK(pennant_hip_calc_accel(m, pb->cv("pf"), pb->f("pmaswt"), pb->vv("pap"), s));
+        return pennant_hip_adv_pos_full(m, pb->cv("px0"), pb->cv("pu0"), pb->cv("pap"),
+                                        pb->vv("px"), pb->vv("pu"), dt, s);
+    case 20: return pennant_hip_calc_ctrs(m, pb->cv("px"), pb->vv("ex"), pb->vv("zx"), s);
+    case 21:
+        return pennant_hip_calc_vols(m, pb->cv("px"), pb->cv("zx"), pb->f("sarea"), pb->f("svol"),
+                                     pb->f("zarea"), pb->f("zvol"), &pb->ctl->numsbad_corr,
+                                     pb->ws, s);
+    case 22:
+        return pennant_hip_calc_work(m, pb->cv("sfp"), pb->cv("sfq"), pb->cv("pu"), pb->cv("pu0"),
+                                     pb->cv("pxp"), pb->f("zw"), pb->f("zetot"), dt, s);
+    case 23:
+        return pennant_hip_calc_work_rate(m, pb->f("zvol0"), pb->f("zvol"), pb->f("zw"),
+                                          pb->f("zp"), pb->f("zwrate"), dt, s);
+    case 24: return pennant_hip_calc_energy(m, pb->f("zetot"), pb->f("zm"), pb->f("ze"), s);
+    case 25: return pennant_hip_calc_rho(m, pb->f("zm"), pb->f("zvol"), pb->f("zr"), s);
+    case 26:
+        return pennant_hip_calc_dt_new(m, pb->f("zdl"), pb->f("zdu"), pb->f("zss"), q.cfl,
+                                       &pb->ctl->dtnew, pb->ws, s);
+    case 27:
+        return pennant_hip_calc_dvol(m, pb->f("zvol"), pb->f("zvol0"), &pb->ctl->dvovmax, pb->ws, s);
+    case 28:
+        return pennant_hip_calc_dt(&pb->ctl->dt, &pb->ctl->dtnew, &pb->ctl->dvovmax, q.cflv,
+                                   &pb->ctl->dtrec, s);
+    case 29: return 0;
+    default: return PENNANT_EINVAL;
+    }
+}
+
+int copy_ctl_to_host(pennant_problem* pb) {
+    CK(hipMemcpyAsync(pb->ctl_host, pb->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, pb->stream));
+    return (int)hipStreamSynchronize(pb->stream);
+}
+
+int bad_sides_error(pennant_problem* pb) {
+    const Ctl& c = *pb->ctl_host;
+    if (c.err_cycle < 0) return 0;
+    char b[160];
+    snprintf(b, sizeof b, "Error: %d negative side volumes on cycle %d",
+             c.err_pred ? c.numsbad_pred : c.numsbad_corr, c.err_cycle);
+    pb->err = b;
+    return PENNANT_EBADSIDES;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen) {
+    pennant_problem* pb = new pennant_problem();
+    auto fail = [&](const std::string& msg) -> pennant_problem* {
+        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", msg.c_str());
+        pennant_destroy(pb);
+        return nullptr;
+    };
+    std::string err;
+    if (!read_params(deck_text, pb->prm, err)) return fail(err);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+    build_mesh(pb->prm, pb->hm);
+    const HostMesh& h = pb->hm;
+    if (h.maxznump > PENNANT_MAX_ZSIDES) return fail("zones with more than 8 sides are not supported");
+    if (hipStreamCreate(&pb->stream) != hipSuccess) return fail("hipStreamCreate failed");
+
+    pennant_mesh& m = pb->dm;
+    m.nump = h.nump; m.numz = h.numz; m.nums = h.nums; m.maxznump = h.maxznump;
+    m.mapsp1 = upload(pb, h.mapsp1); m.mapsp2 = upload(pb, h.mapsp2); m.mapsz = upload(pb, h.mapsz);
+    m.mapss3 = upload(pb, h.mapss3); m.mapss4 = upload(pb, h.mapss4); m.znump = upload(pb, h.znump);
+    m.zsbeg = upload(pb, h.zsbeg); m.pcbeg = upload(pb, h.pcbeg); m.pcside = upload(pb, h.pcside);
+    if (!m.mapsp1 || !m.pcside) return fail("device allocation failed");
+    for (const FieldSpec& fs : kFields) {
+        DevField d;
+        d.kind = fs.kind; d.ncomp = fs.ncomp; d.c[0] = d.c[1] = nullptr;
+        const int64_t n = count_of(pb, fs.kind);
+        for (int c = 0; c < fs.ncomp; ++c) {
+            d.c[c] = dalloc<double>(pb, (size_t)n);
+            if (!d.c[c]) return fail("device allocation failed");
+        }
+        pb->fields[fs.name] = d;
+    }
+    pb->ws = dalloc<double>(pb, pennant_hip_reduce_workspace_bytes() / sizeof(double));
+    pb->ctl = dalloc<Ctl>(pb, 1);
+    if (hipHostMalloc((void**)&pb->ctl_host, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)
+        return fail("hipHostMalloc failed");
+    std::memset(pb->ctl_host, 0, sizeof(Ctl));
+    pb->ctl_host->err_cycle = -1;
+    if (hipMemcpy(pb->ctl, pb->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pb->f("px", 0), h.px.data(), h.px.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pb->f("px", 1), h.py.data(), h.py.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail("initial upload failed");
+
+    // Mesh::init geometry on PX, src/Mesh.cc:616-623
+    hipStream_t s = pb->stream;
+    int rc = pennant_hip_calc_ctrs(&m, pb->cv("px"), pb->vv("ex"), pb->vv("zx"), s);
+    if (!rc) rc = pennant_hip_calc_vols(&m, pb->cv("px"), pb->cv("zx"), pb->f("sarea"), pb->f("svol"),
+                                        pb->f("zarea"), pb->f("zvol"), &pb->ctl->numsbad_corr, pb->ws, s);
+    if (!rc) rc = copy_ctl_to_host(pb);
+    if (rc) return fail("geometry initialisation failed: " + std::string(hipGetErrorString((hipError_t)rc)));
+    if (pb->ctl_host->numsbad_corr > 0) {
+        char b[128];
+        snprintf(b, sizeof b, "Error: %d negative side volumes during initialization", pb->ctl_host->numsbad_corr);
+        return fail(b);
+    }
+    pb->ctl_host->numsbad_corr = 0;
+    if (hipMemcpy(pb->ctl, pb->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
+        return fail("control block upload failed");
+    rc = pennant_hip_calc_side_fracs(&m, pb->f("sarea"), pb->f("zarea"), pb->f("smf"), s);
+
+    // HydroBC point lists, src/HydroBC.cc:63-144 (|coordinate - bound| < 1e-12,
+    // ascending point order)
+    const double eps = 1.e-12;
+    auto add_bc = [&](double b, bool xplane) {
+        std::vector<int> pts;
+        for (int p = 0; p < h.nump; ++p) {
+            const double c = xplane ? h.px[p] : h.py[p];
+            if (std::fabs(c - b) < eps) pts.push_back(p);
+        }
+        BCList L;
+        L.vx = xplane ? 1. : 0.; L.vy = xplane ? 0. : 1.;
+        L.numb = (int)pts.size();
+        L.d_pts = upload(pb, pts);
+        pb->bcs.push_back(L);
+    };
+    for (double b : pb->prm.bcx) add_bc(b, true);
+    for (double b : pb->prm.bcy) add_bc(b, false);
+
+    if (!rc) rc = launch_init_state(pb);
+    if (!rc) rc = launch_first_dt(pb);
+    if (!rc) rc = fused_setup(pb);
+    if (!rc) rc = (int)hipStreamSynchronize(s);
+    if (rc) return fail(std::string("initialisation failed: ") + hipGetErrorString((hipError_t)rc));
+    return pb;
+}
+
+void pennant_destroy(pennant_problem* pb) {
+    if (!pb) return;
+    if (pb->stream) (void)hipStreamSynchronize(pb->stream);
+    for (void* p : pb->allocs) (void)hipFree(p);
+    if (pb->ctl_host) (void)hipHostFree(pb->ctl_host);
+    if (pb->stream) (void)hipStreamDestroy(pb->stream);
+    delete pb;
+}
+
+const char* pennant_error(pennant_problem* pb) { return pb->err.c_str(); }
+
+void pennant_sizes(pennant_problem* pb, int64_t* out4) {
+    out4[0] = pb->dm.nump; out4[1] = pb->dm.numz; out4[2] = pb->dm.nums; out4[3] = (int64_t)pb->bcs.size();
+}
+
+int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
+    if (mode == PENNANT_MODE_FUSED) {
+        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr));
+        return 0;
+    }
+    if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
+    // Task mode mirrors Driver::run: the host reads the control block each
+    // cycle (the reference reads its dt future through host tasks).
+    for (int k = 0; k < ncycles; ++k) {
+        CK(copy_ctl_to_host(pb));
+        if (pb->ctl_host->done) break;
+        for (int st = 0; st < 30; ++st) CK(task_step(pb, st));
+        CK(launch_end_cycle(pb, 0));
+    }
+    return 0;
+}
+
+int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
+                        double* phase_ms) {
+    if (ncycles < 1) return PENNANT_EINVAL;
+    const bool phases = (mode == PENNANT_MODE_FUSED && phase_ms);
+    std::vector<hipEvent_t> ev(phases ? 5 * (size_t)ncycles + 2 : 2);
+    for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
+    int rc = (int)hipEventRecord(ev[0], pb->stream);
+    if (!rc) {
+        if (phases) {
+            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + 5 * (size_t)k]);
+        } else {
+            rc = pennant_launch_cycles(pb, ncycles, mode);
+        }
+    }
+    if (!rc) rc = (int)hipEventRecord(ev[1], pb->stream);
+    if (!rc) rc = (int)hipEventSynchronize(ev[1]);
+    float ms = 0.f;
+    if (!rc) rc = (int)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (!rc) *total_ms = ms;
+    if (!rc && phases) {
+        for (int j = 0; j < 4; ++j) phase_ms[j] = 0.;
+        for (int k = 0; k < ncycles && !rc; ++k)
+            for (int j = 0; j < 4 && !rc; ++j) {
+                rc = (int)hipEventElapsedTime(&ms, ev[2 + 5 * (size_t)k + j], ev[2 + 5 * (size_t)k + j + 1]);
+                phase_ms[j] += ms;
+            }
+    }
+    for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    return pennant_sync(pb);
+}
+
+int pennant_sync(pennant_problem* pb) {
+    CK(copy_ctl_to_host(pb));
+    return bad_sides_error(pb);
+}
+
+int pennant_run(pennant_problem* pb, int ncycles, int mode) {
+    CK(copy_ctl_to_host(pb));
+    const int c0 = pb->ctl_host->cycle;
+    CK(pennant_launch_cycles(pb, ncycles, mode));
+    CK(pennant_sync(pb));
+    return pb->ctl_host->cycle - c0;
+}
+
+int pennant_scalars(pennant_problem* pb, double* o) {
+    CK(copy_ctl_to_host(pb));
+    const Ctl& c = *pb->ctl_host;
+    o[0] = c.time; o[1] = c.dtlast; o[2] = c.dtrec; o[3] = c.cycle; o[4] = c.dtnew;
+    o[5] = c.dvovmax; o[6] = c.numsbad_pred; o[7] = c.numsbad_corr; o[8] = c.dt; o[9] = c.done;
+    return 0;
+}
+
+int pennant_set_scalars(pennant_problem* pb, const double* in) {
+    CK(copy_ctl_to_host(pb));
+    Ctl& c = *pb->ctl_host;
+    c.time = in[0]; c.dtlast = in[1]; c.dtrec = in[2]; c.cycle = (int)in[3]; c.dtnew = in[4];
+    c.dvovmax = in[5]; c.numsbad_pred = (int)in[6]; c.numsbad_corr = (int)in[7]; c.dt = in[8];
+    c.done = (int)in[9]; c.err_cycle = -1; c.err_pred = 0;
+    CK(hipMemcpyAsync(pb->ctl, pb->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, pb->stream));
+    return (int)hipStreamSynchronize(pb->stream);
+}
+
+int pennant_field_info(pennant_problem* pb, const char* name, int64_t* count, int* ncomp) {
+    auto it = pb->fields.find(name);
+    if (it == pb->fields.end()) return PENNANT_EINVAL;
+    *count = count_of(pb, it->second.kind);
+    *ncomp = it->second.ncomp;
+    return 0;
+}
+
+int pennant_get_field(pennant_problem* pb, const char* name, double* out) {
+    auto it = pb->fields.find(name);
+    if (it == pb->fields.end()) return PENNANT_EINVAL;
+    const DevField& d = it->second;
+    const int64_t n = count_of(pb, d.kind);
+    CK(hipStreamSynchronize(pb->stream));
+    std::vector<double> tmp((size_t)n);
+    for (int c = 0; c < d.ncomp; ++c) {
+        CK(hipMemcpy(tmp.data(), d.c[c], (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) out[i * d.ncomp + c] = tmp[(size_t)i];
+    }
+    return 0;
+}
+
+int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
+    auto it = pb->fields.find(name);
+    if (it == pb->fields.end()) return PENNANT_EINVAL;
+    const DevField& d = it->second;
+    const int64_t n = count_of(pb, d.kind);
+    CK(hipStreamSynchronize(pb->stream));
+    std::vector<double> tmp((size_t)n);
+    for (int c = 0; c < d.ncomp; ++c) {
+        for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = in[i * d.ncomp + c];
+        CK(hipMemcpy(d.c[c], tmp.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+double* pennant_field_device(pennant_problem* pb, const char* name, int comp) {
+    auto it = pb->fields.find(name);
+    if (it == pb->fields.end() || comp < 0 || comp >= it->second.ncomp) return nullptr;
+    return it->second.c[comp];
+}
+
+}  // extern "C"
+
+namespace {
+const std::vector<int>* host_map(const HostMesh& h, const char* name) {
+    const std::vector<int>* v = nullptr;
+    if (!strcmp(name, "mapsp1")) v = &h.mapsp1;
+    else if (!strcmp(name, "mapsp2")) v = &h.mapsp2;
+    else if (!strcmp(name, "mapsz")) v = &h.mapsz;
+    else if (!strcmp(name, "mapss3")) v = &h.mapss3;
+    else if (!strcmp(name, "mapss4")) v = &h.mapss4;
+    else if (!strcmp(name, "znump")) v = &h.znump;
+    else if (!strcmp(name, "zsbeg")) v = &h.zsbeg;
+    else if (!strcmp(name, "pcbeg")) v = &h.pcbeg;
+    else if (!strcmp(name, "pcside")) v = &h.pcside;
+    return v;
+}
+}  // namespace
+
+struct pennant_host_mesh {
+    Params prm;
+    HostMesh hm;
+};
+
+extern "C" {
+
+int pennant_get_map(pennant_problem* pb, const char* name, int32_t* out, int64_t* count) {
+    const std::vector<int>* v = host_map(pb->hm, name);
+    if (!v) return PENNANT_EINVAL;
+    *count = (int64_t)v->size();
+    if (out) std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+    return 0;
+}
+
+pennant_host_mesh* pennant_host_mesh_create(const char* deck_text, char* errbuf, int errlen) {
+    pennant_host_mesh* hm = new pennant_host_mesh();
+    std::string err;
+    if (!read_params(deck_text, hm->prm, err)) {
+        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", err.c_str());
+        delete hm;
+        return nullptr;
+    }
+    build_mesh(hm->prm, hm->hm);
+    return hm;
+}
+
+int pennant_host_mesh_get(pennant_host_mesh* hm, const char* name, void* out, int64_t* count) {
+    const HostMesh& h = hm->hm;
+    if (!strcmp(name, "px") || !strcmp(name, "py")) {
+        const std::vector<double>& v = (name[1] == 'x') ? h.px : h.py;
+        *count = (int64_t)v.size();
+        if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
+        return 0;
+    }
+    const std::vector<int>* v = host_map(h, name);
+    if (!v) return PENNANT_EINVAL;
+    *count = (int64_t)v->size();
+    if (out) std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+    return 0;
+}
+
+void pennant_host_mesh_destroy(pennant_host_mesh* hm) { delete hm; }
+
+const pennant_mesh* pennant_mesh_desc(pennant_problem* pb) { return &pb->dm; }
+hipStream_t pennant_stream(pennant_problem* pb) { return pb->stream; }
+
+int pennant_begin_cycle(pennant_problem* pb) {
+    CK(copy_ctl_to_host(pb));
+    return pb->ctl_host->done ? 1 : 0;
+}
+
+int pennant_step(pennant_problem* pb, int step) {
+    CK(task_step(pb, step));
+    CK(copy_ctl_to_host(pb));
+    if (step == 18 && pb->ctl_host->numsbad_pred > 0) return PENNANT_EBADSIDES;
+    if (step == 29 && pb->ctl_host->numsbad_corr > 0) return PENNANT_EBADSIDES;
+    return 0;
+}
+
+int pennant_end_cycle(pennant_problem* pb) {
+    CK(launch_end_cycle(pb, 0));
+    return pennant_sync(pb);
+}
+
+int pennant_write_xy(pennant_problem* pb, const char* path) {
+    const int64_t nz = pb->dm.numz;
+    std::vector<double> zr(nz), ze(nz), zp(nz);
+    CK(pennant_get_field(pb, "zr", zr.data()));
+    CK(pennant_get_field(pb, "ze", ze.data()));
+    CK(pennant_get_field(pb, "zp", zp.data()));
+    FILE* f = std::fopen(path, "w");
+    if (!f) return PENNANT_EINVAL;
+    const std::vector<double>* a[3] = {&zr, &ze, &zp};
+    const char* hdr[3] = {"#  zr", "#  ze", "#  zp"};
+    for (int k = 0; k < 3; ++k) {
+        std::fprintf(f, "%s\n", hdr[k]);
+        for (int64_t z = 0; z < nz; ++z) std::fprintf(f, "%5lld%18.8e\n", (long long)(z + 1), (*a[k])[z]);
+    }
+    std::fclose(f);
+    return 0;
+}
+
+}  // extern "C"

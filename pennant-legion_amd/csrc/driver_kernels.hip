@@ -1,0 +1,139 @@
+// Small device kernels the driver needs around the task entry points:
+// initial state (src/Hydro.cc:241-339, 1738-1807) and the end-of-cycle scalar
+// update that the reference runs as host tasks on futures
+// (src/Driver.cc:213-295, src/Hydro.cc:1647-1667).
+#include "common.h"
+#include "problem.h"
+
+using namespace pn;
+
+namespace {
+
+// fill zr/ze/zwrate (src/Hydro.cc:250-269), initSubrgnTask
+// (src/Hydro.cc:1738-1760), initHydroTask (src/Hydro.cc:1763-1784)
+__global__ void __launch_bounds__(PN_BLOCK) k_init_zones(int numz, pennant_cvec zx,
+                                                         const double* zvol, double* zr,
+                                                         double* ze, double* zwrate, double* zm,
+                                                         double* zetot, double rinit, double einit,
+                                                         int has_sub, double s0, double s1,
+                                                         double s2, double s3, double rinitsub,
+                                                         double einitsub) {
+    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (z >= numz) return;
+    double r = rinit, e = einit;
+    if (has_sub) {
+        const double eps = 1.e-12;
+        const double x = zx.x[z], y = zx.y[z];
+        if (x > (s0 - eps) && x < (s1 + eps) && y > (s2 - eps) && y < (s3 + eps)) {
+            r = rinitsub;
+            e = einitsub;
+        }
+    }
+    zr[z] = r;
+    ze[z] = e;
+    zwrate[z] = 0.;
+    const double m = r * zvol[z];
+    zm[z] = m;
+    zetot[z] = e * m;
+}
+
+// initRadialVelTask, src/Hydro.cc:1787-1807, or the PU = 0 fill
+__global__ void __launch_bounds__(PN_BLOCK) k_init_points(int nump, pennant_cvec px, pennant_vec pu,
+                                                          double vel) {
+    const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (p >= nump) return;
+    d2 u = mk(0., 0.);
+    if (vel != 0.) {
+        const double eps = 1.e-12;
+        const d2 x = ld(px, p);
+        const double pmag = len(x);
+        if (pmag > eps) u = dvs(scl(vel, x), pmag);
+    }
+    st(pu, p, u);
+}
+
+__device__ double global_dt(int cycle, double time, double dtlast, double dtrec, double dtmax,
+                            double dtinit, double dtfac, double tstop) {
+    // calcGlobalDtTask, src/Driver.cc:213-245
+    double dt = dtmax;
+    if (cycle == 0) {
+        if (dtinit < dt) dt = dtinit;
+    } else {
+        const double dtrecover = dtfac * dtlast;
+        if (dtrecover < dt) dt = dtrecover;
+    }
+    if ((tstop - time) < dt) dt = tstop - time;
+    if (cycle > 0) {
+        if (dtrec < dt) dt = dtrec;
+    }
+    return dt;
+}
+
+__global__ void k_first_dt(Ctl* c, int cstop, double dtmax, double dtinit, double dtfac,
+                           double tstop) {
+    c->dt = global_dt(c->cycle, c->time, c->dtlast, c->dtrec, dtmax, dtinit, dtfac, tstop);
+    c->done = (c->cycle >= cstop || c->time >= tstop) ? 1 : 0;
+}
+
+// End of cycle: calcDt (when the fused path did not run it as a task),
+// checkBadSides (src/Mesh.cc:1531-1550), updateTime/updateCycle
+// (src/Driver.cc:266-295), the PENNANT stop test (time >= tstop or
+// cycle >= cstop) and calcGlobalDt for the next cycle.
+__global__ void k_end_cycle(Ctl* c, int compute_dtrec, double cflv, int cstop, double dtmax,
+                            double dtinit, double dtfac, double tstop) {
+    if (c->done) return;
+    if (compute_dtrec) {
+        // calcDtTask, src/Hydro.cc:1647-1667
+        double r = 1.e99;
+        if (c->dtnew < r) r = c->dtnew;
+        const double b = c->dt * cflv / c->dvovmax;
+        if (b < r) r = b;
+        c->dtrec = r;
+    }
+    if (c->numsbad_pred > 0 || c->numsbad_corr > 0) {
+        c->err_cycle = c->cycle;
+        c->err_pred = c->numsbad_pred > 0 ? 1 : 0;
+        c->done = 1;
+        return;
+    }
+    c->time = c->time + c->dt;
+    c->cycle = c->cycle + 1;
+    c->dtlast = c->dt;
+    c->done = (c->cycle >= cstop || c->time >= tstop) ? 1 : 0;
+    c->dt = global_dt(c->cycle, c->time, c->dtlast, c->dtrec, dtmax, dtinit, dtfac, tstop);
+}
+
+}  // namespace
+
+namespace pn {
+
+int launch_init_state(pennant_problem* pb) {
+    const Params& q = pb->prm;
+    const int nz = pb->dm.numz, np = pb->dm.nump;
+    const bool sub = !q.subregion.empty();
+    hipLaunchKernelGGL(k_init_zones, dim3(nblocks(nz)), dim3(PN_BLOCK), 0, pb->stream, nz,
+                       pb->cv("zx"), pb->f("zvol"), pb->f("zr"), pb->f("ze"), pb->f("zwrate"),
+                       pb->f("zm"), pb->f("zetot"), q.rinit, q.einit, sub ? 1 : 0,
+                       sub ? q.subregion[0] : 0., sub ? q.subregion[1] : 0.,
+                       sub ? q.subregion[2] : 0., sub ? q.subregion[3] : 0., q.rinitsub,
+                       q.einitsub);
+    hipLaunchKernelGGL(k_init_points, dim3(nblocks(np)), dim3(PN_BLOCK), 0, pb->stream, np,
+                       pb->cv("px"), pb->vv("pu"), q.uinitradial);
+    return (int)hipGetLastError();
+}
+
+int launch_first_dt(pennant_problem* pb) {
+    const Params& q = pb->prm;
+    hipLaunchKernelGGL(k_first_dt, dim3(1), dim3(1), 0, pb->stream, pb->ctl, q.cstop, q.dtmax,
+                       q.dtinit, q.dtfac, q.tstop);
+    return (int)hipGetLastError();
+}
+
+int launch_end_cycle(pennant_problem* pb, int compute_dtrec) {
+    const Params& q = pb->prm;
+    hipLaunchKernelGGL(k_end_cycle, dim3(1), dim3(1), 0, pb->stream, pb->ctl, compute_dtrec,
+                       q.cflv, q.cstop, q.dtmax, q.dtinit, q.dtfac, q.tstop);
+    return (int)hipGetLastError();
+}
+
+}  // namespace pn

@@ -1,0 +1,101 @@
+// Host-side problem state shared by the driver translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pennant_hip.h"
+
+namespace pn {
+
+// Device control block: the per-cycle scalars the reference threads through
+// futures (src/Driver.cc:127-158, src/Hydro.cc:952-977).  `dt` is the dt of
+// the cycle about to run; kernels read it from here so that no host sync is
+// needed between cycles.
+struct Ctl {
+    double time;
+    double dt;        // dt for the next cycle (calcGlobalDt result)
+    double dtlast;    // dt of the last completed cycle
+    double dtrec;     // calcDt result of the last completed cycle
+    double dtnew;
+    double dvovmax;
+    int cycle;
+    int done;         // 1 once cycle >= cstop, time >= tstop, or bad sides
+    int numsbad_pred;
+    int numsbad_corr;
+    int err_cycle;    // cycle whose bad-side check failed, or -1
+    int err_pred;     // 1 when the failed check was the predictor's
+};
+
+struct Params {
+    int cstop; double tstop, dtmax, dtinit, dtfac; int dtreport;
+    double cfl, cflv, rinit, einit, rinitsub, einitsub, uinitradial;
+    std::vector<double> bcx, bcy, subregion;
+    double gamma, ssmin, alfa, qgamma, q1, q2;
+    std::string meshtype; long nzx, nzy; double lenx, leny;
+};
+
+struct DevField {
+    int kind;        // 0 point, 1 zone, 2 side
+    int ncomp;       // 1 or 2
+    double* c[2];
+};
+
+struct HostMesh {
+    int nump = 0, numz = 0, nums = 0, maxznump = 0;
+    std::vector<int> mapsp1, mapsp2, mapsz, mapss3, mapss4, znump, zsbeg, pcbeg, pcside;
+    std::vector<double> px, py;
+    bool cyclic = true;   // sides of each zone in cyclic order (fused path requirement)
+};
+
+struct BCList {
+    double vx, vy;
+    int numb;
+    int* d_pts;
+};
+
+}  // namespace pn
+
+struct pennant_problem {
+    pn::Params prm;
+    pn::HostMesh hm;
+    pennant_mesh dm;                 // device pointers
+    std::vector<void*> allocs;
+    std::map<std::string, pn::DevField> fields;
+    std::vector<pn::BCList> bcs;
+    pn::Ctl* ctl = nullptr;          // device
+    pn::Ctl* ctl_host = nullptr;     // pinned mirror
+    void* ws = nullptr;              // reduction workspace
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // fused-path buffers
+    double* cfx = nullptr; double* cfy = nullptr; double* cmass = nullptr;
+    double* sfwx = nullptr; double* sfwy = nullptr;
+    double* vsx = nullptr; double* vsy = nullptr; double* pxpx = nullptr;
+    unsigned char* pbc = nullptr;    // per-point BC membership bits (one per list)
+    double* part_d = nullptr;        // fused reduction partials
+    int* part_i = nullptr;
+    int nparts = 0;
+    std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
+    int nbcs = 0;
+    double bcvx[8], bcvy[8];
+
+    double* f(const char* name, int comp = 0) { return fields.at(name).c[comp]; }
+    pennant_cvec cv(const char* name) { pn::DevField& d = fields.at(name); pennant_cvec v = {d.c[0], d.c[1]}; return v; }
+    pennant_vec vv(const char* name) { pn::DevField& d = fields.at(name); pennant_vec v = {d.c[0], d.c[1]}; return v; }
+};
+
+namespace pn {
+// fused cycle (fused.hip)
+int fused_setup(pennant_problem* pb);
+// ev: NULL, or 5 events recorded before K1, K2, K3, K4 and after K4
+int fused_cycle(pennant_problem* pb, hipEvent_t* ev);
+// device init and end-of-cycle kernels (driver_kernels.hip)
+int launch_init_state(pennant_problem* pb);
+int launch_end_cycle(pennant_problem* pb, int compute_dtrec);
+int launch_first_dt(pennant_problem* pb);
+}  // namespace pn

@@ -1,0 +1,294 @@
+"""Python binding of the gfx950 PENNANT hydro library (libpennant_hip.so).
+
+The library's C ABI (include/pennant_hip.h) is the product: task entry points
+``pennant_hip_<task>`` mirroring the reference's Legion task IDs, and a
+whole-problem driver ``pennant_*`` mirroring Driver/Hydro (src/Driver.cc,
+src/Hydro.cc).  This module only loads the shared library with ctypes and
+wraps the driver calls; it has no compute of its own and no fallback: if the
+library is missing, importing :class:`Problem` or calling :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libpennant_hip.so"
+
+MODE_TASKS = 0
+MODE_FUSED = 1
+EBADSIDES = -100
+EINVAL = -22
+
+# doCycle step numbering shared with the oracle (src/Hydro.cc:342-983 order)
+NSTEPS = 30
+
+_lib = None
+
+
+class PennantError(RuntimeError):
+    pass
+
+
+class _Vec(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p)]
+
+
+class Mesh(ctypes.Structure):
+    """pennant_mesh: device connectivity (include/pennant_hip.h)."""
+    _fields_ = [("nump", ctypes.c_int32), ("numz", ctypes.c_int32), ("nums", ctypes.c_int32),
+                ("maxznump", ctypes.c_int32)] + [
+        (n, ctypes.c_void_p) for n in ("mapsp1", "mapsp2", "mapsz", "mapss3", "mapss4", "znump",
+                                       "zsbeg", "pcbeg", "pcside")]
+
+
+def vec(x, y):
+    """SoA vector-field argument from two device addresses."""
+    return _Vec(ctypes.c_void_p(int(x)), ctypes.c_void_p(int(y)))
+
+
+# argument kinds used to declare the task entry points
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int32
+_M = ctypes.POINTER(Mesh)
+_V = _Vec
+
+TASK_SIGNATURES = {
+    "pennant_hip_calc_ctrs": [_M, _V, _V, _V, _P],
+    "pennant_hip_calc_vols": [_M, _V, _V, _P, _P, _P, _P, _P, _P, _P],
+    "pennant_hip_calc_side_fracs": [_M, _P, _P, _P, _P],
+    "pennant_hip_calc_surf_vecs": [_M, _V, _V, _V, _P],
+    "pennant_hip_calc_edge_len": [_M, _V, _P, _P],
+    "pennant_hip_calc_char_len": [_M, _P, _P, _P, _P],
+    "pennant_hip_adv_pos_half": [_M, _V, _V, _V, _P, _P],
+    "pennant_hip_calc_rho": [_M, _P, _P, _P, _P],
+    "pennant_hip_calc_crnr_mass": [_M, _P, _P, _P, _P, _P],
+    "pennant_hip_sum_crnr_force": [_M, _V, _V, _V, _V, _P],
+    "pennant_hip_calc_accel": [_M, _V, _P, _V, _P],
+    "pennant_hip_adv_pos_full": [_M, _V, _V, _V, _V, _V, _P, _P],
+    "pennant_hip_calc_work": [_M, _V, _V, _V, _V, _V, _P, _P, _P, _P],
+    "pennant_hip_calc_work_rate": [_M, _P, _P, _P, _P, _P, _P, _P],
+    "pennant_hip_calc_energy": [_M, _P, _P, _P, _P],
+    "pennant_hip_calc_dt_new": [_M, _P, _P, _P, _D, _P, _P, _P],
+    "pennant_hip_calc_dvol": [_M, _P, _P, _P, _P, _P],
+    "pennant_hip_calc_dt": [_P, _P, _P, _D, _P, _P],
+    "pennant_hip_calc_state_half": [_M, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P],
+    "pennant_hip_calc_force_pgas": [_M, _V, _P, _V, _P],
+    "pennant_hip_calc_force_tts": [_M, _P, _P, _V, _P, _P, _P, _V, _D, _D, _P],
+    "pennant_hip_set_corner_div": [_M, _V, _P, _V, _V, _V, _V, _P, _P, _P, _P, _P, _P],
+    "pennant_hip_set_qcn_force": [_M, _P, _P, _P, _P, _P, _P, _V, _P, _V, _V, _D, _D, _D, _P],
+    "pennant_hip_set_force_qcs": [_M, _P, _V, _V, _P, _P, _P, _V, _P],
+    "pennant_hip_set_vel_diff": [_M, _P, _P, _V, _V, _P, _P, _D, _D, _P],
+    "pennant_hip_apply_fixed_bc": [_P, _I, _D, _D, _V, _V, _P],
+    "pennant_hip_calc_global_dt": [_P, _P, _P, _I, _D, _D, _D, _D, _P, _P],
+}
+
+DRIVER_SIGNATURES = {
+    "pennant_create": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
+    "pennant_destroy": ([_P], None),
+    "pennant_launch_cycles": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pennant_sync": ([_P], ctypes.c_int),
+    "pennant_run": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pennant_error": ([_P], ctypes.c_char_p),
+    "pennant_sizes": ([_P, ctypes.POINTER(ctypes.c_int64)], None),
+    "pennant_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_set_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_field_info": ([_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "pennant_get_field": ([_P, ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "pennant_set_field": ([_P, ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "pennant_field_device": ([_P, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
+    "pennant_get_map": ([_P, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
+                        ctypes.c_int),
+    "pennant_mesh_desc": ([_P], ctypes.POINTER(Mesh)),
+    "pennant_stream": ([_P], ctypes.c_void_p),
+    "pennant_begin_cycle": ([_P], ctypes.c_int),
+    "pennant_step": ([_P, ctypes.c_int], ctypes.c_int),
+    "pennant_end_cycle": ([_P], ctypes.c_int),
+    "pennant_write_xy": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_hip_reduce_workspace_bytes": ([], ctypes.c_size_t),
+    "pennant_host_mesh_create": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
+    "pennant_host_mesh_get": ([_P, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
+                              ctypes.c_int),
+    "pennant_host_mesh_destroy": ([_P], None),
+    "pennant_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpennant_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise PennantError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C pennant-legion_amd/csrc`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, args in TASK_SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        for name, (args, res) in DRIVER_SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(TASK_SIGNATURES) + list(DRIVER_SIGNATURES)
+
+
+class HostMesh:
+    """Deck parsing and mesh generation without a device (host logic only)."""
+
+    def __init__(self, deck_text: str):
+        L = lib()
+        err = ctypes.create_string_buffer(512)
+        h = L.pennant_host_mesh_create(deck_text.encode(), err, len(err))
+        if not h:
+            raise PennantError(err.value.decode())
+        self._h = ctypes.c_void_p(h)
+
+    def get(self, name: str) -> np.ndarray:
+        L = lib()
+        n = ctypes.c_int64()
+        if L.pennant_host_mesh_get(self._h, name.encode(), None, ctypes.byref(n)) != 0:
+            raise KeyError(name)
+        out = np.empty(n.value, dtype=np.float64 if name in ("px", "py") else np.int32)
+        L.pennant_host_mesh_get(self._h, name.encode(), out.ctypes.data, ctypes.byref(n))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pennant_host_mesh_destroy(h)
+            self._h = None
+
+
+class Problem:
+    """One PENNANT problem resident on the current HIP device."""
+
+    def __init__(self, deck_text: str):
+        L = lib()
+        err = ctypes.create_string_buffer(512)
+        h = L.pennant_create(deck_text.encode(), err, len(err))
+        if not h:
+            raise PennantError(err.value.decode())
+        self._h = ctypes.c_void_p(h)
+        sz = (ctypes.c_int64 * 4)()
+        L.pennant_sizes(self._h, sz)
+        self.nump, self.numz, self.nums, self.nbcs = (int(v) for v in sz)
+
+    @classmethod
+    def from_file(cls, path) -> "Problem":
+        return cls(Path(path).read_text())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pennant_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0 and rc == EBADSIDES:
+            raise PennantError(lib().pennant_error(self._h).decode())
+        if rc != 0:
+            raise PennantError(f"HIP/pennant error {rc}")
+
+    # -- time loop ---------------------------------------------------------
+    def run(self, ncycles: int, mode: int = MODE_FUSED) -> int:
+        rc = lib().pennant_run(self._h, int(ncycles), int(mode))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def launch(self, ncycles: int, mode: int = MODE_FUSED):
+        self._check(lib().pennant_launch_cycles(self._h, int(ncycles), int(mode)))
+
+    def sync(self):
+        self._check(lib().pennant_sync(self._h))
+
+    def time_cycles(self, ncycles: int, mode: int = MODE_FUSED, phases: bool = True):
+        """(total_ms, phase_ms[4] or None) for ncycles cycles timed with HIP
+        events on the problem's stream."""
+        tot = ctypes.c_double()
+        ph = (ctypes.c_double * 4)() if phases else None
+        self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
+        return tot.value, (list(ph) if phases and mode == MODE_FUSED else None)
+
+    def scalars(self) -> dict:
+        out = (ctypes.c_double * 10)()
+        self._check(lib().pennant_scalars(self._h, out))
+        keys = ("time", "dt", "dtrec", "cycle", "dtnew", "dvovmax", "numsbad_pred",
+                "numsbad_corr", "dtnext", "done")
+        return dict(zip(keys, list(out)))
+
+    def set_scalars(self, vals):
+        arr = (ctypes.c_double * 10)(*vals)
+        self._check(lib().pennant_set_scalars(self._h, arr))
+
+    # -- fields ------------------------------------------------------------
+    def field_info(self, name: str):
+        n = ctypes.c_int64()
+        c = ctypes.c_int()
+        self._check(lib().pennant_field_info(self._h, name.encode(), ctypes.byref(n), ctypes.byref(c)))
+        return n.value, c.value
+
+    def get(self, name: str) -> np.ndarray:
+        n, c = self.field_info(name)
+        out = np.empty(n * c, dtype=np.float64)
+        self._check(lib().pennant_get_field(self._h, name.encode(), out.ctypes.data))
+        return out.reshape(n, 2) if c == 2 else out
+
+    def set(self, name: str, values):
+        n, c = self.field_info(name)
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if arr.size != n * c:
+            raise ValueError(f"{name}: expected {n * c} values, got {arr.size}")
+        self._check(lib().pennant_set_field(self._h, name.encode(), arr.ctypes.data))
+
+    def device_ptr(self, name: str, comp: int = 0) -> int:
+        p = lib().pennant_field_device(self._h, name.encode(), comp)
+        if not p:
+            raise PennantError(f"unknown field {name}[{comp}]")
+        return int(p)
+
+    def get_map(self, name: str) -> np.ndarray:
+        n = ctypes.c_int64()
+        self._check(lib().pennant_get_map(self._h, name.encode(), None, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.int32)
+        self._check(lib().pennant_get_map(self._h, name.encode(), out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    def mesh_desc(self):
+        return lib().pennant_mesh_desc(self._h)
+
+    def stream(self) -> int:
+        return int(lib().pennant_stream(self._h) or 0)
+
+    # -- step-by-step (task mode) -----------------------------------------
+    def step(self, k: int) -> int:
+        return lib().pennant_step(self._h, int(k))
+
+    def end_cycle(self):
+        self._check(lib().pennant_end_cycle(self._h))
+
+    def write_xy(self, path):
+        self._check(lib().pennant_write_xy(self._h, str(path).encode()))

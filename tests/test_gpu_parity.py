@@ -1,0 +1,246 @@
+"""GPU parity: the gfx950 path against the CPU oracle, bit for bit.
+
+The FP64 plain form must reproduce the reference's serial arithmetic
+exactly (tolerance 0: every double compared by its bit pattern), so these
+tests compare raw bits, not values within a tolerance.
+
+  * step parity: the oracle's state after k warm-up cycles is loaded into the
+    device problem; then each of the 30 doCycle steps runs through the task
+    entry points on the device and through the oracle on the host, and every
+    field (49 arrays) plus the reduction scalars must match after every step;
+  * cycle parity: whole cycles in task mode and in fused mode against the
+    oracle (rect with and without far BCs, pie with triangles, two-state
+    LeBlanc);
+  * full size: Sedov 1440^2 (2.07M zones) -- a few cycles against the oracle,
+    and fused mode against task mode;
+  * the C ABI called directly on caller-owned device buffers.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import pennant_amd as pa
+from oracle_lib import FIELDS, Oracle
+from golden_decks import ref_deck
+from pennant_amd import decks as wd
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("px", "pu", "zr", "ze", "zp", "zetot", "zwrate", "zvol", "zm")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_bits_equal(name, got, want):
+    g, w = bits(got), bits(want)
+    if g.shape != w.shape or not np.array_equal(g, w):
+        gv, wv = np.asarray(got).reshape(-1), np.asarray(want).reshape(-1)
+        bad = np.nonzero(bits(gv) != bits(wv))[0]
+        raise AssertionError(f"{name}: {bad.size} of {gv.size} values differ; first at {bad[:4]}: "
+                             f"got {gv[bad[:4]]}, want {wv[bad[:4]]}")
+
+
+def load_oracle_state(pb, o, dt_current):
+    for f in FIELDS:
+        pb.set(f, o.get(f))
+    s = o.scalars()
+    pb.set_scalars([s["time"], s["dt"], s["dtrec"], s["cycle"], s["dtnew"], s["dvovmax"],
+                    0, 0, dt_current, 0])
+
+
+STEP_DECKS = {
+    "sedov45": lambda: ref_deck("sedov"),
+    "noh_pie30x100": lambda: ref_deck("noh"),
+    "nohsmall_pie2x20": lambda: ref_deck("nohsmall"),
+    "leblanc10x90": lambda: ref_deck("leblanc"),
+}
+
+
+@pytest.mark.parametrize("deck", list(STEP_DECKS))
+def test_step_parity_every_task(deck):
+    text = STEP_DECKS[deck]()
+    say(f"[step parity] {deck}")
+    o = Oracle(text)
+    o.run(3)
+    pb = pa.Problem(text)
+    dt = o.begin_cycle()
+    load_oracle_state(pb, o, dt)
+    for f in FIELDS:
+        assert_bits_equal(f"load {f}", pb.get(f), o.get(f))
+    for k in range(pa.NSTEPS):
+        assert o.step(k) == 0
+        assert pb.step(k) == 0, f"step {k}"
+        for f in FIELDS:
+            assert_bits_equal(f"step {k} field {f}", pb.get(f), o.get(f))
+        if k in (26, 27, 28):
+            so, sp = o.scalars(), pb.scalars()
+            for key in ("dtnew", "dvovmax", "dtrec"):
+                assert_bits_equal(f"step {k} {key}", sp[key], so[key])
+    pb.close()
+
+
+CYCLE_DECKS = {
+    "sedov45_farbc": (lambda: ref_deck("sedov"), 10),
+    "sedov90": (lambda: wd.sedov_90(cstop=40), 40),
+    "noh_pie30x100": (lambda: ref_deck("noh"), 10),
+    "noh_pie64": (lambda: wd.noh_pie(64, 48, cstop=25, dtinit=1e-4), 25),
+    "leblanc10x90": (lambda: ref_deck("leblanc"), 10),
+    "leblanc256x16": (lambda: wd.leblanc_rect(256, 16, cstop=30), 30),
+}
+
+
+@pytest.mark.parametrize("mode", [pa.MODE_TASKS, pa.MODE_FUSED], ids=["tasks", "fused"])
+@pytest.mark.parametrize("deck", list(CYCLE_DECKS))
+def test_cycle_parity(deck, mode):
+    make, ncyc = CYCLE_DECKS[deck]
+    text = make()
+    say(f"[cycle parity] {deck} mode={mode} cycles={ncyc}")
+    o = Oracle(text)
+    assert o.run(1000) == ncyc
+    pb = pa.Problem(text)
+    assert pb.run(1000, mode) == ncyc
+    for f in STATE:
+        assert_bits_equal(f, pb.get(f), o.get(f))
+    if mode == pa.MODE_TASKS:
+        for f in FIELDS:
+            assert_bits_equal(f, pb.get(f), o.get(f))
+    so, sp = o.scalars(), pb.scalars()
+    assert sp["cycle"] == so["cycle"]
+    for key in ("time", "dt", "dtrec", "dtnew", "dvovmax"):
+        assert_bits_equal(key, sp[key], so[key])
+    assert sp["done"] == 1
+    pb.close()
+
+
+def test_sedov_xy_identical_to_oracle(tmp_path):
+    text = ref_deck("sedov")
+    o = Oracle(text)
+    o.run(1000)
+    o.write_xy(tmp_path / "ora.xy")
+    with pa.Problem(text) as pb:
+        pb.run(1000, pa.MODE_FUSED)
+        pb.write_xy(tmp_path / "gpu.xy")
+    assert (tmp_path / "gpu.xy").read_text() == (tmp_path / "ora.xy").read_text()
+
+
+def test_run_to_tstop_in_chunks():
+    # cycles queued past the stop condition do nothing (device predicate)
+    text = wd.sedov_rect(40, tstop=3e-4)
+    o = Oracle(text)
+    n = o.run(100000)
+    with pa.Problem(text) as pb:
+        done = 0
+        while not pb.scalars()["done"]:
+            pb.launch(16, pa.MODE_FUSED)
+            pb.sync()
+            done += 16
+        assert pb.scalars()["cycle"] == n
+        assert_bits_equal("time", pb.scalars()["time"], o.scalars()["time"])
+        for f in STATE:
+            assert_bits_equal(f, pb.get(f), o.get(f))
+
+
+def test_bad_sides_reported_like_reference():
+    # a first dt far above the Courant limit inverts zones on cycle 0
+    text = wd.noh_pie(20, 16, cstop=5, dtinit=0.5)
+    o = Oracle(text)
+    with pytest.raises(Exception, match="negative side volumes on cycle 0"):
+        o.run(5)
+    for mode in (pa.MODE_TASKS, pa.MODE_FUSED):
+        with pa.Problem(text) as pb:
+            with pytest.raises(pa.PennantError, match="negative side volumes on cycle 0"):
+                pb.run(5, mode)
+
+
+def test_full_size_sedov1440_against_oracle():
+    text = wd.sedov_1440(cstop=3)
+    say("[full size] sedov 1440^2: oracle 3 cycles")
+    o = Oracle(text)
+    assert o.run(3) == 3
+    say("[full size] sedov 1440^2: fused 3 cycles")
+    with pa.Problem(text) as pb:
+        assert (pb.numz, pb.nums, pb.nump) == (2073600, 8294400, 2076481)
+        assert pb.run(3, pa.MODE_FUSED) == 3
+        for f in STATE:
+            assert_bits_equal(f, pb.get(f), o.get(f))
+        for key in ("time", "dtrec"):
+            assert_bits_equal(key, pb.scalars()[key], o.scalars()[key])
+
+
+def test_full_size_fused_equals_tasks_and_conserves_mass():
+    text = wd.sedov_1440(cstop=12)
+    say("[full size] sedov 1440^2: tasks vs fused, 12 cycles")
+    with pa.Problem(text) as a, pa.Problem(text) as b:
+        zm0 = a.get("zm")
+        a.run(12, pa.MODE_TASKS)
+        b.run(12, pa.MODE_FUSED)
+        for f in STATE:
+            assert_bits_equal(f, b.get(f), a.get(f))
+        assert np.array_equal(b.get("zm"), zm0)
+        np.testing.assert_allclose(b.get("zr") * b.get("zvol"), zm0, rtol=1e-13)
+        # the blast is symmetric about the diagonal of the square mesh
+        n = 1440
+        zr = b.get("zr").reshape(n, n)
+        np.testing.assert_allclose(zr, zr.T, rtol=1e-9)
+
+
+def test_noh_pie_1024_fused_equals_tasks():
+    text = wd.noh_pie_1024(cstop=6)
+    say("[full size] noh pie 1024^2: tasks vs fused, 6 cycles")
+    with pa.Problem(text) as a, pa.Problem(text) as b:
+        assert (a.numz, a.nums, a.nump) == (1048576, 4193280, 1049601)
+        a.run(6, pa.MODE_TASKS)
+        b.run(6, pa.MODE_FUSED)
+        for f in STATE:
+            assert_bits_equal(f, b.get(f), a.get(f))
+
+
+def test_task_abi_on_caller_owned_buffers():
+    """Calls TID_CALCCTRS / TID_CALCEDGELEN / TID_CALCDTNEW entry points on
+    device buffers the caller allocated (torch), as a Legion task wrapper
+    would with acc.ptr(rect)."""
+    import torch
+    text = wd.noh_pie(12, 9, cstop=1)
+    o = Oracle(text)
+    L = pa.lib()
+    with pa.Problem(text) as pb:
+        m = pb.mesh_desc()
+        dev = torch.device("cuda:0")
+        px = torch.from_numpy(o.get("px")).to(dev)
+        pxx, pxy = px[:, 0].contiguous(), px[:, 1].contiguous()
+        ex = torch.zeros((2, pb.nums), dtype=torch.float64, device=dev)
+        zx = torch.zeros((2, pb.numz), dtype=torch.float64, device=dev)
+        el = torch.zeros(pb.nums, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        assert L.pennant_hip_calc_ctrs(m, pa.vec(pxx.data_ptr(), pxy.data_ptr()),
+                                       pa.vec(ex[0].data_ptr(), ex[1].data_ptr()),
+                                       pa.vec(zx[0].data_ptr(), zx[1].data_ptr()), None) == 0
+        assert L.pennant_hip_calc_edge_len(m, pa.vec(pxx.data_ptr(), pxy.data_ptr()),
+                                           el.data_ptr(), None) == 0
+        torch.cuda.synchronize()
+        assert_bits_equal("zx", zx.T.cpu().numpy(), o.get("zx"))
+        assert_bits_equal("ex", ex.T.cpu().numpy(), o.get("ex"))
+        p1, p2 = o.map("mapsp1"), o.map("mapsp2")
+        d = o.get("px")[p2] - o.get("px")[p1]
+        np.testing.assert_allclose(el.cpu().numpy(), np.hypot(d[:, 0], d[:, 1]), rtol=1e-15)
+        # reduction entry point with a caller workspace
+        zdl = torch.rand(pb.numz, dtype=torch.float64, device=dev) + 0.5
+        zdu = torch.rand(pb.numz, dtype=torch.float64, device=dev)
+        zss = torch.rand(pb.numz, dtype=torch.float64, device=dev)
+        ws = torch.empty(L.pennant_hip_reduce_workspace_bytes() // 8, dtype=torch.float64, device=dev)
+        out = torch.zeros(1, dtype=torch.float64, device=dev)
+        assert L.pennant_hip_calc_dt_new(m, zdl.data_ptr(), zdu.data_ptr(), zss.data_ptr(), 0.6,
+                                         out.data_ptr(), ws.data_ptr(), None) == 0
+        torch.cuda.synchronize()
+        a, u, s = zdl.cpu().numpy(), zdu.cpu().numpy(), zss.cpu().numpy()
+        want = np.min(a * 0.6 / np.maximum(u, np.maximum(s, 1e-99)))
+        assert out.item() == want

@@ -1,0 +1,95 @@
+"""Pins the CPU oracle against the reference's only known-answer test:
+test/sedov/sedov.{xy,geo} (45x45 rect Sedov after 10 cycles, copied to
+tests/golden/sedov by tests/golden/make_fixtures.py).
+
+What the golden holds and how it is compared:
+  * sedov.geo line 2: the simulated time after 10 cycles, 9 digits.  This
+    pins the whole dt chain (calcGlobalDt, calcDtNew, calcDvol, calcDt).
+  * sedov.xy: zr, ze, zp per zone, 9 significant digits.  In zone rows 0-14
+    (the blast and its surroundings) every value with magnitude >= 1e-30
+    must print identically; values below that are round-off noise (the
+    quiescent far field holds energies of 1e-45..1e-93) and must agree to
+    1e-30 absolute.
+  * sedov.geo coordinates (6 significant digits): every point in point rows
+    0-17 must print identically.
+Rows 15 and above of the golden carry spurious bands (energies up to 2e-5
+and density dips of 0.2% at rows 17-18 and 26-28, point displacements of
+2e-5 at point row 18) in an e=0 region that round-off level elsewhere in the
+same file shows to be quiescent; they are not reproducible from the deck
+and are excluded from the comparison.
+"""
+import numpy as np
+
+from oracle_lib import Oracle
+from golden_decks import HERE, ref_deck
+
+GOLD = HERE / "golden" / "sedov"
+NZX = 45
+
+
+def _read_xy(path):
+    vals, texts = [], []
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        t = line.split()[1]
+        texts.append(t)
+        vals.append(float(t))
+    return np.array(vals), np.array(texts)
+
+
+def _run_sedov(tmp_path):
+    o = Oracle(ref_deck("sedov"))
+    assert o.run(1000) == 10
+    assert o.done()
+    out = tmp_path / "sedov.xy"
+    o.write_xy(out)
+    return o, out
+
+
+def test_sedov_time_after_10_cycles(tmp_path):
+    o, _ = _run_sedov(tmp_path)
+    sc = o.scalars()
+    geo = open(GOLD / "sedov.geo").read().split("\n")
+    assert geo[0].split() == ["cycle", "=", "10"]
+    assert geo[1] == "t = %15.8e" % sc["time"]
+    assert sc["cycle"] == 10
+
+
+def test_sedov_zone_fields_match_golden(tmp_path):
+    _, out = _run_sedov(tmp_path)
+    a, ta = _read_xy(out)
+    b, tb = _read_xy(GOLD / "sedov.xy")
+    n = NZX * NZX
+    assert a.size == b.size == 3 * n
+    j = np.arange(n) // NZX
+    for k, name in enumerate(("zr", "ze", "zp")):
+        A, B = a[k * n:(k + 1) * n], b[k * n:(k + 1) * n]
+        TA, TB = ta[k * n:(k + 1) * n], tb[k * n:(k + 1) * n]
+        near = j < 15
+        sig = near & (np.abs(B) >= 1e-30)
+        bad = np.nonzero(sig & (TA != TB))[0]
+        assert bad.size == 0, f"{name}: zones {bad[:5]} differ: {TA[bad[:5]]} vs {TB[bad[:5]]}"
+        noise = near & ~sig
+        if noise.any():
+            assert np.max(np.abs(A[noise] - B[noise])) <= 1e-30, name
+        # the blast region itself is covered
+        assert sig.sum() >= (600 if name == "zr" else 30)
+
+
+def test_sedov_point_coordinates_match_golden():
+    o = Oracle(ref_deck("sedov"))
+    o.run(1000)
+    px = o.get("px")
+    lines = open(GOLD / "sedov.geo").read().split("\n")
+    npts = int(lines[8])
+    assert npts == px.shape[0] == (NZX + 1) ** 2
+    gx = lines[9:9 + npts]
+    gy = lines[9 + npts:9 + 2 * npts]
+    fx = ["%12.5e" % v for v in px[:, 0]]
+    fy = ["%12.5e" % v for v in px[:, 1]]
+    rows = np.arange(npts) // (NZX + 1)
+    for p in np.nonzero(rows < 18)[0]:
+        assert fx[p] == gx[p] and fy[p] == gy[p], (p, fx[p], gx[p], fy[p], gy[p])
+    exact = sum(1 for p in range(npts) if fx[p] == gx[p] and fy[p] == gy[p])
+    assert exact >= 2080

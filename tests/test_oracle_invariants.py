@@ -1,0 +1,283 @@
+"""Pins the parts of the CPU oracle the sedov golden leaves open (pie meshes,
+triangles, EOS, TTS, QCS, BCs, dt logic) against closed forms, invariants and
+brute-force cases -- each chosen so that a plausible mistake (dropped term,
+wrong sign or index, swapped operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle, calc_dt_hydro, calc_global_dt
+from golden_decks import ref_deck
+from pennant_amd import decks as wd
+
+GAMMA = 5.0 / 3.0
+
+
+def rect_deck(nx, ny, lx, ly, extra=""):
+    return (f"cstop 5\nmeshtype rect\nmeshparams {nx} {ny} {lx} {ly}\nrinit 2.0\neinit 0.1\n"
+            f"ssmin 0.1\nq1 0.1\nq2 1.0\ndtinit 1e-4\n{extra}")
+
+
+def pie_deck(nth, nr, extra=""):
+    return (f"cstop 5\nmeshtype pie\nmeshparams {nth} {nr}\nrinit 1.0\neinit 0.0\n"
+            f"ssmin 0.1\nq1 1.0\nq2 1.0\ndtinit 1e-4\n{extra}")
+
+
+def run_steps(o, upto):
+    o.begin_cycle()
+    for k in range(upto + 1):
+        assert o.step(k) == 0
+    return o
+
+
+def polygon_zones(o):
+    sb = 0
+    zsides = []
+    mapsz = o.map("mapsz")
+    for z in range(o.numz):
+        se = sb
+        while se < o.nums and mapsz[se] == z:
+            se += 1
+        zsides.append((sb, se))
+        sb = se
+    return zsides
+
+
+# --- geometry ----------------------------------------------------------------
+
+def test_rect_geometry_closed_form():
+    nx, ny, lx, ly = 6, 4, 1.5, 1.0
+    o = Oracle(rect_deck(nx, ny, lx, ly))
+    dx, dy = lx / nx, ly / ny
+    assert (o.nump, o.numz, o.nums) == ((nx + 1) * (ny + 1), nx * ny, 4 * nx * ny)
+    zi, zj = np.arange(o.numz) % nx, np.arange(o.numz) // nx
+    x1, x2 = zi * dx, (zi + 1) * dx
+    np.testing.assert_allclose(o.get("zx")[:, 0], (zi + 0.5) * dx, rtol=1e-15)
+    np.testing.assert_allclose(o.get("zx")[:, 1], (zj + 0.5) * dy, rtol=1e-15)
+    np.testing.assert_allclose(o.get("zarea"), dx * dy, rtol=1e-14)
+    np.testing.assert_allclose(o.get("zvol"), dy * (x2 ** 2 - x1 ** 2) / 2, rtol=1e-14)
+    np.testing.assert_allclose(o.get("sarea"), dx * dy / 4, rtol=1e-14)
+    np.testing.assert_allclose(o.get("smf"), 0.25, rtol=1e-14)
+    # side midpoints
+    px = o.get("px")
+    ex = o.get("ex")
+    np.testing.assert_allclose(ex, 0.5 * (px[o.map("mapsp1")] + px[o.map("mapsp2")]), rtol=0, atol=0)
+    run_steps(o, 6)
+    el = o.get("elen")
+    side = np.arange(o.nums) % 4
+    np.testing.assert_allclose(el, np.where(side % 2 == 0, dx, dy), rtol=1e-15)
+    # characteristic length: min over sides of 4*(dx*dy/4)/elen
+    np.testing.assert_allclose(o.get("zdl"), dx * dy / max(dx, dy), rtol=1e-14)
+
+
+def test_pie_geometry_shoelace_and_counts():
+    nth, nr = 5, 4
+    o = Oracle(pie_deck(nth, nr))
+    assert o.nums == 4 * nth * (nr - 1) + 3 * nth
+    assert o.nump == (nth + 1) * nr + 1
+    znump = o.map("znump")
+    assert list(znump[:nth]) == [3] * nth and set(znump[nth:]) == {4}
+    px = o.get("px")
+    r = np.hypot(px[:, 0], px[:, 1])
+    np.testing.assert_allclose(np.sort(np.unique(np.round(r, 12))), np.arange(nr + 1) / nr, atol=1e-12)
+    mapsp1 = o.map("mapsp1")
+    zarea, zvol, zx = o.get("zarea"), o.get("zvol"), o.get("zx")
+    for z, (sb, se) in enumerate(polygon_zones(o)):
+        pts = px[mapsp1[sb:se]]
+        xs, ys = pts[:, 0], pts[:, 1]
+        xn, yn = np.roll(xs, -1), np.roll(ys, -1)
+        crs = xs * yn - xn * ys
+        area = 0.5 * crs.sum()
+        moment = ((xs + xn) * crs).sum() / 6.0    # integral of x over the polygon
+        assert area > 0
+        assert zarea[z] == pytest.approx(area, rel=1e-13)
+        assert zvol[z] == pytest.approx(moment, rel=1e-12)
+        np.testing.assert_allclose(zx[z], pts.mean(axis=0), rtol=1e-14, atol=1e-16)
+
+
+def test_pie_char_length_uses_triangle_factor():
+    o = Oracle(pie_deck(6, 3))
+    run_steps(o, 6)
+    sa, el, zdl = o.get("sareap"), o.get("elen"), o.get("zdl")
+    for z, (sb, se) in enumerate(polygon_zones(o)):
+        fac = 3.0 if se - sb == 3 else 4.0
+        assert zdl[z] == min(fac * sa[s] / el[s] for s in range(sb, se))
+
+
+# --- EOS, forces --------------------------------------------------------------
+
+def test_eos_closed_form_uniform_state():
+    o = Oracle(rect_deck(5, 5, 1.0, 1.0))
+    run_steps(o, 9)
+    e, rho = 0.1, 2.0
+    np.testing.assert_allclose(o.get("zp"), (GAMMA - 1) * rho * e, rtol=1e-15)
+    np.testing.assert_allclose(o.get("zss"), math.sqrt(GAMMA * (GAMMA - 1) * e), rtol=1e-15)
+
+
+def test_uniform_pressure_gives_zero_interior_force():
+    nx = 6
+    o = Oracle(rect_deck(nx, nx, 1.0, 1.0))
+    run_steps(o, 16)
+    sfp = o.get("sfp")
+    assert np.all(o.get("sfq") == 0.0)           # no motion: no viscosity
+    assert np.max(np.abs(o.get("sft"))) <= 1e-15 * np.max(np.abs(sfp))
+    pf = o.get("pf")
+    i, j = np.arange(o.nump) % (nx + 1), np.arange(o.nump) // (nx + 1)
+    interior = (i > 0) & (i < nx) & (j > 0) & (j < nx)
+    assert np.max(np.abs(pf[interior])) <= 1e-14 * np.max(np.abs(sfp))
+    assert np.max(np.abs(pf[~interior])) > 1e-3 * np.max(np.abs(sfp))
+
+
+def test_tts_force_sign_opposes_compression():
+    # compress one zone's side: its side density exceeds the zone density,
+    # so the TTS force pushes outward along that side's surface normal
+    o = Oracle(rect_deck(3, 3, 1.0, 1.0))
+    px = o.field("px")
+    px[5] += np.array([0.05, 0.02])        # interior point moves into zone 4
+    o2 = Oracle(rect_deck(3, 3, 1.0, 1.0))
+    del o2
+    run_steps(o, 11)
+    sft, ss = o.get("sft"), o.get("ssurfp")
+    sarea, smf, zarea = o.get("sareap"), o.get("smf"), o.get("zareap")
+    mapsz = o.map("mapsz")
+    srho_minus_r = smf * zarea[mapsz] / sarea - 1.0
+    proj = np.einsum("ij,ij->i", sft, ss)
+    mask = np.abs(srho_minus_r) > 1e-6
+    assert mask.any()
+    assert np.all(np.sign(proj[mask]) == -np.sign(srho_minus_r[mask]))
+
+
+# --- QCS ----------------------------------------------------------------------
+
+def test_qcs_rigid_translation_has_no_viscosity():
+    o = Oracle(rect_deck(5, 4, 1.0, 1.0))
+    o.field("pu")[:] = np.array([0.3, -0.2])
+    run_steps(o, 15)
+    assert np.max(np.abs(o.get("cdiv"))) <= 1e-13
+    assert np.max(np.abs(o.get("sfq"))) <= 1e-12 * np.max(np.abs(o.get("sfp")))
+    zdu = o.get("zdu")
+    np.testing.assert_allclose(zdu, 0.1 * o.get("zss"), rtol=1e-12)
+
+
+@pytest.mark.parametrize("mesh", ["rect", "pie"])
+def test_qcs_corner_divergence_of_linear_field(mesh):
+    a = 0.7
+    o = Oracle(rect_deck(5, 4, 1.0, 1.0) if mesh == "rect" else pie_deck(6, 5))
+    o.field("pu")[:] = -a * o.get("px")
+    dt = o.begin_cycle()
+    for k in range(14):
+        assert o.step(k) == 0
+    # Green's theorem on the corner quadrilateral is exact for linear fields.
+    # The geometry is the half-step one, pxp = px*(1 - a*dt/2), on which the
+    # field u = -a*px reads u = -a*pxp/(1 - a*dt/2): divergence -2a/(1 - a*dt/2)
+    np.testing.assert_allclose(o.get("cdiv"), -2 * a / (1 - a * 0.5 * dt), rtol=1e-12)
+    assert np.all(o.get("cevol") > 0) and np.all(o.get("cdu") > 0)
+    assert np.all(o.get("crmu") > 0)
+
+
+def test_qcs_expansion_switches_viscosity_off():
+    o = Oracle(rect_deck(4, 4, 1.0, 1.0))
+    o.field("pu")[:] = 0.5 * o.get("px")
+    run_steps(o, 14)
+    assert np.all(o.get("cdiv") > 0)
+    assert np.all(o.get("crmu") == 0) and np.all(o.get("sfq") == 0)
+
+
+# --- point sums, BCs ------------------------------------------------------------
+
+def test_corner_mass_and_force_sums():
+    o = Oracle(ref_deck("sedov"))
+    o.run(3)
+    run_steps(o, 16)
+    zrp, zareap = o.get("zrp"), o.get("zareap")
+    assert o.get("pmaswt").sum() == pytest.approx((zrp * zareap).sum(), rel=1e-13)
+    pf = o.get("pf")
+    assert np.all(np.abs(pf.sum(axis=0)) <= 1e-12 * np.abs(pf).sum(axis=0))
+
+
+def test_fixed_bc_projection():
+    o = Oracle(ref_deck("sedov"))
+    o.run(2)
+    run_steps(o, 17)
+    pu0, pf = o.get("pu0"), o.get("pf")
+    bcs = o.bcs()
+    assert len(bcs) == 4
+    for pts, v in bcs:
+        comp = 0 if v == (1.0, 0.0) else 1
+        assert pts.size == 46
+        assert np.all(pu0[pts, comp] == 0.0) and np.all(pf[pts, comp] == 0.0)
+
+
+def test_mass_is_lagrangian_constant():
+    o = Oracle(ref_deck("sedov"))
+    zm0 = o.get("zm").copy()
+    o.run(10)
+    np.testing.assert_array_equal(o.get("zm"), zm0)
+    np.testing.assert_allclose(o.get("zr") * o.get("zvol"), zm0, rtol=1e-14)
+
+
+# --- dt logic -------------------------------------------------------------------
+
+def test_calc_dt_hydro_cases():
+    # dtrec = min(1e99, dtnew, dtlast*cflv/dvovmax)
+    assert calc_dt_hydro(0.1, 1e-3, 5e-4, 0.5) == 2e-4
+    assert calc_dt_hydro(0.1, 1e-3, 1e-4, 0.5) == 1e-4
+    assert calc_dt_hydro(0.1, 1e-3, 2e99, 1e-200) == 1e99
+    assert calc_dt_hydro(0.1, 1e-3, 2e99, 1e-99) == pytest.approx(1e95)
+    assert calc_dt_hydro(0.1, 1e-3, 3e-4, 1e-99) == 3e-4
+
+
+def test_calc_global_dt_cases():
+    # cycle 0: min(dtmax, dtinit, tstop - time); dtlast and dtrec unused
+    assert calc_global_dt(1e99, 1e-4, 1.2, 1.0, 0, 0.0, 7.0, 1e-9) == 1e-4
+    assert calc_global_dt(5e-5, 1e-4, 1.2, 1.0, 0, 0.0, 0.0, 0.0) == 5e-5
+    # later cycles: min(dtmax, dtfac*dtlast, tstop - time, dtrec)
+    assert calc_global_dt(1e99, 1e-4, 1.2, 1.0, 3, 0.5, 1e-3, 5e-3) == pytest.approx(1.2e-3)
+    assert calc_global_dt(1e99, 1e-4, 1.2, 1.0, 3, 0.5, 1e-3, 4e-4) == 4e-4
+    assert calc_global_dt(1e99, 1e-4, 1.2, 0.5003, 3, 0.5, 1e-3, 5e-3) == pytest.approx(3e-4)
+    assert calc_global_dt(2e-4, 1e-4, 1.2, 1.0, 3, 0.5, 1e-3, 5e-3) == 2e-4
+
+
+def test_run_stops_at_tstop_and_cstop():
+    deck = rect_deck(4, 4, 1.0, 1.0).replace("cstop 5", "cstop 1000\ntstop 2.5e-4")
+    o = Oracle(deck)
+    n = o.run(10000)
+    sc = o.scalars()
+    assert sc["time"] == pytest.approx(2.5e-4, rel=1e-12) and n == sc["cycle"] < 1000
+    o2 = Oracle(rect_deck(4, 4, 1.0, 1.0))
+    assert o2.run(10000) == 5
+
+
+# --- OpenMP variant (CPU timing baseline) ---------------------------------------
+
+def test_omp_variant_matches_serial():
+    a = Oracle(ref_deck("sedov"))
+    b = Oracle(ref_deck("sedov"), omp=True)
+    a.run(10)
+    b.run(10)
+    for f in ("zr", "zp"):
+        np.testing.assert_allclose(b.get(f), a.get(f), rtol=1e-10, atol=1e-25)
+    assert b.scalars()["time"] == pytest.approx(a.scalars()["time"], rel=1e-12)
+
+
+# --- coarse physics -------------------------------------------------------------
+
+def test_noh_pie_matches_spherical_noh():
+    # spherical Noh, gamma = 5/3, unit inflow: the shock sits at r = t/3, the
+    # unshocked gas has density (1 + t/r)^2, the shocked gas 64 (approached
+    # from below at this resolution; wall heating lowers the centre)
+    t = 0.6
+    o = Oracle(wd.noh_pie(40, 12, q1=0.1, tstop=t, dtinit=1e-4))
+    o.run(100000)
+    assert o.scalars()["time"] == pytest.approx(t)
+    zx = o.get("zx")
+    r = np.hypot(zx[:, 0], zx[:, 1])
+    zr = o.get("zr")
+    # clear of the shock's smearing and of the unconfined outer edge (r = 0.4)
+    pre = (r > 0.28) & (r < 0.35)
+    assert pre.sum() > 30
+    np.testing.assert_allclose(zr[pre], (1 + t / r[pre]) ** 2, rtol=0.03)
+    post = (r > 0.1) & (r < 0.19)
+    assert np.all((zr[post] > 38) & (zr[post] < 70))
