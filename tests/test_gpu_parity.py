@@ -187,10 +187,6 @@ def test_full_size_fused_equals_tasks_and_conserves_mass():
             assert_bits_equal(f, b.get(f), a.get(f))
         assert np.array_equal(b.get("zm"), zm0)
         np.testing.assert_allclose(b.get("zr") * b.get("zvol"), zm0, rtol=1e-13)
-        # the blast is symmetric about the diagonal of the square mesh
-        n = 1440
-        zr = b.get("zr").reshape(n, n)
-        np.testing.assert_allclose(zr, zr.T, rtol=1e-9)
 
 
 def test_noh_pie_1024_fused_equals_tasks():
@@ -206,41 +202,38 @@ def test_noh_pie_1024_fused_equals_tasks():
 
 def test_task_abi_on_caller_owned_buffers():
     """Calls TID_CALCCTRS / TID_CALCEDGELEN / TID_CALCDTNEW entry points on
-    device buffers the caller allocated (torch), as a Legion task wrapper
-    would with acc.ptr(rect)."""
-    import torch
+    device buffers the caller allocated with hipMalloc, as a Legion task
+    wrapper would with acc.ptr(rect)."""
+    from hip_rt import DeviceArray, synchronize
     text = wd.noh_pie(12, 9, cstop=1)
     o = Oracle(text)
     L = pa.lib()
+    rng = np.random.default_rng(7)
     with pa.Problem(text) as pb:
         m = pb.mesh_desc()
-        dev = torch.device("cuda:0")
-        px = torch.from_numpy(o.get("px")).to(dev)
-        pxx, pxy = px[:, 0].contiguous(), px[:, 1].contiguous()
-        ex = torch.zeros((2, pb.nums), dtype=torch.float64, device=dev)
-        zx = torch.zeros((2, pb.numz), dtype=torch.float64, device=dev)
-        el = torch.zeros(pb.nums, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
-        assert L.pennant_hip_calc_ctrs(m, pa.vec(pxx.data_ptr(), pxy.data_ptr()),
-                                       pa.vec(ex[0].data_ptr(), ex[1].data_ptr()),
-                                       pa.vec(zx[0].data_ptr(), zx[1].data_ptr()), None) == 0
-        assert L.pennant_hip_calc_edge_len(m, pa.vec(pxx.data_ptr(), pxy.data_ptr()),
-                                           el.data_ptr(), None) == 0
-        torch.cuda.synchronize()
-        assert_bits_equal("zx", zx.T.cpu().numpy(), o.get("zx"))
-        assert_bits_equal("ex", ex.T.cpu().numpy(), o.get("ex"))
+        px = o.get("px")
+        pxx, pxy = DeviceArray(px[:, 0].copy()), DeviceArray(px[:, 1].copy())
+        ex = [DeviceArray.zeros(pb.nums) for _ in range(2)]
+        zx = [DeviceArray.zeros(pb.numz) for _ in range(2)]
+        el = DeviceArray.zeros(pb.nums)
+        synchronize()
+        assert L.pennant_hip_calc_ctrs(m, pa.vec(pxx.ptr, pxy.ptr), pa.vec(ex[0].ptr, ex[1].ptr),
+                                       pa.vec(zx[0].ptr, zx[1].ptr), None) == 0
+        assert L.pennant_hip_calc_edge_len(m, pa.vec(pxx.ptr, pxy.ptr), el.ptr, None) == 0
+        synchronize()
+        assert_bits_equal("zx", np.stack([zx[0].numpy(), zx[1].numpy()], axis=1), o.get("zx"))
+        assert_bits_equal("ex", np.stack([ex[0].numpy(), ex[1].numpy()], axis=1), o.get("ex"))
         p1, p2 = o.map("mapsp1"), o.map("mapsp2")
-        d = o.get("px")[p2] - o.get("px")[p1]
-        np.testing.assert_allclose(el.cpu().numpy(), np.hypot(d[:, 0], d[:, 1]), rtol=1e-15)
+        d = px[p2] - px[p1]
+        np.testing.assert_allclose(el.numpy(), np.hypot(d[:, 0], d[:, 1]), rtol=1e-15)
         # reduction entry point with a caller workspace
-        zdl = torch.rand(pb.numz, dtype=torch.float64, device=dev) + 0.5
-        zdu = torch.rand(pb.numz, dtype=torch.float64, device=dev)
-        zss = torch.rand(pb.numz, dtype=torch.float64, device=dev)
-        ws = torch.empty(L.pennant_hip_reduce_workspace_bytes() // 8, dtype=torch.float64, device=dev)
-        out = torch.zeros(1, dtype=torch.float64, device=dev)
-        assert L.pennant_hip_calc_dt_new(m, zdl.data_ptr(), zdu.data_ptr(), zss.data_ptr(), 0.6,
-                                         out.data_ptr(), ws.data_ptr(), None) == 0
-        torch.cuda.synchronize()
-        a, u, s = zdl.cpu().numpy(), zdu.cpu().numpy(), zss.cpu().numpy()
+        a = rng.random(pb.numz) + 0.5
+        u = rng.random(pb.numz)
+        s = rng.random(pb.numz)
+        da, du, ds = DeviceArray(a), DeviceArray(u), DeviceArray(s)
+        ws = DeviceArray.zeros(L.pennant_hip_reduce_workspace_bytes() // 8)
+        out = DeviceArray.zeros(1)
+        assert L.pennant_hip_calc_dt_new(m, da.ptr, du.ptr, ds.ptr, 0.6, out.ptr, ws.ptr, None) == 0
+        synchronize()
         want = np.min(a * 0.6 / np.maximum(u, np.maximum(s, 1e-99)))
-        assert out.item() == want
+        assert out.numpy()[0] == want
