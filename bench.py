@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sedov1440")
     ap.add_argument("--mode", default="fused", choices=["fused", "tasks"])
+    ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
+    ap.add_argument("--max-blocks", type=int, default=None, help="grid cap of the fused zone kernels")
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of CPU time per oracle variant (0 disables)")
     args = ap.parse_args()
@@ -107,6 +109,10 @@ def main():
     mode = pa.MODE_FUSED if args.mode == "fused" else pa.MODE_TASKS
     t0 = time.perf_counter()
     pb = pa.Problem(text)
+    if args.k1 is not None:
+        pb.set_tuning("k1_layout", args.k1)
+    if args.max_blocks is not None:
+        pb.set_tuning("max_blocks", args.max_blocks)
     log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
     if args.warmup > 0:
         pb.run(args.warmup, mode)

@@ -341,6 +341,13 @@ int pennant_end_cycle(pennant_problem* pb);
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms);
 
+/* Launch-shape choices of the fused cycle; results do not depend on them.
+ *   "k1_layout"  0: one zone per thread (default), 1: one corner per lane
+ *   "max_blocks" grid cap of the grid-stride zone kernels, 1..4096 (2048)
+ * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
+int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
+int pennant_get_tuning(pennant_problem* pb, const char* key);
+
 /* Writes <path> in the reference's .xy format (src/WriteXY.cc:28-53). */
 int pennant_write_xy(pennant_problem* pb, const char* path);
 

@@ -472,12 +472,14 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
                         double* phase_ms) {
     if (ncycles < 1) return PENNANT_EINVAL;
     const bool phases = (mode == PENNANT_MODE_FUSED && phase_ms);
-    std::vector<hipEvent_t> ev(phases ? 5 * (size_t)ncycles + 2 : 2);
+    // per cycle: events before K1, K2, K3, K4 and after K4
+    const size_t E = 5;
+    std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
     for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
     int rc = (int)hipEventRecord(ev[0], pb->stream);
     if (!rc) {
         if (phases) {
-            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + 5 * (size_t)k]);
+            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k]);
         } else {
             rc = pennant_launch_cycles(pb, ncycles, mode);
         }
@@ -491,13 +493,33 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
         for (int j = 0; j < 4; ++j) phase_ms[j] = 0.;
         for (int k = 0; k < ncycles && !rc; ++k)
             for (int j = 0; j < 4 && !rc; ++j) {
-                rc = (int)hipEventElapsedTime(&ms, ev[2 + 5 * (size_t)k + j], ev[2 + 5 * (size_t)k + j + 1]);
+                rc = (int)hipEventElapsedTime(&ms, ev[2 + E * (size_t)k + j], ev[2 + E * (size_t)k + j + 1]);
                 phase_ms[j] += ms;
             }
     }
     for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     return pennant_sync(pb);
+}
+
+int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
+    if (!strcmp(key, "k1_layout")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        pb->tune.k1_layout = value;
+        return 0;
+    }
+    if (!strcmp(key, "max_blocks")) {
+        if (value < 1 || value > PN_MAX_GRID) return PENNANT_EINVAL;
+        pb->tune.max_blocks = value;
+        return 0;
+    }
+    return PENNANT_EINVAL;
+}
+
+int pennant_get_tuning(pennant_problem* pb, const char* key) {
+    if (!strcmp(key, "k1_layout")) return pb->tune.k1_layout;
+    if (!strcmp(key, "max_blocks")) return pb->tune.max_blocks;
+    return PENNANT_EINVAL;
 }
 
 int pennant_sync(pennant_problem* pb) {

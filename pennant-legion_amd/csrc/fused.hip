@@ -6,20 +6,20 @@
 // its own sides and corners, so the whole predictor half of doCycle is one
 // zone-local computation; the only cross-zone couplings are the two point
 // sums (corner mass, corner force) and the corrector's point positions.  The
-// fused cycle is therefore three passes plus a one-block finalize:
+// fused cycle is therefore three streaming passes and a one-block finalize:
 //
-//   K1 zone pass   advPosHalf (per zone corner), calcCtrs, calcVols,
-//                  calcSurfVecs, calcEdgeLen, calcCharLen, calcRho,
-//                  calcCrnrMass (per-corner mass), calcStateHalf,
-//                  calcForcePGas, calcForceTTS, setCornerDiv, setQCnForce,
-//                  setForceQCS, setVelDiff, the corner force of
-//                  sumCrnrForce, calcDtNew's per-zone dt
+//   K1 zone pass   advPosHalf (per corner), calcCtrs, calcVols, calcSurfVecs,
+//                  calcEdgeLen, calcCharLen, calcRho, calcCrnrMass (per-corner
+//                  mass), calcStateHalf, calcForcePGas, calcForceTTS,
+//                  setCornerDiv, setQCnForce, setForceQCS, setVelDiff, the
+//                  corner force of sumCrnrForce, calcDtNew's per-zone dt
 //   K2 point pass  corner-mass and corner-force gathers (ascending side
 //                  order), applyFixedBC, calcAccel, advPosFull
 //   K3 zone pass   calcCtrs, calcVols (corrector), calcWork, calcWorkRate,
 //                  calcEnergy, calcRho, calcDvol's per-zone ratio
-//   K4 finalize    dt min, dvol max and bad-side sums over block partials,
-//                  calcDt, checkBadSides, time/cycle update, calcGlobalDt
+//   K4 finalize    dt minimum, dvol maximum and bad-side counts over the
+//                  block partials of K1 and K3, calcDt, checkBadSides, the
+//                  time/cycle update and calcGlobalDt for the next cycle
 //
 // Every expression is evaluated exactly as in the task kernels (and the
 // reference's serial bodies), so the end-of-cycle state is bit-identical to
@@ -28,14 +28,26 @@
 // (src/Driver.cc:140-144): launches after the stop condition do nothing, so
 // the host can queue many cycles without reading anything back.
 //
-// Zones are processed one per thread, templated on the zone's side count NS
-// so that all per-corner values live in registers; the host launches one
-// kernel per run of consecutive zones with equal side count (rect: one run of
-// quads; pie: a run of triangles, then quads).
+// K1 has two layouts, chosen per problem (pennant_set_tuning):
+//   zone-per-thread  one thread holds a whole zone in registers: no
+//                    redundant zone-level arithmetic (EOS, density, dt);
+//   corner-per-lane  a zone with NS sides occupies NS adjacent lanes of a
+//                    64-lane wave (16 quads or 21 triangles per wave); side
+//                    arrays are read and written fully coalesced and zone
+//                    sums read the group's values with cross-lane shuffles in
+//                    ascending corner order, so every lane of the group forms
+//                    the same sum in the reference's order.
+// K1 and K3 are grid-stride kernels with a bounded grid, each block writing
+// one partial per reduction (no same-address atomics on the hot path).  The
+// host launches K1 and K3 once per run of consecutive zones with equal side
+// count (rect: one run of quads; pie: triangles, then quads).
 #include "common.h"
 #include "problem.h"
 
 using namespace pn;
+
+#define PN_HEAVY 64          // points with more corners get a whole block in K2
+#define PN_FIN_BLOCK 1024    // finalize block size
 
 namespace {
 
@@ -55,7 +67,9 @@ struct FusedArgs {
     // per-point intermediates
     double* vsx; double* vsy; double* pxpx;
     const unsigned char* pbc;
-    // partials
+    const int* heavy;
+    int nheavy;
+    // block partials
     double* part_dt; double* part_dv; int* part_bp; int* part_bc;
     Ctl* ctl;
     // physics parameters
@@ -69,223 +83,427 @@ __device__ __forceinline__ int nxt(int k) { return (k + 1 == NS) ? 0 : k + 1; }
 template <int NS>
 __device__ __forceinline__ int prv(int k) { return (k == 0) ? NS - 1 : k - 1; }
 
-// K1: predictor geometry, EOS, forces, QCS for one zone per thread.
+// ---------------------------------------------------------------------------
+// K1, zone-per-thread: the whole predictor half of doCycle for zone z, whose
+// sides are sb..sb+NS-1.  Returns the zone's calcDtNew candidate.
 template <int NS>
-__global__ void __launch_bounds__(PN_BLOCK) k_fused_zone_pred(FusedArgs a, int zbeg, int zend,
-                                                              int sbeg, int boff) {
-    if (a.ctl->done) return;
-    const int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x;
-    const bool live = z < zend;
-    double zdthyd = 1.e99;
-    int bad = 0;
-    if (live) {
-        const double dt = a.ctl->dt;
-        const double dth = 0.5 * dt;
-        const int sb = sbeg + (z - zbeg) * NS;
-        const double n = (double)NS;
-        d2 u0[NS], xp[NS];
-        double mf[NS];
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, double dt,
+                                            int& bad) {
+    const double dth = 0.5 * dt;
+    const double n = (double)NS;
+    d2 u0[NS], xp[NS];
+    double mf[NS];
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int p = a.mapsp1[sb + k];
-            const d2 x0 = mk(a.pxx[p], a.pxy[p]);
-            u0[k] = mk(a.pux[p], a.puy[p]);
-            xp[k] = add(x0, scl(dth, u0[k]));                 // advPosHalf
-            mf[k] = a.smf[sb + k];
-        }
-        // calcCtrs
-        d2 ex[NS];
-        d2 zx = mk(0., 0.);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            ex[k] = scl(0.5, add(xp[k], xp[nxt<NS>(k)]));
-            zx = add(zx, dvs(xp[k], n));
-        }
-        // calcVols, calcEdgeLen, calcCharLen
-        const double third = 1. / 3.;
-        double sa[NS], el[NS];
-        double zarea = 0., zvolp = 0.;
-        const double fac = (NS == 3 ? 3. : 4.);
-        double zdl = 1.e99;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const d2 p1 = xp[k], p2 = xp[nxt<NS>(k)];
-            sa[k] = 0.5 * cross(sub(p2, p1), sub(zx, p1));
-            const double sv = third * sa[k] * (p1.x + p2.x + zx.x);
-            zarea += sa[k];
-            zvolp += sv;
-            if (sv <= 0.) bad += 1;
-            el[k] = len(sub(p2, p1));
-        }
-#pragma unroll
-        for (int k = 0; k < NS; ++k) zdl = smin(zdl, fac * sa[k] / el[k]);
-        // calcRho (pred)
-        const double zm = a.zm[z];
-        const double zrp = zm / zvolp;
-        // calcStateHalf (reads ZR, ZE, ZWRATE of the previous cycle, ZVOL0 = ZVOL)
-        const double gm1 = a.gamma - 1.;
-        const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
-        const double r = a.zr[z];
-        const double e = smax(a.ze[z], 0.);
-        const double p = gm1 * r * e;
-        const double pre = gm1 * e;
-        const double per = gm1 * r;
-        const double csqd = smax(ssmin2, pre + per * p / (r * r));
-        const double zss = __builtin_sqrt(csqd);
-        const double minv = 1. / zm;
-        const double dv = (zvolp - a.zvol[z]) * minv;
-        const double bulk = r * csqd;
-        const double denom = 1. + 0.5 * per * dv;
-        const double src = a.zwrate[z] * dth * minv;
-        const double zp = p + (per * src - r * bulk * dv) / denom;
-        a.zp[z] = zp;
-        // calcSurfVecs, calcForcePGas, calcForceTTS
-        d2 tot[NS], sfw[NS];
-        double sstmp = smax(zss, a.ssmin);
-        sstmp = a.alfa * sstmp * sstmp;
-        d2 sfp[NS], sft[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const d2 ss = rot_ccw(sub(ex[k], zx));
-            sfp[k] = scl(-zp, ss);
-            const double vfacinv = zarea / sa[k];
-            const double srho = zrp * mf[k] * vfacinv;
-            const double sdp = sstmp * (srho - zrp);
-            sft[k] = scl(-sdp, ss);
-        }
-        // setCornerDiv
-        d2 zuc = mk(0., 0.);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) zuc = add(zuc, dvs(u0[k], n));
-        double carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int km = prv<NS>(k), kp = nxt<NS>(k);
-            const d2 up0 = u0[k];
-            const d2 xp0 = xp[k];
-            const d2 up1 = scl(0.5, add(up0, u0[kp]));
-            const d2 xp1 = ex[k];
-            const d2 up2 = zuc;
-            const d2 xp2 = zx;
-            const d2 up3 = scl(0.5, add(u0[km], up0));
-            const d2 xp3 = ex[km];
-            const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
-            carea[k] = cvolume;
-            const d2 v1 = sub(xp3, xp0);
-            const d2 v2 = sub(xp1, xp0);
-            const double de1 = el[km];
-            const double de2 = el[k];
-            const double minelen = smin(de1, de2);
-            ccos[k] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
-            const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
-                               (2.0 * cvolume);
-            cdiv[k] = div;
-            const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
-            const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
-            const double dx1 = len(dxx1);
-            const double dx2 = len(dxx2);
-            const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
-            const double test1 = fabs(dot(dxx1, duav) * dx2);
-            const double test2 = fabs(dot(dxx2, duav) * dx1);
-            const double num = (test1 > test2 ? dx1 : dx2);
-            const double den = (test1 > test2 ? dx2 : dx1);
-            const double rr = num / den;
-            double evol = __builtin_sqrt(4.0 * cvolume * rr);
-            evol = smin(evol, 2.0 * minelen);
-            const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
-            const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
-            double du = __builtin_sqrt(smax(dv1, dv2));
-            cevol[k] = (div < 0.0 ? evol : 0.);
-            cdu[k] = (div < 0.0 ? du : 0.);
-        }
-        // setQCnForce
-        const double gammap1 = a.qgamma + 1.0;
-        const double ztmp1 = a.q1 * zss;
-        d2 cqe1[NS], cqe2[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int km = prv<NS>(k), kp = nxt<NS>(k);
-            const double ztmp2 = a.q2 * 0.25 * gammap1 * cdu[k];
-            const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
-            const double mu = zkur * zrp * cevol[k];
-            const double cm = ((cdiv[k] > 0.0) ? 0. : mu);
-            cqe1[k] = dvs(scl(cm, sub(u0[k], u0[km])), el[km]);
-            cqe2[k] = dvs(scl(cm, sub(u0[kp], u0[k])), el[k]);
-        }
-        // setForceQCS
-        double cw[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const double cc = ccos[k];
-            const double csin2 = 1.0 - cc * cc;
-            cw[k] = ((csin2 < 1.e-4) ? 0. : carea[k] / csin2);
-            ccos[k] = ((csin2 < 1.e-4) ? 0. : cc);
-        }
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int kp = nxt<NS>(k);
-            const d2 f = add(scl(cw[k], add(cqe2[k], scl(ccos[k], cqe1[k]))),
-                             scl(cw[kp], add(cqe1[kp], scl(ccos[kp], cqe2[kp]))));
-            const d2 sfq = dvs(f, el[k]);
-            sfw[k] = add(sfp[k], sfq);                       // calcWork's sf + sf2
-            tot[k] = add(add(sfp[k], sfq), sft[k]);          // sumCrnrForce's sfp+sfq+sft
-        }
-        // setVelDiff
-        double ztmp = 0.;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int kp = nxt<NS>(k);
-            const d2 dx = sub(xp[kp], xp[k]);
-            const d2 du = sub(u0[kp], u0[k]);
-            const double lenx = el[k];
-            double dux = dot(du, dx);
-            dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
-            ztmp = smax(ztmp, dux);
-        }
-        const double zdu = a.q1 * zss + 2. * a.q2 * ztmp;
-        // calcDtNew, per zone
-        const double fuzz = 1.e-99;
-        const double cdt = smax(zdu, smax(zss, fuzz));
-        zdthyd = zdl * a.cfl / cdt;
-        // per-corner outputs: corner force and corner mass at corner k (point
-        // mapsp1[sb+k]), and the work force of side k
-        const double mzz = zrp * zarea;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int km = prv<NS>(k);
-            const d2 cf = sub(tot[k], tot[km]);
-            a.cfx[sb + k] = cf.x;
-            a.cfy[sb + k] = cf.y;
-            a.cmass[sb + k] = mzz * 0.5 * (mf[k] + mf[km]);
-            a.sfwx[sb + k] = sfw[k].x;
-            a.sfwy[sb + k] = sfw[k].y;
-        }
-        // NaN never replaces the running minimum (src/Hydro.cc:1615)
-        if (!(zdthyd < 1.e99)) zdthyd = 1.e99;
+    for (int k = 0; k < NS; ++k) {
+        const int p = a.mapsp1[sb + k];
+        const d2 x0 = mk(a.pxx[p], a.pxy[p]);
+        u0[k] = mk(a.pux[p], a.puy[p]);
+        xp[k] = add(x0, scl(dth, u0[k]));                 // advPosHalf
+        mf[k] = a.smf[sb + k];
     }
-    const double m = block_min(zdthyd);
+    // calcCtrs
+    d2 ex[NS];
+    d2 zx = mk(0., 0.);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        ex[k] = scl(0.5, add(xp[k], xp[nxt<NS>(k)]));
+        zx = add(zx, dvs(xp[k], n));
+    }
+    // calcVols, calcEdgeLen, calcCharLen
+    const double third = 1. / 3.;
+    double sa[NS], el[NS];
+    double zarea = 0., zvolp = 0.;
+    const double fac = (NS == 3 ? 3. : 4.);
+    double zdl = 1.e99;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const d2 p1 = xp[k], p2 = xp[nxt<NS>(k)];
+        sa[k] = 0.5 * cross(sub(p2, p1), sub(zx, p1));
+        const double sv = third * sa[k] * (p1.x + p2.x + zx.x);
+        zarea += sa[k];
+        zvolp += sv;
+        if (sv <= 0.) bad += 1;
+        el[k] = len(sub(p2, p1));
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zdl = smin(zdl, fac * sa[k] / el[k]);
+    // calcRho (pred)
+    const double zm = a.zm[z];
+    const double zrp = zm / zvolp;
+    // calcStateHalf (reads ZR, ZE, ZWRATE of the previous cycle, ZVOL0 = ZVOL)
+    const double gm1 = a.gamma - 1.;
+    const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
+    const double r = a.zr[z];
+    const double e = smax(a.ze[z], 0.);
+    const double p = gm1 * r * e;
+    const double pre = gm1 * e;
+    const double per = gm1 * r;
+    const double csqd = smax(ssmin2, pre + per * p / (r * r));
+    const double zss = __builtin_sqrt(csqd);
+    const double minv = 1. / zm;
+    const double dv = (zvolp - a.zvol[z]) * minv;
+    const double bulk = r * csqd;
+    const double denom = 1. + 0.5 * per * dv;
+    const double src = a.zwrate[z] * dth * minv;
+    const double zp = p + (per * src - r * bulk * dv) / denom;
+    a.zp[z] = zp;
+    // calcSurfVecs, calcForcePGas, calcForceTTS
+    d2 tot[NS], sfw[NS];
+    double sstmp = smax(zss, a.ssmin);
+    sstmp = a.alfa * sstmp * sstmp;
+    d2 sfp[NS], sft[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const d2 ss = rot_ccw(sub(ex[k], zx));
+        sfp[k] = scl(-zp, ss);
+        const double vfacinv = zarea / sa[k];
+        const double srho = zrp * mf[k] * vfacinv;
+        const double sdp = sstmp * (srho - zrp);
+        sft[k] = scl(-sdp, ss);
+    }
+    // setCornerDiv
+    d2 zuc = mk(0., 0.);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zuc = add(zuc, dvs(u0[k], n));
+    double carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k), kp = nxt<NS>(k);
+        const d2 up0 = u0[k];
+        const d2 xp0 = xp[k];
+        const d2 up1 = scl(0.5, add(up0, u0[kp]));
+        const d2 xp1 = ex[k];
+        const d2 up2 = zuc;
+        const d2 xp2 = zx;
+        const d2 up3 = scl(0.5, add(u0[km], up0));
+        const d2 xp3 = ex[km];
+        const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
+        carea[k] = cvolume;
+        const d2 v1 = sub(xp3, xp0);
+        const d2 v2 = sub(xp1, xp0);
+        const double de1 = el[km];
+        const double de2 = el[k];
+        const double minelen = smin(de1, de2);
+        ccos[k] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
+        const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
+                           (2.0 * cvolume);
+        cdiv[k] = div;
+        const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
+        const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
+        const double dx1 = len(dxx1);
+        const double dx2 = len(dxx2);
+        const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
+        const double test1 = fabs(dot(dxx1, duav) * dx2);
+        const double test2 = fabs(dot(dxx2, duav) * dx1);
+        const double num = (test1 > test2 ? dx1 : dx2);
+        const double den = (test1 > test2 ? dx2 : dx1);
+        const double rr = num / den;
+        double evol = __builtin_sqrt(4.0 * cvolume * rr);
+        evol = smin(evol, 2.0 * minelen);
+        const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
+        const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
+        double du = __builtin_sqrt(smax(dv1, dv2));
+        cevol[k] = (div < 0.0 ? evol : 0.);
+        cdu[k] = (div < 0.0 ? du : 0.);
+    }
+    // setQCnForce
+    const double gammap1 = a.qgamma + 1.0;
+    const double ztmp1 = a.q1 * zss;
+    d2 cqe1[NS], cqe2[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k), kp = nxt<NS>(k);
+        const double ztmp2 = a.q2 * 0.25 * gammap1 * cdu[k];
+        const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
+        const double mu = zkur * zrp * cevol[k];
+        const double cm = ((cdiv[k] > 0.0) ? 0. : mu);
+        cqe1[k] = dvs(scl(cm, sub(u0[k], u0[km])), el[km]);
+        cqe2[k] = dvs(scl(cm, sub(u0[kp], u0[k])), el[k]);
+    }
+    // setForceQCS
+    double cw[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double cc = ccos[k];
+        const double csin2 = 1.0 - cc * cc;
+        cw[k] = ((csin2 < 1.e-4) ? 0. : carea[k] / csin2);
+        ccos[k] = ((csin2 < 1.e-4) ? 0. : cc);
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int kp = nxt<NS>(k);
+        const d2 f = add(scl(cw[k], add(cqe2[k], scl(ccos[k], cqe1[k]))),
+                         scl(cw[kp], add(cqe1[kp], scl(ccos[kp], cqe2[kp]))));
+        const d2 sfq = dvs(f, el[k]);
+        sfw[k] = add(sfp[k], sfq);                       // calcWork's sf + sf2
+        tot[k] = add(add(sfp[k], sfq), sft[k]);          // sumCrnrForce's sfp+sfq+sft
+    }
+    // setVelDiff
+    double ztmp = 0.;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int kp = nxt<NS>(k);
+        const d2 dx = sub(xp[kp], xp[k]);
+        const d2 du = sub(u0[kp], u0[k]);
+        const double lenx = el[k];
+        double dux = dot(du, dx);
+        dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
+        ztmp = smax(ztmp, dux);
+    }
+    const double zdu = a.q1 * zss + 2. * a.q2 * ztmp;
+    // per-corner outputs: corner force and corner mass at corner k (point
+    // mapsp1[sb+k]), and the work force of side k
+    const double mzz = zrp * zarea;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k);
+        const d2 cf = sub(tot[k], tot[km]);
+        a.cfx[sb + k] = cf.x;
+        a.cfy[sb + k] = cf.y;
+        a.cmass[sb + k] = mzz * 0.5 * (mf[k] + mf[km]);
+        a.sfwx[sb + k] = sfw[k].x;
+        a.sfwy[sb + k] = sfw[k].y;
+    }
+    // calcDtNew for this zone
+    const double fuzz = 1.e-99;
+    const double cdt = smax(zdu, smax(zss, fuzz));
+    return zdl * a.cfl / cdt;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                        int poff) {
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    double dtmin = 1.e99;
+    int bad = 0;
+    for (int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x; z < zend; z += gridDim.x * PN_BLOCK) {
+        const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS, dt, bad);
+        // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
+        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+    }
+    const double m = block_min(dtmin);
     const int b = block_sum(bad);
     if (threadIdx.x == 0) {
-        a.part_dt[boff + blockIdx.x] = m;
-        a.part_bp[boff + blockIdx.x] = b;
+        a.part_dt[poff + blockIdx.x] = m;
+        a.part_bp[poff + blockIdx.x] = b;
     }
 }
 
-// K2: point gathers, fixed BCs, acceleration and position update.
-__global__ void __launch_bounds__(PN_BLOCK) k_fused_point(FusedArgs a) {
-    if (a.ctl->done) return;
-    const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (p >= a.nump) return;
-    const double dt = a.ctl->dt;
-    const double dth = 0.5 * dt;
-    double mass = 0.;
-    d2 f = mk(0., 0.);
-    const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
-    for (int k = kb; k < ke; ++k) {
-        const int s = a.pcside[k];
-        mass += a.cmass[s];
-        f = add(f, mk(a.cfx[s], a.cfy[s]));
+// ---------------------------------------------------------------------------
+// K1, corner-per-lane.
+
+// Corner group of a lane: the NS lanes holding one zone's corners.
+template <int NS>
+struct Grp {
+    int base;   // lane of corner 0
+    int k;      // this lane's corner
+    __device__ __forceinline__ double at(double v, int j) const { return __shfl(v, base + j, 64); }
+    __device__ __forceinline__ double nx(double v) const {
+        return __shfl(v, base + (k + 1 == NS ? 0 : k + 1), 64);
     }
+    __device__ __forceinline__ double pv(double v) const {
+        return __shfl(v, base + (k == 0 ? NS - 1 : k - 1), 64);
+    }
+    __device__ __forceinline__ d2 at(d2 v, int j) const { return mk(at(v.x, j), at(v.y, j)); }
+    __device__ __forceinline__ d2 nx(d2 v) const { return mk(nx(v.x), nx(v.y)); }
+    __device__ __forceinline__ d2 pv(d2 v) const { return mk(pv(v.x), pv(v.y)); }
+    // sum over the group's corners in ascending order, starting from 0
+    __device__ __forceinline__ double sum(double v) const {
+        double acc = 0.;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) acc += at(v, j);
+        return acc;
+    }
+    __device__ __forceinline__ d2 sum(d2 v) const {
+        d2 acc = mk(0., 0.);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) acc = add(acc, at(v, j));
+        return acc;
+    }
+};
+
+// One corner of zone z (sides s - k .. s - k + NS - 1); returns the zone's
+// calcDtNew candidate on the group's corner 0 and 1e99 elsewhere.
+template <int NS>
+__device__ __forceinline__ double corner_pred(const FusedArgs& a, const Grp<NS>& g, int z, int s,
+                                              bool live, double dt, int& bad) {
+    const double dth = 0.5 * dt;
+    const double n = (double)NS;
+    // advPosHalf at this corner's point
+    const int p = a.mapsp1[s];
+    const d2 x0 = mk(a.pxx[p], a.pxy[p]);
+    const d2 u0 = mk(a.pux[p], a.puy[p]);
+    const d2 xp = add(x0, scl(dth, u0));
+    const double mf = a.smf[s];
+    const d2 xpn = g.nx(xp), u0n = g.nx(u0), u0p = g.pv(u0);
+    // calcCtrs
+    const d2 ex = scl(0.5, add(xp, xpn));
+    const d2 zx = g.sum(dvs(xp, n));
+    const d2 exp_ = g.pv(ex);
+    // calcVols, calcEdgeLen
+    const double third = 1. / 3.;
+    const double sa = 0.5 * cross(sub(xpn, xp), sub(zx, xp));
+    const double sv = third * sa * (xp.x + xpn.x + zx.x);
+    const double zarea = g.sum(sa);
+    const double zvolp = g.sum(sv);
+    if (live && sv <= 0.) bad += 1;
+    const double el = len(sub(xpn, xp));
+    const double elp = g.pv(el);
+    // calcCharLen: min over the zone's sides in ascending order from 1e99
+    const double fac = (NS == 3 ? 3. : 4.);
+    const double sdl = fac * sa / el;
+    double zdl = 1.e99;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) zdl = smin(zdl, g.at(sdl, j));
+    // calcRho (pred)
+    const double zm = a.zm[z];
+    const double zrp = zm / zvolp;
+    // calcStateHalf (reads ZR, ZE, ZWRATE of the previous cycle, ZVOL0 = ZVOL)
+    const double gm1 = a.gamma - 1.;
+    const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
+    const double r = a.zr[z];
+    const double e = smax(a.ze[z], 0.);
+    const double pp = gm1 * r * e;
+    const double pre = gm1 * e;
+    const double per = gm1 * r;
+    const double csqd = smax(ssmin2, pre + per * pp / (r * r));
+    const double zss = __builtin_sqrt(csqd);
+    const double minv = 1. / zm;
+    const double dv = (zvolp - a.zvol[z]) * minv;
+    const double bulk = r * csqd;
+    const double denom = 1. + 0.5 * per * dv;
+    const double src = a.zwrate[z] * dth * minv;
+    const double zp = pp + (per * src - r * bulk * dv) / denom;
+    // calcSurfVecs, calcForcePGas, calcForceTTS
+    const d2 ssurf = rot_ccw(sub(ex, zx));
+    const d2 sfp = scl(-zp, ssurf);
+    double sstmp = smax(zss, a.ssmin);
+    sstmp = a.alfa * sstmp * sstmp;
+    const double vfacinv = zarea / sa;
+    const double srho = zrp * mf * vfacinv;
+    const double sdp = sstmp * (srho - zrp);
+    const d2 sft = scl(-sdp, ssurf);
+    // setCornerDiv for corner k (s = ss3[c] is side k-1, s2 = c is side k)
+    const d2 zuc = g.sum(dvs(u0, n));
+    const d2 up0 = u0;
+    const d2 xp0 = xp;
+    const d2 up1 = scl(0.5, add(up0, u0n));
+    const d2 xp1 = ex;
+    const d2 up2 = zuc;
+    const d2 xp2 = zx;
+    const d2 up3 = scl(0.5, add(u0p, up0));
+    const d2 xp3 = exp_;
+    const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
+    const d2 v1 = sub(xp3, xp0);
+    const d2 v2 = sub(xp1, xp0);
+    const double de1 = elp;
+    const double de2 = el;
+    const double minelen = smin(de1, de2);
+    double ccos = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
+    const double cdiv = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
+                        (2.0 * cvolume);
+    const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
+    const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
+    const double dx1 = len(dxx1);
+    const double dx2 = len(dxx2);
+    const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
+    const double test1 = fabs(dot(dxx1, duav) * dx2);
+    const double test2 = fabs(dot(dxx2, duav) * dx1);
+    const double num = (test1 > test2 ? dx1 : dx2);
+    const double den = (test1 > test2 ? dx2 : dx1);
+    const double rr = num / den;
+    double evol = __builtin_sqrt(4.0 * cvolume * rr);
+    evol = smin(evol, 2.0 * minelen);
+    const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
+    const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
+    double du = __builtin_sqrt(smax(dv1, dv2));
+    evol = (cdiv < 0.0 ? evol : 0.);
+    du = (cdiv < 0.0 ? du : 0.);
+    // setQCnForce
+    const double gammap1 = a.qgamma + 1.0;
+    const double ztmp2 = a.q2 * 0.25 * gammap1 * du;
+    const double ztmp1 = a.q1 * zss;
+    const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
+    const double mu = zkur * zrp * evol;
+    const double crmu = ((cdiv > 0.0) ? 0. : mu);
+    const d2 cqe1 = dvs(scl(crmu, sub(u0, u0p)), elp);
+    const d2 cqe2 = dvs(scl(crmu, sub(u0n, u0)), el);
+    // setForceQCS: side k joins corner k (c1) and corner k+1 (c2)
+    const double csin2 = 1.0 - ccos * ccos;
+    const double cw = ((csin2 < 1.e-4) ? 0. : cvolume / csin2);
+    ccos = ((csin2 < 1.e-4) ? 0. : ccos);
+    const double cw2 = g.nx(cw), ccos2 = g.nx(ccos);
+    const d2 cqe12 = g.nx(cqe1), cqe22 = g.nx(cqe2);
+    const d2 fq = add(scl(cw, add(cqe2, scl(ccos, cqe1))), scl(cw2, add(cqe12, scl(ccos2, cqe22))));
+    const d2 sfq = dvs(fq, el);
+    // setVelDiff: max over the zone's sides from 0
+    const d2 dxs = sub(xpn, xp);
+    const d2 dus = sub(u0n, u0);
+    double dux = dot(dus, dxs);
+    dux = (el > 0. ? fabs(dux) / el : 0.);
+    double ztmp = 0.;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) ztmp = smax(ztmp, g.at(dux, j));
+    const double zdu = a.q1 * zss + 2. * a.q2 * ztmp;
+    // sumCrnrForce's corner force, calcCrnrMass's corner mass, calcWork's
+    // side force
+    const d2 tot = add(add(sfp, sfq), sft);
+    const d2 cf = sub(tot, g.pv(tot));
+    const double mfp = g.pv(mf);
+    const double cmass = zrp * zarea * 0.5 * (mf + mfp);
+    if (live) {
+        a.cfx[s] = cf.x;
+        a.cfy[s] = cf.y;
+        a.cmass[s] = cmass;
+        const d2 sfw = add(sfp, sfq);
+        a.sfwx[s] = sfw.x;
+        a.sfwy[s] = sfw.y;
+        if (g.k == 0) a.zp[z] = zp;
+    }
+    const double fuzz = 1.e-99;
+    const double cdt = smax(zdu, smax(zss, fuzz));
+    const double zdthyd = zdl * a.cfl / cdt;
+    return (live && g.k == 0) ? zdthyd : 1.e99;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(PN_BLOCK) k_pred_corner(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                          int poff) {
+    if (a.ctl->done) return;
+    constexpr int ZPW = 64 / NS;                      // zones per wave
+    constexpr int ZPB = ZPW * (PN_BLOCK / 64);        // zones per block tile
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    Grp<NS> g;
+    const int gi = lane / NS;
+    g.k = lane - gi * NS;
+    g.base = gi * NS;
+    const double dt = a.ctl->dt;
+    double dtmin = 1.e99;
+    int bad = 0;
+    for (int z0 = zbeg + blockIdx.x * ZPB; z0 < zend; z0 += gridDim.x * ZPB) {
+        const int zw = z0 + wave * ZPW + gi;
+        const bool live = (gi < ZPW) && (zw < zend);
+        const int z = live ? zw : zbeg;               // idle lanes read a valid zone
+        const double zdthyd = corner_pred<NS>(a, g, z, sbeg + (z - zbeg) * NS + g.k, live, dt, bad);
+        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+    }
+    const double m = block_min(dtmin);
+    const int b = block_sum(bad);
+    if (threadIdx.x == 0) {
+        a.part_dt[poff + blockIdx.x] = m;
+        a.part_bp[poff + blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2
+
+// point update shared by the regular and the heavy-point paths
+__device__ __forceinline__ void point_update(const FusedArgs& a, int p, double mass, d2 f,
+                                             double dt) {
+    const double dth = 0.5 * dt;
     const d2 x0 = mk(a.pxx[p], a.pxy[p]);
     const d2 uin = mk(a.pux[p], a.puy[p]);
     d2 u0 = uin;
@@ -313,17 +531,65 @@ __global__ void __launch_bounds__(PN_BLOCK) k_fused_point(FusedArgs a) {
     a.puy[p] = u.y;
 }
 
+// Point gathers, fixed BCs, acceleration and position update.  Blocks
+// [0, nheavy) each take one high-valence point (the pie origin has one
+// corner per centre triangle): the block stages the corner values in LDS and
+// one thread adds them in ascending order.
+__global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    if ((int)blockIdx.x < a.nheavy) {
+        __shared__ double sm[PN_BLOCK], sx[PN_BLOCK], sy[PN_BLOCK];
+        const int p = a.heavy[blockIdx.x];
+        const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
+        double mass = 0.;
+        d2 f = mk(0., 0.);
+        for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
+            const int k = c0 + threadIdx.x;
+            if (k < ke) {
+                const int s = a.pcside[k];
+                sm[threadIdx.x] = a.cmass[s];
+                sx[threadIdx.x] = a.cfx[s];
+                sy[threadIdx.x] = a.cfy[s];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int cnt = (ke - c0 < PN_BLOCK) ? ke - c0 : PN_BLOCK;
+                for (int i = 0; i < cnt; ++i) {
+                    mass += sm[i];
+                    f = add(f, mk(sx[i], sy[i]));
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) point_update(a, p, mass, f, dt);
+        return;
+    }
+    const int p = (blockIdx.x - a.nheavy) * PN_BLOCK + threadIdx.x;
+    if (p >= a.nump) return;
+    const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
+    if (ke - kb > PN_HEAVY) return;          // done by a heavy-point block
+    double mass = 0.;
+    d2 f = mk(0., 0.);
+    for (int k = kb; k < ke; ++k) {
+        const int s = a.pcside[k];
+        mass += a.cmass[s];
+        f = add(f, mk(a.cfx[s], a.cfy[s]));
+    }
+    point_update(a, p, mass, f, dt);
+}
+
+// ---------------------------------------------------------------------------
 // K3: corrector geometry, work, work rate, energy, density, dvol.
 template <int NS>
-__global__ void __launch_bounds__(PN_BLOCK) k_fused_zone_corr(FusedArgs a, int zbeg, int zend,
-                                                              int sbeg, int boff) {
+__global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                   int poff) {
     if (a.ctl->done) return;
-    const int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x;
+    const double dt = a.ctl->dt;
+    const double dth = 0.5 * dt;
     double dvov = 1.e-99;
     int bad = 0;
-    if (z < zend) {
-        const double dt = a.ctl->dt;
-        const double dth = 0.5 * dt;
+    for (int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x; z < zend; z += gridDim.x * PN_BLOCK) {
         const int sb = sbeg + (z - zbeg) * NS;
         const double n = (double)NS;
         int pp[NS];
@@ -375,35 +641,71 @@ __global__ void __launch_bounds__(PN_BLOCK) k_fused_zone_corr(FusedArgs a, int z
     const double m = block_max(dvov);
     const int b = block_sum(bad);
     if (threadIdx.x == 0) {
-        a.part_dv[boff + blockIdx.x] = m;
-        a.part_bc[boff + blockIdx.x] = b;
+        a.part_dv[poff + blockIdx.x] = m;
+        a.part_bc[poff + blockIdx.x] = b;
     }
 }
 
-// K4: reductions over block partials and the end-of-cycle scalars.
-__global__ void __launch_bounds__(PN_BLOCK) k_fused_finalize(FusedArgs a, int nparts, double cflv,
-                                                             int cstop, double dtmax, double dtinit,
-                                                             double dtfac, double tstop) {
+// ---------------------------------------------------------------------------
+// K4: reductions over the block partials (min and max are exact in any
+// order; NaN partials cannot occur, the per-zone folds skip NaN) and the end
+// of the cycle: calcDt (src/Hydro.cc:1647-1667), checkBadSides
+// (src/Mesh.cc:1531-1550), the time/cycle update (src/Driver.cc:266-295), the
+// stop test and calcGlobalDt (src/Driver.cc:213-245) for the next cycle.
+__device__ __forceinline__ double blk_min_fin(double v) {
+    __shared__ double part[PN_FIN_BLOCK / 64];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v = (part[i] < v) ? part[i] : v;
+    return v;
+}
+__device__ __forceinline__ double blk_max_fin(double v) {
+    __shared__ double part[PN_FIN_BLOCK / 64];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v = (part[i] > v) ? part[i] : v;
+    return v;
+}
+__device__ __forceinline__ int blk_sum_fin(int v) {
+    __shared__ int part[PN_FIN_BLOCK / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v += part[i];
+    return v;
+}
+
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
+                                                           double cflv, int cstop, double dtmax,
+                                                           double dtfac, double tstop) {
     Ctl* c = a.ctl;
     if (c->done) return;
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
-    for (int i = threadIdx.x; i < nparts; i += PN_BLOCK) {
-        mn = (a.part_dt[i] < mn) ? a.part_dt[i] : mn;
-        mx = (a.part_dv[i] > mx) ? a.part_dv[i] : mx;
+    for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
+        const double v = a.part_dt[i];
+        mn = (v < mn) ? v : mn;
         bp += a.part_bp[i];
+    }
+    for (int i = threadIdx.x; i < ncorr; i += PN_FIN_BLOCK) {
+        const double v = a.part_dv[i];
+        mx = (v > mx) ? v : mx;
         bc += a.part_bc[i];
     }
-    mn = block_min(mn);
-    mx = block_max(mx);
-    bp = block_sum(bp);
-    bc = block_sum(bc);
+    mn = blk_min_fin(mn);
+    mx = blk_max_fin(mx);
+    bp = blk_sum_fin(bp);
+    bc = blk_sum_fin(bc);
     if (threadIdx.x != 0) return;
     c->dtnew = mn;
     c->dvovmax = mx;
     c->numsbad_pred = bp;
     c->numsbad_corr = bc;
-    // calcDtTask, src/Hydro.cc:1647-1667
     double r = 1.e99;
     if (mn < r) r = mn;
     const double d2v = c->dt * cflv / mx;
@@ -419,7 +721,6 @@ __global__ void __launch_bounds__(PN_BLOCK) k_fused_finalize(FusedArgs a, int np
     c->cycle = c->cycle + 1;
     c->dtlast = c->dt;
     c->done = (c->cycle >= cstop || c->time >= tstop) ? 1 : 0;
-    // calcGlobalDtTask, src/Driver.cc:213-245, for the next cycle
     double dt = dtmax;
     const double dtrecover = dtfac * c->dtlast;
     if (dtrecover < dt) dt = dtrecover;
@@ -438,8 +739,9 @@ FusedArgs make_args(pennant_problem* pb) {
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
     a.cfx = pb->cfx; a.cfy = pb->cfy; a.cmass = pb->cmass; a.sfwx = pb->sfwx; a.sfwy = pb->sfwy;
     a.vsx = pb->vsx; a.vsy = pb->vsy; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
-    a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->nparts;
-    a.part_bp = pb->part_i; a.part_bc = pb->part_i + pb->nparts;
+    a.heavy = pb->heavy; a.nheavy = pb->nheavy;
+    a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->npart_pred;
+    a.part_bp = pb->part_i; a.part_bc = pb->part_i + pb->npart_pred;
     a.ctl = pb->ctl;
     a.gamma = q.gamma; a.ssmin = q.ssmin; a.alfa = q.alfa; a.qgamma = q.qgamma;
     a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
@@ -448,36 +750,51 @@ FusedArgs make_args(pennant_problem* pb) {
     return a;
 }
 
+int grid_for(int needed, int cap) { return needed < 1 ? 1 : (needed < cap ? needed : cap); }
+
+// blocks of each K1 / K3 launch for a run of nz zones with NS sides
+int pred_grid(const pennant_problem* pb, int nz, int ns) {
+    const int per = (pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
+    return grid_for((nz + per - 1) / per, pb->tune.max_blocks);
+}
+int corr_grid(const pennant_problem* pb, int nz) {
+    return grid_for(nblocks(nz), pb->tune.max_blocks);
+}
+
 template <int NS>
-void launch_pred(const FusedArgs& a, int zb, int ze, int sb, int boff, hipStream_t s) {
-    hipLaunchKernelGGL(k_fused_zone_pred<NS>, dim3(nblocks(ze - zb)), dim3(PN_BLOCK), 0, s, a, zb,
-                       ze, sb, boff);
+void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
+                 hipStream_t s) {
+    const dim3 grid(pred_grid(pb, ze - zb, NS));
+    if (pb->tune.k1_layout == 1)
+        hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else
+        hipLaunchKernelGGL(k_pred_zone<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
 }
 template <int NS>
-void launch_corr(const FusedArgs& a, int zb, int ze, int sb, int boff, hipStream_t s) {
-    hipLaunchKernelGGL(k_fused_zone_corr<NS>, dim3(nblocks(ze - zb)), dim3(PN_BLOCK), 0, s, a, zb,
-                       ze, sb, boff);
+void launch_corr(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_corr<NS>, dim3(corr_grid(pb, ze - zb)), dim3(PN_BLOCK), 0, s, a, zb, ze, sb,
+                       poff);
 }
 
 }  // namespace
 
 namespace pn {
 
-// Builds the zone runs, the per-point BC bits and the fused-path buffers.
+// Builds the zone runs, the per-point BC bits, the heavy-point list and the
+// fused-path buffers (partials sized for the largest grids the tuning
+// allows).
 int fused_setup(pennant_problem* pb) {
     const HostMesh& h = pb->hm;
     pb->zruns.clear();
-    int nparts = 0;
     for (int z = 0; z < h.numz;) {
         int e = z + 1;
         while (e < h.numz && h.znump[e] == h.znump[z]) ++e;
         pb->zruns.push_back(z);
         pb->zruns.push_back(e);
         pb->zruns.push_back(h.znump[z]);
-        nparts += nblocks(e - z);
         z = e;
     }
-    pb->nparts = nparts;
     if (pb->bcs.size() > 8) return PENNANT_EINVAL;
     pb->nbcs = (int)pb->bcs.size();
     std::vector<unsigned char> bits((size_t)h.nump, 0);
@@ -494,6 +811,9 @@ int fused_setup(pennant_problem* pb) {
     };
     for (double b : pb->prm.bcx) mark(b, true);
     for (double b : pb->prm.bcy) mark(b, false);
+    std::vector<int> heavy;
+    for (int p = 0; p < h.nump; ++p)
+        if (h.pcbeg[p + 1] - h.pcbeg[p] > PN_HEAVY) heavy.push_back(p);
     auto al = [&](size_t n, size_t sz) -> void* {
         void* p = nullptr;
         if (hipMalloc(&p, n * sz > 0 ? n * sz : sz) != hipSuccess) return nullptr;
@@ -506,11 +826,21 @@ int fused_setup(pennant_problem* pb) {
     pb->sfwx = (double*)al(S, 8); pb->sfwy = (double*)al(S, 8);
     pb->vsx = (double*)al(P, 8); pb->vsy = (double*)al(P, 8); pb->pxpx = (double*)al(P, 8);
     pb->pbc = (unsigned char*)al(P, 1);
-    pb->part_d = (double*)al(2 * (size_t)nparts, 8);
-    pb->part_i = (int*)al(2 * (size_t)nparts, 4);
+    pb->heavy = (int*)al(heavy.size(), 4);
+    pb->nheavy = (int)heavy.size();
+    // partial slots: one per block of every K1 (resp. K3) launch of a cycle,
+    // at the largest grid any tuning allows
+    const int nruns = (int)pb->zruns.size() / 3;
+    pb->npart_pred = nruns * PN_MAX_GRID;
+    pb->npart_corr = nruns * PN_MAX_GRID;
+    pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
+    pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
     if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
-        !pb->pxpx || !pb->pbc || !pb->part_d || !pb->part_i)
+        !pb->pxpx || !pb->pbc || !pb->heavy || !pb->part_d || !pb->part_i)
         return (int)hipErrorOutOfMemory;
+    if (!heavy.empty() &&
+        hipMemcpy(pb->heavy, heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return (int)hipErrorUnknown;
     return (int)hipMemcpy(pb->pbc, bits.data(), bits.size(), hipMemcpyHostToDevice);
 }
 
@@ -520,42 +850,44 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev) {
     hipStream_t s = pb->stream;
     const HostMesh& h = pb->hm;
     if (ev) (void)hipEventRecord(ev[0], s);
-    int boff = 0;
+    int poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_pred<3>(a, zb, ze, sb, boff, s); break;
-            case 4: launch_pred<4>(a, zb, ze, sb, boff, s); break;
-            case 5: launch_pred<5>(a, zb, ze, sb, boff, s); break;
-            case 6: launch_pred<6>(a, zb, ze, sb, boff, s); break;
-            case 7: launch_pred<7>(a, zb, ze, sb, boff, s); break;
-            case 8: launch_pred<8>(a, zb, ze, sb, boff, s); break;
+            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, s); break;
+            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, s); break;
+            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, s); break;
+            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, s); break;
+            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, s); break;
+            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, s); break;
             default: return PENNANT_EINVAL;
         }
-        boff += nblocks(ze - zb);
+        poff += pred_grid(pb, ze - zb, ns);
     }
+    const int npred = poff;
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_fused_point, dim3(nblocks(h.nump)), dim3(PN_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_point, dim3(pb->nheavy + nblocks(h.nump)), dim3(PN_BLOCK), 0, s, a);
     if (ev) (void)hipEventRecord(ev[2], s);
-    boff = 0;
+    poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_corr<3>(a, zb, ze, sb, boff, s); break;
-            case 4: launch_corr<4>(a, zb, ze, sb, boff, s); break;
-            case 5: launch_corr<5>(a, zb, ze, sb, boff, s); break;
-            case 6: launch_corr<6>(a, zb, ze, sb, boff, s); break;
-            case 7: launch_corr<7>(a, zb, ze, sb, boff, s); break;
-            case 8: launch_corr<8>(a, zb, ze, sb, boff, s); break;
+            case 3: launch_corr<3>(pb, a, zb, ze, sb, poff, s); break;
+            case 4: launch_corr<4>(pb, a, zb, ze, sb, poff, s); break;
+            case 5: launch_corr<5>(pb, a, zb, ze, sb, poff, s); break;
+            case 6: launch_corr<6>(pb, a, zb, ze, sb, poff, s); break;
+            case 7: launch_corr<7>(pb, a, zb, ze, sb, poff, s); break;
+            case 8: launch_corr<8>(pb, a, zb, ze, sb, poff, s); break;
             default: return PENNANT_EINVAL;
         }
-        boff += nblocks(ze - zb);
+        poff += corr_grid(pb, ze - zb);
     }
+    const int ncorr = poff;
     if (ev) (void)hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL(k_fused_finalize, dim3(1), dim3(PN_BLOCK), 0, s, a, pb->nparts, q.cflv,
-                       q.cstop, q.dtmax, q.dtinit, q.dtfac, q.tstop);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred, ncorr, q.cflv,
+                       q.cstop, q.dtmax, q.dtfac, q.tstop);
     if (ev) (void)hipEventRecord(ev[4], s);
     return (int)hipGetLastError();
 }

@@ -30,6 +30,14 @@ struct Ctl {
     int err_pred;     // 1 when the failed check was the predictor's
 };
 
+#define PN_MAX_GRID 4096   // largest grid of a fused K1/K3 launch
+
+// Launch-shape choices of the fused cycle (pennant_set_tuning).
+struct Tuning {
+    int k1_layout = 0;       // 0 zone per thread, 1 corner per lane
+    int max_blocks = 2048;   // grid cap of the grid-stride K1/K3 launches
+};
+
 struct Params {
     int cstop; double tstop, dtmax, dtinit, dtfac; int dtreport;
     double cfl, cflv, rinit, einit, rinitsub, einitsub, uinitradial;
@@ -77,9 +85,12 @@ struct pennant_problem {
     double* sfwx = nullptr; double* sfwy = nullptr;
     double* vsx = nullptr; double* vsy = nullptr; double* pxpx = nullptr;
     unsigned char* pbc = nullptr;    // per-point BC membership bits (one per list)
-    double* part_d = nullptr;        // fused reduction partials
-    int* part_i = nullptr;
-    int nparts = 0;
+    double* part_d = nullptr;        // block partials: dt minima, then dvol maxima
+    int* part_i = nullptr;           // block partials: bad-side counts (pred, corr)
+    int npart_pred = 0, npart_corr = 0;
+    int* heavy = nullptr;            // points with more corners than a K2 thread takes
+    int nheavy = 0;
+    pn::Tuning tune;
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
     double bcvx[8], bcvy[8];

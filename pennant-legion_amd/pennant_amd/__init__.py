@@ -117,6 +117,8 @@ DRIVER_SIGNATURES = {
     "pennant_host_mesh_destroy": ([_P], None),
     "pennant_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_set_tuning": ([_P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "pennant_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
 }
 
 
@@ -226,12 +228,20 @@ class Problem:
         self._check(lib().pennant_sync(self._h))
 
     def time_cycles(self, ncycles: int, mode: int = MODE_FUSED, phases: bool = True):
-        """(total_ms, phase_ms[4] or None) for ncycles cycles timed with HIP
-        events on the problem's stream."""
+        """(total_ms, phase_ms or None) for ncycles cycles timed with HIP
+        events on the problem's stream; phase_ms holds the summed durations of
+        the zone-predictor, point, zone-corrector and finalize kernels."""
         tot = ctypes.c_double()
         ph = (ctypes.c_double * 4)() if phases else None
         self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
         return tot.value, (list(ph) if phases and mode == MODE_FUSED else None)
+
+    def set_tuning(self, key: str, value: int):
+        if lib().pennant_set_tuning(self._h, key.encode(), int(value)) != 0:
+            raise ValueError(f"bad tuning {key}={value}")
+
+    def get_tuning(self, key: str) -> int:
+        return lib().pennant_get_tuning(self._h, key.encode())
 
     def scalars(self) -> dict:
         out = (ctypes.c_double * 10)()
