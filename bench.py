@@ -92,8 +92,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sedov1440")
     ap.add_argument("--mode", default="fused", choices=["fused", "tasks"])
+    ap.add_argument("--copy-mb", type=int, default=512,
+                    help="size of the streaming-copy calibration (0 disables)")
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
     ap.add_argument("--max-blocks", type=int, default=None, help="grid cap of the fused zone kernels")
+    ap.add_argument("--xcd", type=int, default=None, help="XCD band mapping of the zone kernels (0/1)")
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of CPU time per oracle variant (0 disables)")
     args = ap.parse_args()
@@ -113,6 +116,8 @@ def main():
         pb.set_tuning("k1_layout", args.k1)
     if args.max_blocks is not None:
         pb.set_tuning("max_blocks", args.max_blocks)
+    if args.xcd is not None:
+        pb.set_tuning("xcd", args.xcd)
     log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
     if args.warmup > 0:
         pb.run(args.warmup, mode)
@@ -140,6 +145,14 @@ def main():
                     "traffic": None,
                     "alg_bytes_per_launch": kb[dom], "avg_launch_us": round(per[dom] * 1e3, 2),
                     "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1)}
+        if args.copy_mb > 0:
+            import ctypes
+            ms = ctypes.c_double()
+            nbytes = args.copy_mb << 20
+            if pa.lib().pennant_bench_copy(nbytes, 5, ctypes.byref(ms)) == 0:
+                copy_gbs = 2 * nbytes / (ms.value * 1e-3) / 1e9
+                log(f"  streaming copy of {args.copy_mb} MiB: {copy_gbs:.1f} GB/s (read + write)")
+                roofline["copy_GBs"] = round(copy_gbs, 1)
 
     cpu = None
     if args.cpu_budget > 0:

@@ -341,9 +341,18 @@ int pennant_end_cycle(pennant_problem* pb);
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms);
 
+/* Measurement helper: copies nbytes between two device buffers `reps` times
+ * with a plain streaming kernel (k_stream_copy) and reports the average time
+ * per copy.  Under a hardware-counter profiler its dispatches read and write
+ * exactly nbytes each, which calibrates the FETCH_SIZE / WRITE_SIZE counters
+ * against known traffic.  Allocates and frees its own buffers. */
+int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
+
 /* Launch-shape choices of the fused cycle; results do not depend on them.
  *   "k1_layout"  0: one zone per thread (default), 1: one corner per lane
  *   "max_blocks" grid cap of the grid-stride zone kernels, 1..4096 (2048)
+ *   "xcd"        1: each of the 8 XCDs takes a contiguous band of zones (default),
+ *                0: plain grid-stride order
  * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
 int pennant_get_tuning(pennant_problem* pb, const char* key);

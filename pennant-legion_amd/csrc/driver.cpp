@@ -513,12 +513,18 @@ int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
         pb->tune.max_blocks = value;
         return 0;
     }
+    if (!strcmp(key, "xcd")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        pb->tune.xcd = value;
+        return 0;
+    }
     return PENNANT_EINVAL;
 }
 
 int pennant_get_tuning(pennant_problem* pb, const char* key) {
     if (!strcmp(key, "k1_layout")) return pb->tune.k1_layout;
     if (!strcmp(key, "max_blocks")) return pb->tune.max_blocks;
+    if (!strcmp(key, "xcd")) return pb->tune.xcd;
     return PENNANT_EINVAL;
 }
 

@@ -13,8 +13,10 @@
 //                  mass), calcStateHalf, calcForcePGas, calcForceTTS,
 //                  setCornerDiv, setQCnForce, setForceQCS, setVelDiff, the
 //                  corner force of sumCrnrForce, calcDtNew's per-zone dt
-//   K2 point pass  corner-mass and corner-force gathers (ascending side
-//                  order), applyFixedBC, calcAccel, advPosFull
+//   K2 point pass  corner-mass and corner-force sums (ascending side order;
+//                  K1 stores the corner values in point-corner CSR order, so
+//                  each point reads one contiguous run), applyFixedBC,
+//                  calcAccel, advPosFull
 //   K3 zone pass   calcCtrs, calcVols (corrector), calcWork, calcWorkRate,
 //                  calcEnergy, calcRho, calcDvol's per-zone ratio
 //   K4 finalize    dt minimum, dvol maximum and bad-side counts over the
@@ -55,7 +57,7 @@ struct FusedArgs {
     // mesh
     const int32_t* mapsp1;
     const int32_t* pcbeg;
-    const int32_t* pcside;
+    const int32_t* pcpos;      // side -> slot of its corner in the point-corner CSR
     const double* smf;
     int nump;
     // state (in place)
@@ -76,7 +78,32 @@ struct FusedArgs {
     double gamma, ssmin, alfa, qgamma, q1, q2, cfl;
     int nbcs;
     double bcvx[8], bcvy[8];
+    int xcd;   // XCD-aware unit mapping in the grid-stride kernels
 };
+
+// Work units of a grid-stride launch.  MI355X dispatches workgroups to its 8
+// XCDs round-robin by workgroup id, and each XCD has its own L2; with `xcd`
+// set (and a grid that is a multiple of 8) XCD x's blocks take the x-th
+// contiguous eighth of the units, so neighbouring zones -- which share points
+// and write neighbouring point-corner slots -- meet in the same L2.
+struct Units {
+    int u, end, step;
+};
+__device__ __forceinline__ Units block_units(int nunits, int xcd) {
+    const int G = gridDim.x, b = blockIdx.x;
+    Units r;
+    if (xcd && G >= 8 && (G & 7) == 0) {
+        const int x = b & 7, per = G >> 3;
+        r.u = (int)((long long)nunits * x / 8) + (b >> 3);
+        r.end = (int)((long long)nunits * (x + 1) / 8);
+        r.step = per;
+    } else {
+        r.u = b;
+        r.end = nunits;
+        r.step = G;
+    }
+    return r;
+}
 
 template <int NS>
 __device__ __forceinline__ int nxt(int k) { return (k + 1 == NS) ? 0 : k + 1; }
@@ -258,9 +285,10 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, d
     for (int k = 0; k < NS; ++k) {
         const int km = prv<NS>(k);
         const d2 cf = sub(tot[k], tot[km]);
-        a.cfx[sb + k] = cf.x;
-        a.cfy[sb + k] = cf.y;
-        a.cmass[sb + k] = mzz * 0.5 * (mf[k] + mf[km]);
+        const int slot = a.pcpos[sb + k];
+        a.cfx[slot] = cf.x;
+        a.cfy[slot] = cf.y;
+        a.cmass[slot] = mzz * 0.5 * (mf[k] + mf[km]);
         a.sfwx[sb + k] = sfw[k].x;
         a.sfwy[sb + k] = sfw[k].y;
     }
@@ -277,7 +305,10 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
     int bad = 0;
-    for (int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x; z < zend; z += gridDim.x * PN_BLOCK) {
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+        const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
+        if (z >= zend) continue;
         const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS, dt, bad);
         // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
@@ -453,9 +484,10 @@ __device__ __forceinline__ double corner_pred(const FusedArgs& a, const Grp<NS>&
     const double mfp = g.pv(mf);
     const double cmass = zrp * zarea * 0.5 * (mf + mfp);
     if (live) {
-        a.cfx[s] = cf.x;
-        a.cfy[s] = cf.y;
-        a.cmass[s] = cmass;
+        const int slot = a.pcpos[s];
+        a.cfx[slot] = cf.x;
+        a.cfy[slot] = cf.y;
+        a.cmass[slot] = cmass;
         const d2 sfw = add(sfp, sfq);
         a.sfwx[s] = sfw.x;
         a.sfwy[s] = sfw.y;
@@ -482,8 +514,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_corner(FusedArgs a, int zbeg,
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
     int bad = 0;
-    for (int z0 = zbeg + blockIdx.x * ZPB; z0 < zend; z0 += gridDim.x * ZPB) {
-        const int zw = z0 + wave * ZPW + gi;
+    const int nunits = (zend - zbeg + ZPB - 1) / ZPB;
+    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+        const int zw = zbeg + w.u * ZPB + wave * ZPW + gi;
         const bool live = (gi < ZPW) && (zw < zend);
         const int z = live ? zw : zbeg;               // idle lanes read a valid zone
         const double zdthyd = corner_pred<NS>(a, g, z, sbeg + (z - zbeg) * NS + g.k, live, dt, bad);
@@ -547,10 +580,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
         for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
             const int k = c0 + threadIdx.x;
             if (k < ke) {
-                const int s = a.pcside[k];
-                sm[threadIdx.x] = a.cmass[s];
-                sx[threadIdx.x] = a.cfx[s];
-                sy[threadIdx.x] = a.cfy[s];
+                sm[threadIdx.x] = a.cmass[k];
+                sx[threadIdx.x] = a.cfx[k];
+                sy[threadIdx.x] = a.cfy[k];
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -572,9 +604,8 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
     double mass = 0.;
     d2 f = mk(0., 0.);
     for (int k = kb; k < ke; ++k) {
-        const int s = a.pcside[k];
-        mass += a.cmass[s];
-        f = add(f, mk(a.cfx[s], a.cfy[s]));
+        mass += a.cmass[k];
+        f = add(f, mk(a.cfx[k], a.cfy[k]));
     }
     point_update(a, p, mass, f, dt);
 }
@@ -589,7 +620,10 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
     const double dth = 0.5 * dt;
     double dvov = 1.e-99;
     int bad = 0;
-    for (int z = zbeg + blockIdx.x * PN_BLOCK + threadIdx.x; z < zend; z += gridDim.x * PN_BLOCK) {
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    for (Units wu = block_units(nunits, a.xcd); wu.u < wu.end; wu.u += wu.step) {
+        const int z = zbeg + wu.u * PN_BLOCK + threadIdx.x;
+        if (z >= zend) continue;
         const int sb = sbeg + (z - zbeg) * NS;
         const double n = (double)NS;
         int pp[NS];
@@ -732,7 +766,7 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npre
 FusedArgs make_args(pennant_problem* pb) {
     FusedArgs a;
     const Params& q = pb->prm;
-    a.mapsp1 = pb->dm.mapsp1; a.pcbeg = pb->dm.pcbeg; a.pcside = pb->dm.pcside;
+    a.mapsp1 = pb->dm.mapsp1; a.pcbeg = pb->dm.pcbeg; a.pcpos = pb->pcpos;
     a.smf = pb->f("smf"); a.nump = pb->dm.nump;
     a.pxx = pb->f("px", 0); a.pxy = pb->f("px", 1); a.pux = pb->f("pu", 0); a.puy = pb->f("pu", 1);
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
@@ -747,6 +781,7 @@ FusedArgs make_args(pennant_problem* pb) {
     a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
     a.nbcs = pb->nbcs;
     for (int i = 0; i < 8; ++i) { a.bcvx[i] = pb->bcvx[i]; a.bcvy[i] = pb->bcvy[i]; }
+    a.xcd = pb->tune.xcd;
     return a;
 }
 
@@ -828,6 +863,14 @@ int fused_setup(pennant_problem* pb) {
     pb->pbc = (unsigned char*)al(P, 1);
     pb->heavy = (int*)al(heavy.size(), 4);
     pb->nheavy = (int)heavy.size();
+    // K1 writes each corner's force and mass straight into its point's CSR
+    // slot, so that K2 reads every point's corners as one contiguous run
+    std::vector<int> pcpos((size_t)h.nums);
+    for (int k = 0; k < h.nums; ++k) pcpos[h.pcside[k]] = k;
+    pb->pcpos = (int*)al(S, 4);
+    if (pb->pcpos &&
+        hipMemcpy(pb->pcpos, pcpos.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return (int)hipErrorUnknown;
     // partial slots: one per block of every K1 (resp. K3) launch of a cycle,
     // at the largest grid any tuning allows
     const int nruns = (int)pb->zruns.size() / 3;
@@ -836,7 +879,7 @@ int fused_setup(pennant_problem* pb) {
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
     if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
-        !pb->pxpx || !pb->pbc || !pb->heavy || !pb->part_d || !pb->part_i)
+        !pb->pxpx || !pb->pbc || !pb->heavy || !pb->pcpos || !pb->part_d || !pb->part_i)
         return (int)hipErrorOutOfMemory;
     if (!heavy.empty() &&
         hipMemcpy(pb->heavy, heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice) != hipSuccess)

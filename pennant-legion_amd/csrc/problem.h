@@ -36,6 +36,7 @@ struct Ctl {
 struct Tuning {
     int k1_layout = 0;       // 0 zone per thread, 1 corner per lane
     int max_blocks = 2048;   // grid cap of the grid-stride K1/K3 launches
+    int xcd = 1;             // XCD-aware unit mapping of the grid-stride launches
 };
 
 struct Params {
@@ -88,6 +89,7 @@ struct pennant_problem {
     double* part_d = nullptr;        // block partials: dt minima, then dvol maxima
     int* part_i = nullptr;           // block partials: bad-side counts (pred, corr)
     int npart_pred = 0, npart_corr = 0;
+    int* pcpos = nullptr;            // side -> slot of its corner in the point-corner CSR
     int* heavy = nullptr;            // points with more corners than a K2 thread takes
     int nheavy = 0;
     pn::Tuning tune;

@@ -118,6 +118,8 @@ DRIVER_SIGNATURES = {
     "pennant_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_set_tuning": ([_P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "pennant_bench_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                           ctypes.c_int),
     "pennant_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
 }
 

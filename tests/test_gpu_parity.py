@@ -121,10 +121,12 @@ def test_cycle_parity(deck, mode):
     pb.close()
 
 
-@pytest.mark.parametrize("k1,maxb", [(0, 1), (0, 7), (1, 1), (1, 5), (1, 4096)])
-def test_fused_launch_shapes_do_not_change_results(k1, maxb):
-    """Grid-stride zone kernels with tiny and large grids, both K1 layouts,
-    on a pie mesh (a triangle run and a quad run) and a rect mesh."""
+@pytest.mark.parametrize("k1,maxb,xcd", [(0, 1, 1), (0, 7, 1), (0, 8, 1), (0, 16, 0), (1, 1, 1),
+                                         (1, 5, 1), (1, 24, 1), (1, 4096, 0)])
+def test_fused_launch_shapes_do_not_change_results(k1, maxb, xcd):
+    """Grid-stride zone kernels with tiny and large grids, with and without
+    the XCD band mapping, both K1 layouts, on a pie mesh (a triangle run and a
+    quad run) and a rect mesh."""
     for text, ncyc in ((wd.noh_pie(40, 24, cstop=12, dtinit=1e-4), 12),
                        (wd.sedov_rect(37, cstop=12), 12)):
         o = Oracle(text)
@@ -132,6 +134,7 @@ def test_fused_launch_shapes_do_not_change_results(k1, maxb):
         with pa.Problem(text) as pb:
             pb.set_tuning("k1_layout", k1)
             pb.set_tuning("max_blocks", maxb)
+            pb.set_tuning("xcd", xcd)
             assert pb.run(100, pa.MODE_FUSED) == ncyc
             for f in STATE:
                 assert_bits_equal(f"{f} k1={k1} maxb={maxb}", pb.get(f), o.get(f))
