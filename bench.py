@@ -53,16 +53,33 @@ def kernel_bytes(S: int, Z: int, P: int):
     every distinct input element read once, every output written once
     (int32 maps, FP64 values)."""
     return {
-        # mapsp1 4, smf 8, cf.x/cf.y/cmass/sfw.x/sfw.y 40 per side; zm zr ze
-        # zwrate zvol 40 + zp 8 per zone; px, pu 32 per point
-        "zone_pred": 52 * S + 48 * Z + 32 * P,
-        # pcside 4, cmass 8, cf 16 per corner; pcbeg 4, px/pu 32 in + 32 out,
-        # bc bits 1, vs 16 + pxp.x 8 out per point
-        "point": 28 * S + 93 * P,
+        # mapsp1 4, pcpos 4, smf 8, cf.x/cf.y/cmass/sfw.x/sfw.y 40 per side;
+        # zm zr ze zwrate zvol 40 + zp 8 per zone; px, pu 32 per point
+        "zone_pred": 56 * S + 48 * Z + 32 * P,
+        # cmass 8, cf 16 per corner; pcbeg 4, px/pu 32 in + 32 out, bc bits 1,
+        # vs 16 + pxp.x 8 out per point
+        "point": 24 * S + 93 * P,
         # mapsp1 4, sfw 16 per side; zetot zvol zm zp 32 in, zwrate zetot ze
         # zr zvol 40 out per zone; px 16, vs 16, pxp.x 8 per point
         "zone_corr": 20 * S + 72 * Z + 40 * P,
     }
+
+
+PROFILE_KERNELS = {"zone_pred": "k_pred_zone<4>", "point": "k_point", "zone_corr": "k_corr<4>"}
+
+
+def profiled_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = ROOT / "profiles" / f"fused_fp64_{workload}.md"
+    if not path.exists() or kernel not in PROFILE_KERNELS:
+        return None
+    name = f"`{PROFILE_KERNELS[kernel]}`"
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 8 and cells[0] == name:
+            return int((float(cells[3]) + float(cells[4])) * 1e6)
+    return None
 
 
 def cpu_baseline(deck_text: str, numz: int, budget_s: float):
@@ -102,7 +119,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs "
-              "on a single device; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "on a single device)", file=sys.stderr)
         sys.exit(2)
 
     import pennant_amd as pa
@@ -140,11 +157,17 @@ def main():
                 (f"  {kb[n] / 1e6:8.1f} MB  {gbs:8.1f} GB/s" if n in kb else ""))
         dom = max(("zone_pred", "point", "zone_corr"), key=lambda n: per[n])
         achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
+        # the reference's task decomposition moves 688 S + 536 Z + 512 P bytes
+        # per cycle; at the fused cycle's rate that is this effective bandwidth
+        b_tasks = 688 * pb.nums + 536 * pb.numz + 512 * pb.nump
+        traffic = (profiled_traffic(args.workload, dom)
+                   if args.mode == "fused" and args.k1 in (None, 0) else None)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None,
+                    "traffic": traffic,
                     "alg_bytes_per_launch": kb[dom], "avg_launch_us": round(per[dom] * 1e3, 2),
-                    "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1)}
+                    "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1),
+                    "task_decomposition_effective_GBs": round(b_tasks / (ms_per_step * 1e-3) / 1e9, 1)}
         if args.copy_mb > 0:
             import ctypes
             ms = ctypes.c_double()

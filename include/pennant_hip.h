@@ -8,7 +8,7 @@
  *     task ID (see the citation on each declaration) and takes exactly the
  *     fields of the task's region requirements, as device pointers, in the
  *     reference's positional order.  A Legion HIP task wrapper unpacks its
- *     accessors with acc.ptr(rect) and forwards them (INTEGRATION.md).
+ *     accessors with acc.ptr(rect) and forwards them.
  *
  *  2. Driver entry points, pennant_*(): a whole-problem driver (deck parsing,
  *     mesh generation, initialisation, the cycle loop with dt kept on the

@@ -16,7 +16,7 @@ Rows 15 and above of the golden carry spurious bands (energies up to 2e-5
 and density dips of 0.2% at rows 17-18 and 26-28, point displacements of
 2e-5 at point row 18) in an e=0 region that round-off level elsewhere in the
 same file shows to be quiescent; they are not reproducible from the deck
-and are excluded from the comparison (see DESIGN.md).
+and are excluded from the comparison.
 """
 import numpy as np
 
