@@ -48,10 +48,17 @@ def workload_deck(name: str, decks):
     raise SystemExit(f"unknown workload {name}")
 
 
-def kernel_bytes(S: int, Z: int, P: int):
+def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False):
     """Algorithmic (compulsory) HBM bytes per launch of each fused kernel:
     every distinct input element read once, every output written once
-    (int32 maps, FP64 values)."""
+    (int32 maps, FP64 values; FP32 side and corner arrays in the mixed form)."""
+    if mixed:
+        return {
+            # mapsp1 4, pcpos 4, smf 4, cf 8, cmass 4, sfw 8 per side
+            "zone_pred": 32 * S + 48 * Z + 32 * P,
+            "point": 12 * S + 93 * P,
+            "zone_corr": 12 * S + 72 * Z + 40 * P,
+        }
     return {
         # mapsp1 4, pcpos 4, smf 8, cf.x/cf.y/cmass/sfw.x/sfw.y 40 per side;
         # zm zr ze zwrate zvol 40 + zp 8 per zone; px, pu 32 per point
@@ -65,16 +72,20 @@ def kernel_bytes(S: int, Z: int, P: int):
     }
 
 
-PROFILE_KERNELS = {"zone_pred": "k_pred_zone<4>", "point": "k_point", "zone_corr": "k_corr<4>"}
+PROFILE_KERNELS = {
+    "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
+    "mixed": {"zone_pred": "k_pred_mixed<4>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
+}
 
 
-def profiled_traffic(workload: str, kernel: str):
+def profiled_traffic(workload: str, kernel: str, form: str = "fp64"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
-    path = ROOT / "profiles" / f"fused_fp64_{workload}.md"
-    if not path.exists() or kernel not in PROFILE_KERNELS:
+    path = ROOT / "profiles" / f"fused_{form}_{workload}.md"
+    kmap = PROFILE_KERNELS[form]
+    if not path.exists() or kernel not in kmap:
         return None
-    name = f"`{PROFILE_KERNELS[kernel]}`"
+    name = f"`{kmap[kernel]}`"
     for line in path.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) == 8 and cells[0] == name:
@@ -108,7 +119,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sedov1440")
-    ap.add_argument("--mode", default="fused", choices=["fused", "tasks"])
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fused", "tasks"],
+                    help="mixed: fused cycle in mixed precision (reported against the plain FP64 "
+                         "fused cycle on the same workload); fused: plain FP64 fused cycle; "
+                         "tasks: one launch per task")
     ap.add_argument("--copy-mb", type=int, default=512,
                     help="size of the streaming-copy calibration (0 disables)")
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
@@ -124,31 +138,60 @@ def main():
 
     import pennant_amd as pa
     from pennant_amd import decks
+    from pennant_amd.accuracy import field_errors
 
     text, wname = workload_deck(args.workload, decks)
-    mode = pa.MODE_FUSED if args.mode == "fused" else pa.MODE_TASKS
-    t0 = time.perf_counter()
-    pb = pa.Problem(text)
-    if args.k1 is not None:
-        pb.set_tuning("k1_layout", args.k1)
-    if args.max_blocks is not None:
-        pb.set_tuning("max_blocks", args.max_blocks)
-    if args.xcd is not None:
-        pb.set_tuning("xcd", args.xcd)
-    log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
-    if args.warmup > 0:
-        pb.run(args.warmup, mode)
-    total_ms, phase_ms = pb.time_cycles(args.steps, mode, phases=(mode == pa.MODE_FUSED))
-    sc = pb.scalars()
-    assert sc["cycle"] == args.warmup + args.steps, sc
+    modes = {"mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED, "tasks": pa.MODE_TASKS}
+
+    def measure(mode):
+        t0 = time.perf_counter()
+        pb = pa.Problem(text)
+        if args.k1 is not None:
+            pb.set_tuning("k1_layout", args.k1)
+        if args.max_blocks is not None:
+            pb.set_tuning("max_blocks", args.max_blocks)
+        if args.xcd is not None:
+            pb.set_tuning("xcd", args.xcd)
+        log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
+        if args.warmup > 0:
+            pb.run(args.warmup, mode)
+        total_ms, phase_ms = pb.time_cycles(args.steps, mode, phases=(mode != pa.MODE_TASKS))
+        sc = pb.scalars()
+        assert sc["cycle"] == args.warmup + args.steps, sc
+        state = {f: pb.get(f) for f in ("zr", "ze", "zp")}
+        dims = (pb.numz, pb.nums, pb.nump)
+        pb.close()
+        return total_ms, phase_ms, sc, state, dims
+
+    mode = modes[args.mode]
+    total_ms, phase_ms, sc, state, (Z, S, P) = measure(mode)
     ms_per_step = total_ms / args.steps
-    value = pb.numz * args.steps / (total_ms * 1e-3)
+    value = Z * args.steps / (total_ms * 1e-3)
     log(f"{args.mode}: {args.steps} cycles in {total_ms:.3f} ms -> {ms_per_step * 1e3:.1f} us/cycle, "
         f"{value:.4e} zone-cycles/s; t={sc['time']:.6e}")
 
+    accuracy = None
+    if mode == pa.MODE_MIXED:
+        # the plain FP64 form on the same workload, same cycle count: its rate
+        # is the denominator of the speedup and its state the accuracy reference
+        p_ms, _, p_sc, p_state, _ = measure(pa.MODE_FUSED)
+        plain = Z * args.steps / (p_ms * 1e-3)
+        err = field_errors(state, p_state)
+        log(f"plain fp64 fused: {p_ms / args.steps * 1e3:.1f} us/cycle, {plain:.4e} zone-cycles/s; "
+            f"speedup {value / plain:.3f}; " +
+            " ".join(f"{k} L1 {v[0]:.2e} Linf {v[1]:.2e}" for k, v in err.items()))
+        accuracy = {"reference": "plain FP64 fused form (bit-identical to the CPU oracle)",
+                    "cycles": args.warmup + args.steps,
+                    "time": sc["time"], "time_plain": p_sc["time"],
+                    "rel_l1": {k: float(f"{v[0]:.3e}") for k, v in err.items()},
+                    "rel_linf": {k: float(f"{v[1]:.3e}") for k, v in err.items()},
+                    "plain_fp64_value": round(plain, 1),
+                    "speedup_vs_plain_fp64": round(value / plain, 4)}
+
     roofline = None
     if phase_ms is not None:
-        kb = kernel_bytes(pb.nums, pb.numz, pb.nump)
+        mixed = (mode == pa.MODE_MIXED)
+        kb = kernel_bytes(S, Z, P, mixed)
         names = ("zone_pred", "point", "zone_corr", "finalize")
         per = {n: phase_ms[i] / args.steps for i, n in enumerate(names)}
         for n in names:
@@ -159,9 +202,9 @@ def main():
         achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
         # the reference's task decomposition moves 688 S + 536 Z + 512 P bytes
         # per cycle; at the fused cycle's rate that is this effective bandwidth
-        b_tasks = 688 * pb.nums + 536 * pb.numz + 512 * pb.nump
-        traffic = (profiled_traffic(args.workload, dom)
-                   if args.mode == "fused" and args.k1 in (None, 0) else None)
+        b_tasks = 688 * S + 536 * Z + 512 * P
+        traffic = (profiled_traffic(args.workload, dom, "mixed" if mixed else "fp64")
+                   if args.k1 in (None, 0) else None)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
@@ -179,7 +222,7 @@ def main():
 
     cpu = None
     if args.cpu_budget > 0:
-        res, threads = cpu_baseline(text, pb.numz, args.cpu_budget)
+        res, threads = cpu_baseline(text, Z, args.cpu_budget)
         v_omp, n_omp, t_omp = res["omp"]
         v_ser, n_ser, t_ser = res["serial"]
         log(f"cpu oracle serial {v_ser:.3e} zone-cycles/s ({n_ser} cycles, {t_ser:.1f}s); "
@@ -188,17 +231,19 @@ def main():
                "sample": f"{n_omp} cycles of {wname} after setup (OpenMP variant); serial "
                          f"variant {v_ser:.4g} zone-cycles/s on 1 core over {n_ser} cycles",
                "serial_value": round(v_ser, 1)}
-    pb.close()
 
+    form = {"mixed": "mixed-fp32-qcs-fused", "fused": "plain-fp64-fused", "tasks": "plain-fp64-tasks"}
     line = {
         "metric": BASELINE_METRIC,
         "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": wname, "form": f"plain-fp64-{args.mode}", "zones": pb.numz,
-                   "sides": pb.nums, "points": pb.nump},
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if mode == pa.MODE_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": wname, "form": form[args.mode], "zones": Z, "sides": S, "points": P},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if accuracy is not None:
+        line["accuracy"] = accuracy
     print(json.dumps(line), flush=True)
 
 

@@ -263,7 +263,9 @@ typedef struct pennant_problem pennant_problem;
 
 enum pennant_mode {
     PENNANT_MODE_TASKS = 0,   /* one launch per task ID, src/Hydro.cc:342-983 order */
-    PENNANT_MODE_FUSED = 1    /* fused cycle kernels (same arithmetic, fewer HBM passes) */
+    PENNANT_MODE_FUSED = 1,   /* fused cycle kernels (same arithmetic, fewer HBM passes) */
+    PENNANT_MODE_MIXED = 2    /* fused cycle, mixed precision: FP32 side/corner/QCS math,
+                                 FP64 positions, volumes, EOS state, energy, work and dt */
 };
 
 /* Parses a .pnt deck (src/InputFile.cc:24-104 syntax), generates the mesh
@@ -283,8 +285,10 @@ int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode);
  * pennant_error) or a hipError_t. */
 int pennant_sync(pennant_problem* pb);
 
-/* Convenience: launch_cycles + sync, returns the number of cycles run or a
- * negative error code. */
+/* Runs up to ncycles cycles and waits for them; returns the number of cycles
+ * run or a negative error code.  Cycles are launched in chunks (32 growing to
+ * 1024) with a stream sync after each, so a run that stops at cstop or tstop
+ * long before ncycles does not queue the remaining (no-op) launches. */
 int pennant_run(pennant_problem* pb, int ncycles, int mode);
 
 const char* pennant_error(pennant_problem* pb);
@@ -359,6 +363,21 @@ int pennant_get_tuning(pennant_problem* pb, const char* key);
 
 /* Writes <path> in the reference's .xy format (src/WriteXY.cc:28-53). */
 int pennant_write_xy(pennant_problem* pb, const char* path);
+
+/* Writes the EnSight Gold set <base>.case, <base>.geo, <base>.zr, <base>.ze,
+ * <base>.zp of the problem's current state (cycle, time, px, zr, ze, zp), in
+ * the reference's format (src/ExportGold.cc:31-307, called from
+ * Mesh::writeTask, src/Mesh.cc:1570-1604). */
+int pennant_write_gold(pennant_problem* pb, const char* base);
+
+/* The same writer on host arrays (no device needed): px/py are the point
+ * coordinates, znump and mapsp1 the mesh maps (zone z's sides are contiguous,
+ * starting at the sum of znump over the zones before z).  Returns 0, or
+ * PENNANT_EINVAL when a file cannot be opened. */
+int pennant_write_gold_host(const char* base, int32_t cycle, double time, const double* zr,
+                            const double* ze, const double* zp, const int32_t* znump,
+                            int32_t numz, const double* px, const double* py, int32_t nump,
+                            const int32_t* mapsp1);
 
 /* Host-only view of the deck and generated mesh (no device needed): the same
  * parsing, error messages and numbering pennant_create uses.  Names for

@@ -452,8 +452,9 @@ void pennant_sizes(pennant_problem* pb, int64_t* out4) {
 }
 
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
-    if (mode == PENNANT_MODE_FUSED) {
-        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr));
+    if (mode == PENNANT_MODE_FUSED || mode == PENNANT_MODE_MIXED) {
+        const bool mixed = (mode == PENNANT_MODE_MIXED);
+        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr, mixed));
         return 0;
     }
     if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
@@ -471,7 +472,8 @@ int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms) {
     if (ncycles < 1) return PENNANT_EINVAL;
-    const bool phases = (mode == PENNANT_MODE_FUSED && phase_ms);
+    const bool mixed = (mode == PENNANT_MODE_MIXED);
+    const bool phases = ((mode == PENNANT_MODE_FUSED || mixed) && phase_ms);
     // per cycle: events before K1, K2, K3, K4 and after K4
     const size_t E = 5;
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
@@ -479,7 +481,7 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     int rc = (int)hipEventRecord(ev[0], pb->stream);
     if (!rc) {
         if (phases) {
-            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k]);
+            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k], mixed);
         } else {
             rc = pennant_launch_cycles(pb, ncycles, mode);
         }
@@ -536,8 +538,14 @@ int pennant_sync(pennant_problem* pb) {
 int pennant_run(pennant_problem* pb, int ncycles, int mode) {
     CK(copy_ctl_to_host(pb));
     const int c0 = pb->ctl_host->cycle;
-    CK(pennant_launch_cycles(pb, ncycles, mode));
-    CK(pennant_sync(pb));
+    int left = ncycles, chunk = 32;
+    while (left > 0 && !pb->ctl_host->done) {
+        const int k = left < chunk ? left : chunk;
+        CK(pennant_launch_cycles(pb, k, mode));
+        CK(pennant_sync(pb));
+        left -= k;
+        chunk = chunk < 1024 ? 2 * chunk : 1024;
+    }
     return pb->ctl_host->cycle - c0;
 }
 
@@ -700,6 +708,91 @@ int pennant_write_xy(pennant_problem* pb, const char* path) {
     }
     std::fclose(f);
     return 0;
+}
+
+// EnSight Gold export, the file set of src/ExportGold.cc:31-307: <base>.case,
+// <base>.geo (coordinates; then triangles, quads and n-sided zones, each group
+// listing its zone ids and its corner points, 1-based) and one per-element
+// scalar file per zone field.  The reference writes through iostreams in
+// scientific notation; the printf conversions below produce the same text
+// (setw(w) << int is %wd, setprecision(p) << double is %w.pe).
+int pennant_write_gold_host(const char* base, int32_t cycle, double time, const double* zr,
+                            const double* ze, const double* zp, const int32_t* znump,
+                            int32_t numz, const double* px, const double* py, int32_t nump,
+                            const int32_t* mapsp1) {
+    if (!base || numz < 0 || nump < 0) return PENNANT_EINVAL;
+    const std::string b(base);
+    // zones grouped by size (ExportGold::sortZones), with their first side
+    std::vector<int> tris, quads, others;
+    std::vector<int64_t> zs(numz);
+    int64_t sc = 0;
+    for (int z = 0; z < numz; ++z) {
+        if (znump[z] == 3) tris.push_back(z);
+        else if (znump[z] == 4) quads.push_back(z);
+        else others.push_back(z);
+        zs[z] = sc;
+        sc += znump[z];
+    }
+    FILE* f = std::fopen((b + ".case").c_str(), "w");
+    if (!f) return PENNANT_EINVAL;
+    std::fprintf(f, "#\n# Created by PENNANT\n#\nFORMAT\ntype: ensight gold\nGEOMETRY\nmodel: %s.geo\n"
+                    "VARIABLE\nscalar per element: zr %s.zr\nscalar per element: ze %s.ze\n"
+                    "scalar per element: zp %s.zp\n", base, base, base, base);
+    std::fclose(f);
+
+    f = std::fopen((b + ".geo").c_str(), "w");
+    if (!f) return PENNANT_EINVAL;
+    std::fprintf(f, "cycle = %8d\nt = %15.8e\nnode id assign\nelement id given\npart\n%10d\n"
+                    "universe\ncoordinates\n%10d\n", cycle, time, 1, nump);
+    for (int p = 0; p < nump; ++p) std::fprintf(f, "%12.5e\n", px[p]);
+    for (int p = 0; p < nump; ++p) std::fprintf(f, "%12.5e\n", py[p]);
+    for (int p = 0; p < nump; ++p) std::fprintf(f, "%12.5e\n", 0.);
+    auto group = [&](const char* kind, const std::vector<int>& zl, bool nsided) {
+        if (zl.empty()) return;
+        std::fprintf(f, "%s\n%10d\n", kind, (int)zl.size());
+        for (int z : zl) std::fprintf(f, "%10d\n", z + 1);
+        if (nsided)
+            for (int z : zl) std::fprintf(f, "%10d\n", znump[z]);
+        for (int z : zl) {
+            for (int i = 0; i < znump[z]; ++i) std::fprintf(f, "%10d", mapsp1[zs[z] + i] + 1);
+            std::fputc('\n', f);
+        }
+    };
+    group("tria3", tris, false);
+    group("quad4", quads, false);
+    group("nsided", others, true);
+    std::fclose(f);
+
+    const char* names[3] = {"zr", "ze", "zp"};
+    const double* vars[3] = {zr, ze, zp};
+    for (int k = 0; k < 3; ++k) {
+        f = std::fopen((b + "." + names[k]).c_str(), "w");
+        if (!f) return PENNANT_EINVAL;
+        std::fprintf(f, "%s\npart\n%10d\n", names[k], 1);
+        const char* kinds[3] = {"tria3", "quad4", "nsided"};
+        const std::vector<int>* lists[3] = {&tris, &quads, &others};
+        for (int g = 0; g < 3; ++g) {
+            if (lists[g]->empty()) continue;
+            std::fprintf(f, "%s\n", kinds[g]);
+            for (int z : *lists[g]) std::fprintf(f, "%12.5e\n", vars[k][z]);
+        }
+        std::fclose(f);
+    }
+    return 0;
+}
+
+int pennant_write_gold(pennant_problem* pb, const char* base) {
+    const int nz = pb->dm.numz, np = pb->dm.nump;
+    std::vector<double> zr(nz), ze(nz), zp(nz), px(np), py(np);
+    CK(pennant_get_field(pb, "zr", zr.data()));
+    CK(pennant_get_field(pb, "ze", ze.data()));
+    CK(pennant_get_field(pb, "zp", zp.data()));
+    CK(hipMemcpy(px.data(), pb->f("px", 0), sizeof(double) * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(py.data(), pb->f("px", 1), sizeof(double) * np, hipMemcpyDeviceToHost));
+    CK(copy_ctl_to_host(pb));
+    return pennant_write_gold_host(base, pb->ctl_host->cycle, pb->ctl_host->time, zr.data(),
+                                   ze.data(), zp.data(), pb->hm.znump.data(), nz, px.data(),
+                                   py.data(), np, pb->hm.mapsp1.data());
 }
 
 }  // extern "C"

@@ -66,6 +66,9 @@ struct FusedArgs {
     const double* zm;
     // per-side intermediates
     double* cfx; double* cfy; double* cmass; double* sfwx; double* sfwy;
+    // the mixed-precision form's FP32 counterparts, and smf in FP32
+    float* fcfx; float* fcfy; float* fcmass; float* fsfwx; float* fsfwy;
+    const float* fsmf;
     // per-point intermediates
     double* vsx; double* vsy; double* pxpx;
     const unsigned char* pbc;
@@ -322,6 +325,229 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
 }
 
 // ---------------------------------------------------------------------------
+// K1, mixed precision (zone per thread).
+//
+// FP64: positions and their midpoints, the zone centre, side and zone areas
+// and volumes (cross products of nearly equal coordinates), the zone EOS with
+// its (zvolp - zvol0) difference, the TTS density difference srho - r, the
+// corner-force difference tot[k] - tot[k-1], the corner mass, and the dt
+// candidate.  FP32: edge lengths, the per-side pressure/TTS force products
+// and the whole QCS chain (corner divergence, evolution factor, Kurapatenko
+// coefficient, corner weights, edge viscous force).  Coordinate and velocity
+// differences are formed in FP64 and rounded once.  Contraction into fused
+// multiply-adds is allowed here.
+
+struct f2 {
+    float x, y;
+};
+__device__ __forceinline__ f2 fmk(float x, float y) { f2 v; v.x = x; v.y = y; return v; }
+__device__ __forceinline__ f2 tof(d2 v) { return fmk((float)v.x, (float)v.y); }
+__device__ __forceinline__ f2 fadd(f2 a, f2 b) { return fmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ f2 fscl(float r, f2 v) { return fmk(v.x * r, v.y * r); }
+__device__ __forceinline__ float fdot(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ float fcross(f2 a, f2 b) { return a.x * b.y - a.y * b.x; }
+__device__ __forceinline__ float flen(f2 v) { return __builtin_sqrtf(v.x * v.x + v.y * v.y); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
+
+template <int NS>
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
+                                                  int& bad) {
+#pragma clang fp contract(fast)
+    const double dth = 0.5 * dt;
+    const double n = (double)NS;
+    d2 u0[NS], xp[NS];
+    float mf[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int p = a.mapsp1[sb + k];
+        const d2 x0 = mk(a.pxx[p], a.pxy[p]);
+        u0[k] = mk(a.pux[p], a.puy[p]);
+        xp[k] = add(x0, scl(dth, u0[k]));
+        mf[k] = a.fsmf[sb + k];
+    }
+    // zone-mean velocity; velocities from here on live in FP32 relative to it
+    d2 zuc = mk(0., 0.);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zuc = add(zuc, dvs(u0[k], n));
+    f2 ul[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ul[k] = tof(sub(u0[k], zuc));
+    d2 zx = mk(0., 0.);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zx = add(zx, dvs(xp[k], n));
+    const double third = 1. / 3.;
+    double sa[NS];
+    float el[NS];
+    double zarea = 0., zvolp = 0.;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const d2 p1 = xp[k], p2 = xp[nxt<NS>(k)];
+        const d2 e = sub(p2, p1);
+        sa[k] = 0.5 * cross(e, sub(zx, p1));
+        const double sv = third * sa[k] * (p1.x + p2.x + zx.x);
+        zarea += sa[k];
+        zvolp += sv;
+        if (sv <= 0.) bad += 1;
+        el[k] = flen(tof(e));
+    }
+    // positions from here on live in FP32 relative to the zone centre (which
+    // is the origin of this frame); edge midpoints follow from them
+    f2 xl[NS], exl[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) xl[k] = tof(sub(xp[k], zx));
+#pragma unroll
+    for (int k = 0; k < NS; ++k) exl[k] = fscl(0.5f, fadd(xl[k], xl[nxt<NS>(k)]));
+    const float fac = (NS == 3 ? 3.f : 4.f);
+    float zdl = 1.e30f;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zdl = fminr(zdl, fac * (float)sa[k] * frcp(el[k]));
+    // zone EOS in FP64
+    const double zm = a.zm[z];
+    const double zrp = zm / zvolp;
+    const double gm1 = a.gamma - 1.;
+    const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
+    const double r = a.zr[z];
+    const double e = smax(a.ze[z], 0.);
+    const double pz = gm1 * r * e;
+    const double pre = gm1 * e;
+    const double per = gm1 * r;
+    const double csqd = smax(ssmin2, pre + per * pz / (r * r));
+    const double zss = __builtin_sqrt(csqd);
+    const double minv = 1. / zm;
+    const double dv = (zvolp - a.zvol[z]) * minv;
+    const double bulk = r * csqd;
+    const double denom = 1. + 0.5 * per * dv;
+    const double src = a.zwrate[z] * dth * minv;
+    const double zp = pz + (per * src - r * bulk * dv) / denom;
+    a.zp[z] = zp;
+    // pressure and TTS side forces
+    const float zpf = (float)zp;
+    double sstmp = smax(zss, a.ssmin);
+    sstmp = a.alfa * sstmp * sstmp;
+    f2 sfp[NS], sft[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const f2 ss = fmk(-exl[k].y, exl[k].x);                       // rotCCW(ex - zx)
+        sfp[k] = fscl(-zpf, ss);
+        const double drho = zrp * (double)mf[k] * (zarea / sa[k]) - zrp;   // srho - r
+        sft[k] = fscl(-(float)(sstmp * drho), ss);
+    }
+    // QCS corner quantities (corner k: xp0 = x[k], xp1 = ex[k], xp2 = zx,
+    // xp3 = ex[k-1]; up2 = zuc is the frame's zero velocity)
+    const f2 zucf = tof(zuc);
+    float carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k), kp = nxt<NS>(k);
+        const f2 u1 = fscl(0.5f, fadd(ul[k], ul[kp]));                 // up1 - zuc
+        const f2 u3 = fscl(0.5f, fadd(ul[km], ul[k]));                 // up3 - zuc
+        const f2 A = fmk(-xl[k].x, -xl[k].y);                           // xp2 - xp0
+        const f2 B = fmk(exl[km].x - exl[k].x, exl[km].y - exl[k].y);   // xp3 - xp1
+        const f2 v1 = fmk(exl[km].x - xl[k].x, exl[km].y - xl[k].y);    // xp3 - xp0
+        const f2 v2 = fmk(exl[k].x - xl[k].x, exl[k].y - xl[k].y);      // xp1 - xp0
+        const f2 dxx1 = fscl(0.5f, fmk(exl[k].x - xl[k].x - exl[km].x, exl[k].y - xl[k].y - exl[km].y));
+        const f2 dxx2 = fscl(0.5f, fmk(exl[km].x - xl[k].x - exl[k].x, exl[km].y - xl[k].y - exl[k].y));
+        const f2 du20 = fmk(-ul[k].x, -ul[k].y);                        // up2 - up0
+        const f2 du31 = fmk(u3.x - u1.x, u3.y - u1.y);                  // up3 - up1
+        const f2 dva = fmk(u1.x - ul[k].x - u3.x, u1.y - ul[k].y - u3.y);   // up1+up2-up0-up3
+        const f2 dvb = fmk(u3.x - ul[k].x - u1.x, u3.y - ul[k].y - u1.y);   // up2+up3-up0-up1
+        const f2 duav = fadd(zucf, fscl(0.25f, fadd(fadd(ul[k], u1), u3)));
+        const float cvol = 0.5f * fcross(A, B);
+        carea[k] = cvol;
+        const float de1 = el[km], de2 = el[k];
+        const float minelen = fminr(de1, de2);
+        ccos[k] = (minelen < 1.e-12f) ? 0.f : 4.f * fdot(v1, v2) * frcp(de1 * de2);
+        const float div = (fcross(du20, B) - fcross(du31, A)) * frcp(2.f * cvol);
+        cdiv[k] = div;
+        const float dx1 = flen(dxx1), dx2 = flen(dxx2);
+        const float t1 = fabsf(fdot(dxx1, duav) * dx2);
+        const float t2 = fabsf(fdot(dxx2, duav) * dx1);
+        const float num = (t1 > t2) ? dx1 : dx2;
+        const float den = (t1 > t2) ? dx2 : dx1;
+        float evol = __builtin_sqrtf(4.f * cvol * num * frcp(den));
+        evol = fminr(evol, 2.f * minelen);
+        const float du = __builtin_sqrtf(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
+        cevol[k] = (div < 0.f) ? evol : 0.f;
+        cdu[k] = (div < 0.f) ? du : 0.f;
+    }
+    const float gq = (float)(a.q2 * 0.25 * (a.qgamma + 1.0));
+    const float zt1 = (float)(a.q1 * zss);
+    const float zrpf = (float)zrp;
+    f2 cqe1[NS], cqe2[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k), kp = nxt<NS>(k);
+        const float zt2 = gq * cdu[k];
+        const float zkur = zt2 + __builtin_sqrtf(zt2 * zt2 + zt1 * zt1);
+        const float crmu = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
+        cqe1[k] = fscl(crmu * frcp(el[km]), fmk(ul[k].x - ul[km].x, ul[k].y - ul[km].y));
+        cqe2[k] = fscl(crmu * frcp(el[k]), fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y));
+    }
+    float cw[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const float csin2 = 1.f - ccos[k] * ccos[k];
+        cw[k] = (csin2 < 1.e-4f) ? 0.f : carea[k] * frcp(csin2);
+        ccos[k] = (csin2 < 1.e-4f) ? 0.f : ccos[k];
+    }
+    d2 tot[NS];
+    float dux_max = 0.f;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int kp = nxt<NS>(k);
+        const f2 fq = fadd(fscl(cw[k], fadd(cqe2[k], fscl(ccos[k], cqe1[k]))),
+                           fscl(cw[kp], fadd(cqe1[kp], fscl(ccos[kp], cqe2[kp]))));
+        const f2 sfq = fscl(frcp(el[k]), fq);
+        const f2 sfw = fadd(sfp[k], sfq);
+        a.fsfwx[sb + k] = sfw.x;
+        a.fsfwy[sb + k] = sfw.y;
+        tot[k] = mk((double)sfw.x + (double)sft[k].x, (double)sfw.y + (double)sft[k].y);
+        // setVelDiff
+        const f2 du = fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y);
+        const f2 dx = fmk(xl[kp].x - xl[k].x, xl[kp].y - xl[k].y);
+        const float dux = fabsf(fdot(du, dx)) * frcp(el[k]);
+        dux_max = fmaxr(dux_max, (el[k] > 0.f) ? dux : 0.f);
+    }
+    const double zdu = a.q1 * zss + 2. * a.q2 * (double)dux_max;
+    const double mzz = zrp * zarea;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int km = prv<NS>(k);
+        const int slot = a.pcpos[sb + k];
+        const d2 cf = sub(tot[k], tot[km]);
+        a.fcfx[slot] = (float)cf.x;
+        a.fcfy[slot] = (float)cf.y;
+        a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+    }
+    const double cdt = smax(zdu, smax(zss, 1.e-99));
+    return (double)zdl * a.cfl / cdt;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(PN_BLOCK) k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                         int poff) {
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    double dtmin = 1.e99;
+    int bad = 0;
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+        const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
+        if (z >= zend) continue;
+        const double zdthyd = zone_pred_mixed<NS>(a, z, sbeg + (z - zbeg) * NS, dt, bad);
+        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+    }
+    const double m = block_min(dtmin);
+    const int b = block_sum(bad);
+    if (threadIdx.x == 0) {
+        a.part_dt[poff + blockIdx.x] = m;
+        a.part_bp[poff + blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K1, corner-per-lane.
 
 // Corner group of a lane: the NS lanes holding one zone's corners.
@@ -564,11 +790,31 @@ __device__ __forceinline__ void point_update(const FusedArgs& a, int p, double m
     a.puy[p] = u.y;
 }
 
+// Corner values in point-corner CSR order: FP64 arrays for the plain form,
+// FP32 arrays for the mixed-precision form (summed in FP64 either way).
+template <typename T> struct Corners;
+template <> struct Corners<double> {
+    __device__ static double mass(const FusedArgs& a, int k) { return a.cmass[k]; }
+    __device__ static double fx(const FusedArgs& a, int k) { return a.cfx[k]; }
+    __device__ static double fy(const FusedArgs& a, int k) { return a.cfy[k]; }
+    __device__ static double wx(const FusedArgs& a, int s) { return a.sfwx[s]; }
+    __device__ static double wy(const FusedArgs& a, int s) { return a.sfwy[s]; }
+};
+template <> struct Corners<float> {
+    __device__ static double mass(const FusedArgs& a, int k) { return (double)a.fcmass[k]; }
+    __device__ static double fx(const FusedArgs& a, int k) { return (double)a.fcfx[k]; }
+    __device__ static double fy(const FusedArgs& a, int k) { return (double)a.fcfy[k]; }
+    __device__ static double wx(const FusedArgs& a, int s) { return (double)a.fsfwx[s]; }
+    __device__ static double wy(const FusedArgs& a, int s) { return (double)a.fsfwy[s]; }
+};
+
 // Point gathers, fixed BCs, acceleration and position update.  Blocks
 // [0, nheavy) each take one high-valence point (the pie origin has one
 // corner per centre triangle): the block stages the corner values in LDS and
 // one thread adds them in ascending order.
+template <typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+    using C = Corners<T>;
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     if ((int)blockIdx.x < a.nheavy) {
@@ -580,9 +826,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
         for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
             const int k = c0 + threadIdx.x;
             if (k < ke) {
-                sm[threadIdx.x] = a.cmass[k];
-                sx[threadIdx.x] = a.cfx[k];
-                sy[threadIdx.x] = a.cfy[k];
+                sm[threadIdx.x] = C::mass(a, k);
+                sx[threadIdx.x] = C::fx(a, k);
+                sy[threadIdx.x] = C::fy(a, k);
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -604,17 +850,18 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
     double mass = 0.;
     d2 f = mk(0., 0.);
     for (int k = kb; k < ke; ++k) {
-        mass += a.cmass[k];
-        f = add(f, mk(a.cfx[k], a.cfy[k]));
+        mass += C::mass(a, k);
+        f = add(f, mk(C::fx(a, k), C::fy(a, k)));
     }
     point_update(a, p, mass, f, dt);
 }
 
 // ---------------------------------------------------------------------------
 // K3: corrector geometry, work, work rate, energy, density, dvol.
-template <int NS>
+template <int NS, typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int zend, int sbeg,
                                                    int poff) {
+    using C = Corners<T>;
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     const double dth = 0.5 * dt;
@@ -651,7 +898,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const int p1 = pp[k], p2 = pp[nxt<NS>(k)];
-            const d2 sftot = mk(a.sfwx[sb + k], a.sfwy[sb + k]);
+            const d2 sftot = mk(C::wx(a, sb + k), C::wy(a, sb + k));
             const double sd1 = dot(sftot, mk(a.vsx[p1], a.vsy[p1]));
             const double sd2 = dot(neg(sftot), mk(a.vsx[p2], a.vsy[p2]));
             const double dwork = -dth * (sd1 * a.pxpx[p1] + sd2 * a.pxpx[p2]);
@@ -772,6 +1019,8 @@ FusedArgs make_args(pennant_problem* pb) {
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
     a.cfx = pb->cfx; a.cfy = pb->cfy; a.cmass = pb->cmass; a.sfwx = pb->sfwx; a.sfwy = pb->sfwy;
+    a.fcfx = pb->fcfx; a.fcfy = pb->fcfy; a.fcmass = pb->fcmass; a.fsfwx = pb->fsfwx;
+    a.fsfwy = pb->fsfwy; a.fsmf = pb->fsmf;
     a.vsx = pb->vsx; a.vsy = pb->vsy; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
     a.heavy = pb->heavy; a.nheavy = pb->nheavy;
     a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->npart_pred;
@@ -785,11 +1034,16 @@ FusedArgs make_args(pennant_problem* pb) {
     return a;
 }
 
+__global__ void __launch_bounds__(PN_BLOCK) k_to_float(int n, const double* src, float* dst) {
+    const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
 int grid_for(int needed, int cap) { return needed < 1 ? 1 : (needed < cap ? needed : cap); }
 
 // blocks of each K1 / K3 launch for a run of nz zones with NS sides
-int pred_grid(const pennant_problem* pb, int nz, int ns) {
-    const int per = (pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
+int pred_grid(const pennant_problem* pb, int nz, int ns, bool mixed) {
+    const int per = (!mixed && pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
     return grid_for((nz + per - 1) / per, pb->tune.max_blocks);
 }
 int corr_grid(const pennant_problem* pb, int nz) {
@@ -798,18 +1052,23 @@ int corr_grid(const pennant_problem* pb, int nz) {
 
 template <int NS>
 void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
-                 hipStream_t s) {
-    const dim3 grid(pred_grid(pb, ze - zb, NS));
-    if (pb->tune.k1_layout == 1)
+                 bool mixed, hipStream_t s) {
+    const dim3 grid(pred_grid(pb, ze - zb, NS, mixed));
+    if (mixed)
+        hipLaunchKernelGGL(k_pred_mixed<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else if (pb->tune.k1_layout == 1)
         hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else
         hipLaunchKernelGGL(k_pred_zone<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
 }
 template <int NS>
 void launch_corr(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
-                 hipStream_t s) {
-    hipLaunchKernelGGL(k_corr<NS>, dim3(corr_grid(pb, ze - zb)), dim3(PN_BLOCK), 0, s, a, zb, ze, sb,
-                       poff);
+                 bool mixed, hipStream_t s) {
+    const dim3 grid(corr_grid(pb, ze - zb));
+    if (mixed)
+        hipLaunchKernelGGL((k_corr<NS, float>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else
+        hipLaunchKernelGGL((k_corr<NS, double>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
 }
 
 }  // namespace
@@ -863,6 +1122,16 @@ int fused_setup(pennant_problem* pb) {
     pb->pbc = (unsigned char*)al(P, 1);
     pb->heavy = (int*)al(heavy.size(), 4);
     pb->nheavy = (int)heavy.size();
+    // mixed-precision intermediates and the FP32 copy of smf
+    pb->fcfx = (float*)al(S, 4); pb->fcfy = (float*)al(S, 4); pb->fcmass = (float*)al(S, 4);
+    pb->fsfwx = (float*)al(S, 4); pb->fsfwy = (float*)al(S, 4);
+    float* fsmf = (float*)al(S, 4);
+    pb->fsmf = fsmf;
+    if (!pb->fcfx || !pb->fcfy || !pb->fcmass || !pb->fsfwx || !pb->fsfwy || !fsmf)
+        return (int)hipErrorOutOfMemory;
+    if (S > 0)
+        hipLaunchKernelGGL(k_to_float, dim3(nblocks((int64_t)S)), dim3(PN_BLOCK), 0, pb->stream,
+                           (int)S, (const double*)pb->f("smf"), fsmf);
     // K1 writes each corner's force and mass straight into its point's CSR
     // slot, so that K2 reads every point's corners as one contiguous run
     std::vector<int> pcpos((size_t)h.nums);
@@ -887,7 +1156,7 @@ int fused_setup(pennant_problem* pb) {
     return (int)hipMemcpy(pb->pbc, bits.data(), bits.size(), hipMemcpyHostToDevice);
 }
 
-int fused_cycle(pennant_problem* pb, hipEvent_t* ev) {
+int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed) {
     const FusedArgs a = make_args(pb);
     const Params& q = pb->prm;
     hipStream_t s = pb->stream;
@@ -898,31 +1167,35 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, s); break;
-            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, s); break;
-            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, s); break;
-            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, s); break;
-            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, s); break;
-            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, s); break;
+            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, mixed, s); break;
             default: return PENNANT_EINVAL;
         }
-        poff += pred_grid(pb, ze - zb, ns);
+        poff += pred_grid(pb, ze - zb, ns, mixed);
     }
     const int npred = poff;
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_point, dim3(pb->nheavy + nblocks(h.nump)), dim3(PN_BLOCK), 0, s, a);
+    const dim3 pgrid(pb->nheavy + nblocks(h.nump));
+    if (mixed)
+        hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_point<double>, pgrid, dim3(PN_BLOCK), 0, s, a);
     if (ev) (void)hipEventRecord(ev[2], s);
     poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_corr<3>(pb, a, zb, ze, sb, poff, s); break;
-            case 4: launch_corr<4>(pb, a, zb, ze, sb, poff, s); break;
-            case 5: launch_corr<5>(pb, a, zb, ze, sb, poff, s); break;
-            case 6: launch_corr<6>(pb, a, zb, ze, sb, poff, s); break;
-            case 7: launch_corr<7>(pb, a, zb, ze, sb, poff, s); break;
-            case 8: launch_corr<8>(pb, a, zb, ze, sb, poff, s); break;
+            case 3: launch_corr<3>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 4: launch_corr<4>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 5: launch_corr<5>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 6: launch_corr<6>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 7: launch_corr<7>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 8: launch_corr<8>(pb, a, zb, ze, sb, poff, mixed, s); break;
             default: return PENNANT_EINVAL;
         }
         poff += corr_grid(pb, ze - zb);

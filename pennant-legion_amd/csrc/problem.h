@@ -85,6 +85,8 @@ struct pennant_problem {
     double* cfx = nullptr; double* cfy = nullptr; double* cmass = nullptr;
     double* sfwx = nullptr; double* sfwy = nullptr;
     double* vsx = nullptr; double* vsy = nullptr; double* pxpx = nullptr;
+    float* fcfx = nullptr; float* fcfy = nullptr; float* fcmass = nullptr;   // mixed form
+    float* fsfwx = nullptr; float* fsfwy = nullptr; const float* fsmf = nullptr;
     unsigned char* pbc = nullptr;    // per-point BC membership bits (one per list)
     double* part_d = nullptr;        // block partials: dt minima, then dvol maxima
     int* part_i = nullptr;           // block partials: bad-side counts (pred, corr)
@@ -106,7 +108,8 @@ namespace pn {
 // fused cycle (fused.hip)
 int fused_setup(pennant_problem* pb);
 // ev: NULL, or 5 events recorded before K1, K2, K3, K4 and after K4
-int fused_cycle(pennant_problem* pb, hipEvent_t* ev);
+// mixed: the mixed-precision form instead of the plain FP64 one
+int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);
 int launch_end_cycle(pennant_problem* pb, int compute_dtrec);

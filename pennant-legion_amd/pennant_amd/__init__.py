@@ -20,6 +20,7 @@ LIB_PATH = _HERE / "libpennant_hip.so"
 
 MODE_TASKS = 0
 MODE_FUSED = 1
+MODE_MIXED = 2     # fused, FP32 side/corner/QCS math, FP64 state, volumes, energy and dt
 EBADSIDES = -100
 EINVAL = -22
 
@@ -110,6 +111,9 @@ DRIVER_SIGNATURES = {
     "pennant_step": ([_P, ctypes.c_int], ctypes.c_int),
     "pennant_end_cycle": ([_P], ctypes.c_int),
     "pennant_write_xy": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_write_gold": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_write_gold_host": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, _P, _P, _P, _P,
+                                 ctypes.c_int32, _P, _P, ctypes.c_int32, _P], ctypes.c_int),
     "pennant_hip_reduce_workspace_bytes": ([], ctypes.c_size_t),
     "pennant_host_mesh_create": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
     "pennant_host_mesh_get": ([_P, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
@@ -236,7 +240,7 @@ class Problem:
         tot = ctypes.c_double()
         ph = (ctypes.c_double * 4)() if phases else None
         self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
-        return tot.value, (list(ph) if phases and mode == MODE_FUSED else None)
+        return tot.value, (list(ph) if phases and mode in (MODE_FUSED, MODE_MIXED) else None)
 
     def set_tuning(self, key: str, value: int):
         if lib().pennant_set_tuning(self._h, key.encode(), int(value)) != 0:
@@ -304,3 +308,19 @@ class Problem:
 
     def write_xy(self, path):
         self._check(lib().pennant_write_xy(self._h, str(path).encode()))
+
+    def write_gold(self, base):
+        """EnSight Gold files <base>.case/.geo/.zr/.ze/.zp of the current state."""
+        self._check(lib().pennant_write_gold(self._h, str(base).encode()))
+
+
+def write_gold_host(base, cycle: int, time: float, zr, ze, zp, znump, px, py, mapsp1):
+    """The EnSight Gold writer on host arrays (no device needed)."""
+    f64 = [np.ascontiguousarray(a, dtype=np.float64) for a in (zr, ze, zp, px, py)]
+    i32 = [np.ascontiguousarray(a, dtype=np.int32) for a in (znump, mapsp1)]
+    rc = lib().pennant_write_gold_host(str(base).encode(), int(cycle), float(time),
+                                       f64[0].ctypes.data, f64[1].ctypes.data, f64[2].ctypes.data,
+                                       i32[0].ctypes.data, int(i32[0].size), f64[3].ctypes.data,
+                                       f64[4].ctypes.data, int(f64[3].size), i32[1].ctypes.data)
+    if rc != 0:
+        raise PennantError(f"cannot write {base}.*")

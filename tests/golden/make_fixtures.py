@@ -3,7 +3,7 @@ output set) into tests/golden/ so that tests never read the reference tree at
 run time.  These are data files (decks: key/value text; golden: printed
 field values), copied unchanged.
 
-Provenance: test/<deck>/<deck>.pnt and test/sedov/sedov.{xy,geo} of
+Provenance: test/<deck>/<deck>.pnt and test/sedov/sedov.{xy,geo,case,zr,ze,zp} of
 StanfordLegion/pennant-legion.
 """
 import shutil
@@ -22,7 +22,7 @@ def main():
     for d in DECKS:
         shutil.copyfile(REF / "test" / d / f"{d}.pnt", HERE / "decks" / f"{d}.pnt")
     (HERE / "sedov").mkdir(exist_ok=True)
-    for ext in ("xy", "geo"):
+    for ext in ("xy", "geo", "case", "zr", "ze", "zp"):
         shutil.copyfile(REF / "test" / "sedov" / f"sedov.{ext}", HERE / "sedov" / f"sedov.{ext}")
 
 

@@ -152,6 +152,26 @@ def test_sedov_xy_identical_to_oracle(tmp_path):
     assert (tmp_path / "gpu.xy").read_text() == (tmp_path / "ora.xy").read_text()
 
 
+@pytest.mark.parametrize("deck", ["sedov", "nohsmall"])
+def test_gold_export_identical_to_oracle(tmp_path, deck):
+    """EnSight Gold files of the device state equal those written from the
+    oracle's state byte for byte (the host writer itself is checked against
+    the reference's sedov.{case,geo,zr,ze,zp} by the CPU tests)."""
+    text = ref_deck(deck)
+    o = Oracle(text)
+    o.run(1000)
+    hm = pa.HostMesh(text)
+    sc = o.scalars()
+    px = o.get("px")
+    pa.write_gold_host(tmp_path / "ora", int(sc["cycle"]), sc["time"], o.get("zr"), o.get("ze"),
+                       o.get("zp"), hm.get("znump"), px[:, 0], px[:, 1], hm.get("mapsp1"))
+    with pa.Problem(text) as pb:
+        pb.run(1000, pa.MODE_FUSED)
+        pb.write_gold(tmp_path / "gpu")
+    for ext in ("geo", "zr", "ze", "zp"):
+        assert (tmp_path / f"gpu.{ext}").read_bytes() == (tmp_path / f"ora.{ext}").read_bytes(), ext
+
+
 def test_run_to_tstop_in_chunks():
     # cycles queued past the stop condition do nothing (device predicate)
     text = wd.sedov_rect(40, tstop=3e-4)
