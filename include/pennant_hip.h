@@ -264,8 +264,10 @@ typedef struct pennant_problem pennant_problem;
 enum pennant_mode {
     PENNANT_MODE_TASKS = 0,   /* one launch per task ID, src/Hydro.cc:342-983 order */
     PENNANT_MODE_FUSED = 1,   /* fused cycle kernels (same arithmetic, fewer HBM passes) */
-    PENNANT_MODE_MIXED = 2    /* fused cycle, mixed precision: FP32 side/corner/QCS math,
+    PENNANT_MODE_MIXED = 2,   /* fused cycle, mixed precision: FP32 side/corner/QCS math,
                                  FP64 positions, volumes, EOS state, energy, work and dt */
+    PENNANT_MODE_LEARNED = 3  /* the mixed form with the learned QCS coefficient
+                                 (pennant_set_closure; exact QCS outside its envelope) */
 };
 
 /* Parses a .pnt deck (src/InputFile.cc:24-104 syntax), generates the mesh
@@ -387,6 +389,66 @@ typedef struct pennant_host_mesh pennant_host_mesh;
 pennant_host_mesh* pennant_host_mesh_create(const char* deck_text, char* errbuf, int errlen);
 int pennant_host_mesh_get(pennant_host_mesh* hm, const char* name, void* host_out, int64_t* count);
 void pennant_host_mesh_destroy(pennant_host_mesh* hm);
+
+/* ======================================================================== */
+/* Learned QCS closure.  No counterpart in the reference: it replaces the    */
+/* Kurapatenko coefficient of setQCnForce (src/QCS.cc:285-296) by a small    */
+/* network, 10 per-corner features -> 64 -> 64 -> 2 coefficients, BF16      */
+/* weights and activations with FP32 accumulation (MFMA).  The features,     */
+/* outputs and training targets are defined in csrc/closure.h.               */
+
+typedef struct pennant_mlp pennant_mlp;
+
+/* Number of FP32 parameters: W1[64][16] (input columns 0-9 features, 10 the
+ * bias, 11-15 zero), W2[64][64], b2[64], W3[2][64], b3[2]. */
+int pennant_mlp_num_params(void);
+/* He-uniform weights from `seed`, zero biases, an unbounded envelope. */
+pennant_mlp* pennant_mlp_create(uint64_t seed);
+void pennant_mlp_destroy(pennant_mlp* m);
+int pennant_mlp_set_params(pennant_mlp* m, const float* host, int n);   /* resets Adam */
+int pennant_mlp_get_params(pennant_mlp* m, float* host, int n);
+/* Feature envelope: a zone whose compressing corners have a feature outside
+ * [lo[k], hi[k]] uses the exact QCS coefficient instead of the network. */
+int pennant_mlp_set_envelope(pennant_mlp* m, const float* lo10, const float* hi10);
+int pennant_mlp_get_envelope(pennant_mlp* m, float* lo10, float* hi10);
+/* Batched inference on device arrays x (n x 10) -> y (n x 2).  path 0: the
+ * MFMA GEMM kernels used by training; path 1: the wave-level routine the
+ * fused predictor kernel runs. */
+int pennant_mlp_forward(pennant_mlp* m, const float* x, int64_t n, float* y, int path);
+/* Loss (mean over the n tuples of the summed squared output error) and its
+ * gradient with respect to every parameter, on device tuples (n x 12: 10
+ * features, 2 targets); grad is a host array of pennant_mlp_num_params(). */
+int pennant_mlp_loss_grad(pennant_mlp* m, const float* tuples, int64_t n, float* grad, float* loss);
+/* `steps` Adam steps (lr, beta 0.9/0.999) on minibatches of `batch` tuples
+ * drawn with replacement from the device tuples by a generator seeded with
+ * `seed`; loss_out (host, may be NULL) receives each step's minibatch loss. */
+int pennant_mlp_train(pennant_mlp* m, const float* tuples, int64_t n, int steps, int batch,
+                      float lr, uint64_t seed, float* loss_out);
+
+/* Attaches a trained network to a problem (NULL detaches); cycles in
+ * PENNANT_MODE_LEARNED use it.  The problem keeps a pointer: the network
+ * must outlive the problem's learned cycles. */
+int pennant_set_closure(pennant_problem* pb, pennant_mlp* m);
+/* Counters of learned cycles since the problem was created: out[0] zones
+ * with compressing corners evaluated by the network, out[1] such zones that
+ * fell back to the exact coefficient (a feature outside the envelope). */
+int pennant_closure_stats(pennant_problem* pb, int64_t* out2);
+
+/* Training data: while a capture buffer is set (device, cap x 12 floats),
+ * every PENNANT_MODE_MIXED cycle appends one tuple (features, exact target
+ * coefficients) per compressing corner at its predictor half step; tuples
+ * past `cap` are counted but not stored.  The order of the tuples within a
+ * cycle is not fixed.  pennant_set_capture(pb, NULL, 0) stops capturing. */
+int pennant_set_capture(pennant_problem* pb, float* dev_buf, int64_t cap);
+/* Tuples produced since the capture buffer was set (may exceed cap). */
+int pennant_capture_count(pennant_problem* pb, int64_t* count);
+
+/* Total energy of the current state (reported, not used by the cycle):
+ * internal = sum_z zetot; kinetic = sum over corners of
+ * zm * (zarea * x_p * (smf_s + smf_s3) / 2) / zvol * |u_p|^2 / 2, the
+ * corner-volume-weighted kinetic energy of each zone (RZ volumes per radian,
+ * as zm and zvol). */
+int pennant_energy(pennant_problem* pb, double* internal, double* kinetic);
 
 #ifdef __cplusplus
 }

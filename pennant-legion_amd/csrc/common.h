@@ -65,8 +65,11 @@ __device__ __forceinline__ int wave_sum(int v) {
 }
 
 // Block-wide reductions for PN_BLOCK threads; the result is valid in thread 0.
+// Each starts with a barrier, so that calling one twice in a row cannot
+// overwrite a partial that thread 0 is still reading.
 __device__ __forceinline__ double block_min(double v) {
     __shared__ double part[PN_BLOCK / 64];
+    __syncthreads();
     v = wave_min(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) part[w] = v;
@@ -78,6 +81,7 @@ __device__ __forceinline__ double block_min(double v) {
 }
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double part[PN_BLOCK / 64];
+    __syncthreads();
     v = wave_max(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) part[w] = v;
@@ -89,6 +93,7 @@ __device__ __forceinline__ double block_max(double v) {
 }
 __device__ __forceinline__ int block_sum(int v) {
     __shared__ int part[PN_BLOCK / 64];
+    __syncthreads();
     v = wave_sum(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) part[w] = v;

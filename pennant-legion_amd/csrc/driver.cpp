@@ -451,10 +451,20 @@ void pennant_sizes(pennant_problem* pb, int64_t* out4) {
     out4[0] = pb->dm.nump; out4[1] = pb->dm.numz; out4[2] = pb->dm.nums; out4[3] = (int64_t)pb->bcs.size();
 }
 
+// fused-cycle form of a driver mode, or -1 for task mode / unknown modes
+static int fused_form(int mode) {
+    switch (mode) {
+        case PENNANT_MODE_FUSED: return FORM_FP64;
+        case PENNANT_MODE_MIXED: return FORM_MIXED;
+        case PENNANT_MODE_LEARNED: return FORM_LEARNED;
+        default: return -1;
+    }
+}
+
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
-    if (mode == PENNANT_MODE_FUSED || mode == PENNANT_MODE_MIXED) {
-        const bool mixed = (mode == PENNANT_MODE_MIXED);
-        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr, mixed));
+    const int form = fused_form(mode);
+    if (form >= 0) {
+        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr, form));
         return 0;
     }
     if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
@@ -472,8 +482,8 @@ int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms) {
     if (ncycles < 1) return PENNANT_EINVAL;
-    const bool mixed = (mode == PENNANT_MODE_MIXED);
-    const bool phases = ((mode == PENNANT_MODE_FUSED || mixed) && phase_ms);
+    const int form = fused_form(mode);
+    const bool phases = (form >= 0 && phase_ms);
     // per cycle: events before K1, K2, K3, K4 and after K4
     const size_t E = 5;
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
@@ -481,7 +491,7 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     int rc = (int)hipEventRecord(ev[0], pb->stream);
     if (!rc) {
         if (phases) {
-            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k], mixed);
+            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k], form);
         } else {
             rc = pennant_launch_cycles(pb, ncycles, mode);
         }
@@ -778,6 +788,78 @@ int pennant_write_gold_host(const char* base, int32_t cycle, double time, const 
         }
         std::fclose(f);
     }
+    return 0;
+}
+
+const void* pennant_mlp_device_pack(pennant_mlp* m);
+
+int pennant_set_closure(pennant_problem* pb, pennant_mlp* m) {
+    pb->closure = m ? pennant_mlp_device_pack(m) : nullptr;
+    return 0;
+}
+
+int pennant_closure_stats(pennant_problem* pb, int64_t* out2) {
+    unsigned long long c[3];
+    CK(hipMemcpyAsync(c, pb->cl_counters, sizeof c, hipMemcpyDeviceToHost, pb->stream));
+    CK(hipStreamSynchronize(pb->stream));
+    out2[0] = (int64_t)c[1];
+    out2[1] = (int64_t)c[2];
+    return 0;
+}
+
+int pennant_set_capture(pennant_problem* pb, float* dev_buf, int64_t cap) {
+    if (cap < 0 || (dev_buf == nullptr) != (cap == 0)) return PENNANT_EINVAL;
+    CK(hipStreamSynchronize(pb->stream));
+    pb->cap_buf = dev_buf;
+    pb->cap = cap;
+    const unsigned long long zero = 0;
+    return (int)hipMemcpy(pb->cl_counters, &zero, sizeof zero, hipMemcpyHostToDevice);
+}
+
+int pennant_capture_count(pennant_problem* pb, int64_t* count) {
+    unsigned long long c = 0;
+    CK(hipMemcpyAsync(&c, pb->cl_counters, sizeof c, hipMemcpyDeviceToHost, pb->stream));
+    CK(hipStreamSynchronize(pb->stream));
+    *count = (int64_t)c;
+    return 0;
+}
+
+int pennant_energy(pennant_problem* pb, double* internal, double* kinetic) {
+    const HostMesh& h = pb->hm;
+    const int nz = h.numz, np = h.nump, ns = h.nums;
+    std::vector<double> zetot(nz), zm(nz), zvol(nz), px(np), pux(np), puy(np), smf(ns);
+    CK(pennant_get_field(pb, "zetot", zetot.data()));
+    CK(pennant_get_field(pb, "zm", zm.data()));
+    CK(pennant_get_field(pb, "zvol", zvol.data()));
+    CK(pennant_get_field(pb, "smf", smf.data()));
+    CK(hipMemcpy(px.data(), pb->f("px", 0), sizeof(double) * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(pux.data(), pb->f("pu", 0), sizeof(double) * np, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(puy.data(), pb->f("pu", 1), sizeof(double) * np, hipMemcpyDeviceToHost));
+    // planar zone areas of the current positions
+    std::vector<double> py(np);
+    CK(hipMemcpy(py.data(), pb->f("px", 1), sizeof(double) * np, hipMemcpyDeviceToHost));
+    double ei = 0., ek = 0.;
+    for (int z = 0; z < nz; ++z) ei += zetot[z];
+    for (int z = 0; z < nz; ++z) {
+        const int sb = h.zsbeg[z], n = h.znump[z];
+        double cx = 0., cy = 0.;
+        for (int k = 0; k < n; ++k) {
+            cx += px[h.mapsp1[sb + k]] / n;
+            cy += py[h.mapsp1[sb + k]] / n;
+        }
+        double zarea = 0.;
+        for (int k = 0; k < n; ++k) {
+            const int p1 = h.mapsp1[sb + k], p2 = h.mapsp2[sb + k];
+            zarea += 0.5 * ((px[p2] - px[p1]) * (cy - py[p1]) - (py[p2] - py[p1]) * (cx - px[p1]));
+        }
+        for (int k = 0; k < n; ++k) {
+            const int s = sb + k, s3 = h.mapss3[s], p = h.mapsp1[s];
+            const double cvol = zarea * px[p] * 0.5 * (smf[s] + smf[s3]);
+            ek += zm[z] * cvol / zvol[z] * 0.5 * (pux[p] * pux[p] + puy[p] * puy[p]);
+        }
+    }
+    *internal = ei;
+    *kinetic = ek;
     return 0;
 }
 

@@ -43,6 +43,7 @@
 // one partial per reduction (no same-address atomics on the hot path).  The
 // host launches K1 and K3 once per run of consecutive zones with equal side
 // count (rect: one run of quads; pie: triangles, then quads).
+#include "closure.h"
 #include "common.h"
 #include "problem.h"
 
@@ -82,6 +83,12 @@ struct FusedArgs {
     int nbcs;
     double bcvx[8], bcvy[8];
     int xcd;   // XCD-aware unit mapping in the grid-stride kernels
+    // learned closure and training-tuple capture
+    const ClosurePack* pack;
+    float* cap_buf;
+    long long cap;
+    unsigned long long* cap_count;
+    unsigned long long* cl_stats;
 };
 
 // Work units of a grid-stride launch.  MI355X dispatches workgroups to its 8
@@ -351,9 +358,24 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
-template <int NS>
+// Closure kinds of the mixed K1: the exact QCS coefficient, the learned one
+// (exact outside the network's envelope), or the exact one while recording
+// training tuples.
+enum { CL_EXACT = 0, CL_LEARNED = 1, CL_CAPTURE = 2 };
+
+// Zone counters of the learned form, summed per thread.
+struct ClCount {
+    int learned = 0;    // zones with compressing corners evaluated by the network
+    int fallback = 0;   // such zones outside the envelope (exact coefficient)
+};
+
+// The mixed-precision predictor half of zone z (sides sb..sb+NS-1).  With
+// CL != CL_EXACT it must be called by all 64 lanes of a wave together (the
+// network runs across the wave); `valid` false marks a lane that only takes
+// part in the wave-wide steps and stores nothing.
+template <int NS, int CL>
 __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
-                                                  int& bad) {
+                                                  int& bad, bool valid, ClCount& cnt) {
 #pragma clang fp contract(fast)
     const double dth = 0.5 * dt;
     const double n = (double)NS;
@@ -389,7 +411,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const double sv = third * sa[k] * (p1.x + p2.x + zx.x);
         zarea += sa[k];
         zvolp += sv;
-        if (sv <= 0.) bad += 1;
+        if (valid && sv <= 0.) bad += 1;
         el[k] = flen(tof(e));
     }
     // positions from here on live in FP32 relative to the zone centre (which
@@ -421,7 +443,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     const double denom = 1. + 0.5 * per * dv;
     const double src = a.zwrate[z] * dth * minv;
     const double zp = pz + (per * src - r * bulk * dv) / denom;
-    a.zp[z] = zp;
+    if (valid) a.zp[z] = zp;
     // pressure and TTS side forces
     const float zpf = (float)zp;
     double sstmp = smax(zss, a.ssmin);
@@ -437,7 +459,16 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     // QCS corner quantities (corner k: xp0 = x[k], xp1 = ex[k], xp2 = zx,
     // xp3 = ex[k-1]; up2 = zuc is the frame's zero velocity)
     const f2 zucf = tof(zuc);
+    const float gq = (float)(a.q2 * 0.25 * (a.qgamma + 1.0));
+    const float q1f = (float)a.q1;
+    const float zssf = (float)zss;
+    const float zt1 = (float)(a.q1 * zss);
+    const float zrpf = (float)zrp;
     float carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
+    u32x4 flo[NS];            // packed network inputs, slots 0-7
+    unsigned fhi[NS];         // slots 8-9
+    bool inenv = true;        // every compressing corner inside the envelope
+    bool comp = false;        // some corner compressing
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int km = prv<NS>(k), kp = nxt<NS>(k);
@@ -471,19 +502,89 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const float du = __builtin_sqrtf(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
         cevol[k] = (div < 0.f) ? evol : 0.f;
         cdu[k] = (div < 0.f) ? du : 0.f;
+        if constexpr (CL != CL_EXACT) {
+            // network features of this corner (closure.h)
+            const float inv = 1.f / zssf;
+            float f[PN_MLP_NF];
+            f[0] = (ul[k].x - ul[km].x) * inv;
+            f[1] = (ul[k].y - ul[km].y) * inv;
+            f[2] = (ul[kp].x - ul[k].x) * inv;
+            f[3] = (ul[kp].y - ul[k].y) * inv;
+            f[4] = div * evol * inv;
+            f[5] = evol / minelen;
+            f[6] = zssf / (zssf + flen(duav));
+            f[7] = zrpf / (float)r;
+            f[8] = du * inv;
+            const float tl = q1f * zssf, tq = gq * du;
+            f[9] = (tl + tq > 0.f) ? tq / (tl + tq) : 0.f;
+            if (div < 0.f) {
+                comp = true;
+                if constexpr (CL == CL_LEARNED) {
+#pragma unroll
+                    for (int j = 0; j < PN_MLP_NF; ++j)
+                        inenv = inenv && (f[j] >= a.pack->lo[j]) && (f[j] <= a.pack->hi[j]);
+                }
+            }
+            if constexpr (CL == CL_LEARNED) {
+                u32x4 hi;
+                pack_features(f, flo[k], hi);
+                fhi[k] = hi[0];
+            } else {
+                // a training tuple: features and the exact coefficients
+                const uint64_t m = __ballot(valid && div < 0.f);
+                if (m) {
+                    unsigned long long base = 0;
+                    if ((threadIdx.x & 63) == 0) base = atomicAdd(a.cap_count, (unsigned long long)__popcll(m));
+                    base = __shfl(base, 0, 64);
+                    const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
+                    const unsigned long long slot = base + (unsigned long long)__popcll(below);
+                    if (valid && div < 0.f && slot < (unsigned long long)a.cap) {
+                        const float s2 = __builtin_sqrtf(tl * tl + tq * tq);
+                        float* o = a.cap_buf + slot * PN_MLP_TUPLE;
+#pragma unroll
+                        for (int j = 0; j < PN_MLP_NF; ++j) o[j] = f[j];
+                        o[PN_MLP_NF] = (s2 > 0.f) ? tl / s2 : 1.f;
+                        o[PN_MLP_NF + 1] = (s2 > 0.f) ? 1.f + tq / s2 : 1.f;
+                    }
+                }
+            }
+        }
     }
-    const float gq = (float)(a.q2 * 0.25 * (a.qgamma + 1.0));
-    const float zt1 = (float)(a.q1 * zss);
-    const float zrpf = (float)zrp;
     f2 cqe1[NS], cqe2[NS];
+    float crmu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const float zt2 = gq * cdu[k];
+        const float zkur = zt2 + __builtin_sqrtf(zt2 * zt2 + zt1 * zt1);
+        crmu[k] = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
+    }
+    if constexpr (CL == CL_LEARNED) {
+        const bool use = valid && comp && inenv;
+        if (valid && comp) {
+            if (inenv) cnt.learned += 1;
+            else cnt.fallback += 1;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const bool need = use && cdiv[k] < 0.f;
+            const uint64_t m = __ballot(need);
+            if (m) {
+                u32x4 hi;
+                hi[0] = fhi[k];
+                hi[1] = pack_bf16(1.f, 0.f);
+                hi[2] = 0u;
+                hi[3] = 0u;
+                float y0, y1;
+                mlp_wave(a.pack, flo[k], hi, (uint32_t)m != 0u, (m >> 32) != 0u, y0, y1);
+                if (need) crmu[k] = (q1f * y0 * zssf + gq * y1 * cdu[k]) * zrpf * cevol[k];
+            }
+        }
+    }
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int km = prv<NS>(k), kp = nxt<NS>(k);
-        const float zt2 = gq * cdu[k];
-        const float zkur = zt2 + __builtin_sqrtf(zt2 * zt2 + zt1 * zt1);
-        const float crmu = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
-        cqe1[k] = fscl(crmu * frcp(el[km]), fmk(ul[k].x - ul[km].x, ul[k].y - ul[km].y));
-        cqe2[k] = fscl(crmu * frcp(el[k]), fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y));
+        cqe1[k] = fscl(crmu[k] * frcp(el[km]), fmk(ul[k].x - ul[km].x, ul[k].y - ul[km].y));
+        cqe2[k] = fscl(crmu[k] * frcp(el[k]), fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y));
     }
     float cw[NS];
 #pragma unroll
@@ -501,8 +602,10 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
                            fscl(cw[kp], fadd(cqe1[kp], fscl(ccos[kp], cqe2[kp]))));
         const f2 sfq = fscl(frcp(el[k]), fq);
         const f2 sfw = fadd(sfp[k], sfq);
-        a.fsfwx[sb + k] = sfw.x;
-        a.fsfwy[sb + k] = sfw.y;
+        if (valid) {
+            a.fsfwx[sb + k] = sfw.x;
+            a.fsfwy[sb + k] = sfw.y;
+        }
         tot[k] = mk((double)sfw.x + (double)sft[k].x, (double)sfw.y + (double)sft[k].y);
         // setVelDiff
         const f2 du = fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y);
@@ -512,14 +615,16 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     }
     const double zdu = a.q1 * zss + 2. * a.q2 * (double)dux_max;
     const double mzz = zrp * zarea;
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int km = prv<NS>(k);
-        const int slot = a.pcpos[sb + k];
-        const d2 cf = sub(tot[k], tot[km]);
-        a.fcfx[slot] = (float)cf.x;
-        a.fcfy[slot] = (float)cf.y;
-        a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+        for (int k = 0; k < NS; ++k) {
+            const int km = prv<NS>(k);
+            const int slot = a.pcpos[sb + k];
+            const d2 cf = sub(tot[k], tot[km]);
+            a.fcfx[slot] = (float)cf.x;
+            a.fcfy[slot] = (float)cf.y;
+            a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+        }
     }
     const double cdt = smax(zdu, smax(zss, 1.e-99));
     return (double)zdl * a.cfl / cdt;
@@ -532,15 +637,53 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_mixed(FusedArgs a, int zbeg, 
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
     int bad = 0;
+    ClCount cnt;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
     for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
         const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
         if (z >= zend) continue;
-        const double zdthyd = zone_pred_mixed<NS>(a, z, sbeg + (z - zbeg) * NS, dt, bad);
+        const double zdthyd =
+            zone_pred_mixed<NS, CL_EXACT>(a, z, sbeg + (z - zbeg) * NS, dt, bad, true, cnt);
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
     const double m = block_min(dtmin);
     const int b = block_sum(bad);
+    if (threadIdx.x == 0) {
+        a.part_dt[poff + blockIdx.x] = m;
+        a.part_bp[poff + blockIdx.x] = b;
+    }
+}
+
+// Mixed K1 with the learned closure or with tuple capture: every lane of a
+// wave runs every unit (lanes past the run's end work on a copy of its last
+// zone and store nothing), because the network and the capture slots are
+// computed across the wave.
+template <int NS, int CL>
+__global__ void __launch_bounds__(PN_BLOCK) k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                      int poff) {
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    double dtmin = 1.e99;
+    int bad = 0;
+    ClCount cnt;
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+        const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
+        const bool valid = z < zend;
+        const int zc = valid ? z : zend - 1;
+        const double zdthyd = zone_pred_mixed<NS, CL>(a, zc, sbeg + (zc - zbeg) * NS, dt, bad, valid, cnt);
+        if (valid) dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+    }
+    const double m = block_min(dtmin);
+    const int b = block_sum(bad);
+    if constexpr (CL == CL_LEARNED) {
+        const int nl = block_sum(cnt.learned);
+        const int nf = block_sum(cnt.fallback);
+        if (threadIdx.x == 0 && (nl | nf)) {
+            atomicAdd(a.cl_stats, (unsigned long long)nl);
+            atomicAdd(a.cl_stats + 1, (unsigned long long)nf);
+        }
+    }
     if (threadIdx.x == 0) {
         a.part_dt[poff + blockIdx.x] = m;
         a.part_bp[poff + blockIdx.x] = b;
@@ -1031,6 +1174,11 @@ FusedArgs make_args(pennant_problem* pb) {
     a.nbcs = pb->nbcs;
     for (int i = 0; i < 8; ++i) { a.bcvx[i] = pb->bcvx[i]; a.bcvy[i] = pb->bcvy[i]; }
     a.xcd = pb->tune.xcd;
+    a.pack = (const ClosurePack*)pb->closure;
+    a.cap_buf = pb->cap_buf;
+    a.cap = pb->cap;
+    a.cap_count = pb->cl_counters;
+    a.cl_stats = pb->cl_counters + 1;
     return a;
 }
 
@@ -1042,8 +1190,8 @@ __global__ void __launch_bounds__(PN_BLOCK) k_to_float(int n, const double* src,
 int grid_for(int needed, int cap) { return needed < 1 ? 1 : (needed < cap ? needed : cap); }
 
 // blocks of each K1 / K3 launch for a run of nz zones with NS sides
-int pred_grid(const pennant_problem* pb, int nz, int ns, bool mixed) {
-    const int per = (!mixed && pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
+int pred_grid(const pennant_problem* pb, int nz, int ns, int form) {
+    const int per = (form == FORM_FP64 && pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
     return grid_for((nz + per - 1) / per, pb->tune.max_blocks);
 }
 int corr_grid(const pennant_problem* pb, int nz) {
@@ -1052,9 +1200,13 @@ int corr_grid(const pennant_problem* pb, int nz) {
 
 template <int NS>
 void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
-                 bool mixed, hipStream_t s) {
-    const dim3 grid(pred_grid(pb, ze - zb, NS, mixed));
-    if (mixed)
+                 int form, hipStream_t s) {
+    const dim3 grid(pred_grid(pb, ze - zb, NS, form));
+    if (form == FORM_LEARNED)
+        hipLaunchKernelGGL((k_pred_cl<NS, CL_LEARNED>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else if (form == FORM_MIXED && a.cap_buf)
+        hipLaunchKernelGGL((k_pred_cl<NS, CL_CAPTURE>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else if (form == FORM_MIXED)
         hipLaunchKernelGGL(k_pred_mixed<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else if (pb->tune.k1_layout == 1)
         hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
@@ -1063,9 +1215,9 @@ void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, 
 }
 template <int NS>
 void launch_corr(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
-                 bool mixed, hipStream_t s) {
+                 int form, hipStream_t s) {
     const dim3 grid(corr_grid(pb, ze - zb));
-    if (mixed)
+    if (form != FORM_FP64)
         hipLaunchKernelGGL((k_corr<NS, float>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else
         hipLaunchKernelGGL((k_corr<NS, double>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
@@ -1145,6 +1297,8 @@ int fused_setup(pennant_problem* pb) {
     const int nruns = (int)pb->zruns.size() / 3;
     pb->npart_pred = nruns * PN_MAX_GRID;
     pb->npart_corr = nruns * PN_MAX_GRID;
+    pb->cl_counters = (unsigned long long*)al(3, 8);
+    if (!pb->cl_counters) return (int)hipErrorOutOfMemory;
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
     if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
@@ -1156,7 +1310,9 @@ int fused_setup(pennant_problem* pb) {
     return (int)hipMemcpy(pb->pbc, bits.data(), bits.size(), hipMemcpyHostToDevice);
 }
 
-int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed) {
+int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
+    if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
+    const bool mixed = (form != FORM_FP64);
     const FusedArgs a = make_args(pb);
     const Params& q = pb->prm;
     hipStream_t s = pb->stream;
@@ -1167,15 +1323,15 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, form, s); break;
             default: return PENNANT_EINVAL;
         }
-        poff += pred_grid(pb, ze - zb, ns, mixed);
+        poff += pred_grid(pb, ze - zb, ns, form);
     }
     const int npred = poff;
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -1190,12 +1346,12 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_corr<3>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 4: launch_corr<4>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 5: launch_corr<5>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 6: launch_corr<6>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 7: launch_corr<7>(pb, a, zb, ze, sb, poff, mixed, s); break;
-            case 8: launch_corr<8>(pb, a, zb, ze, sb, poff, mixed, s); break;
+            case 3: launch_corr<3>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 4: launch_corr<4>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 5: launch_corr<5>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 6: launch_corr<6>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 7: launch_corr<7>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 8: launch_corr<8>(pb, a, zb, ze, sb, poff, form, s); break;
             default: return PENNANT_EINVAL;
         }
         poff += corr_grid(pb, ze - zb);

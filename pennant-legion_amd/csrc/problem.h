@@ -94,6 +94,11 @@ struct pennant_problem {
     int* pcpos = nullptr;            // side -> slot of its corner in the point-corner CSR
     int* heavy = nullptr;            // points with more corners than a K2 thread takes
     int nheavy = 0;
+    // learned closure (a ClosurePack on the device) and training-tuple capture
+    const void* closure = nullptr;
+    float* cap_buf = nullptr;
+    int64_t cap = 0;
+    unsigned long long* cl_counters = nullptr;   // [0] tuples, [1] learned zones, [2] fallback zones
     pn::Tuning tune;
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
@@ -107,9 +112,10 @@ struct pennant_problem {
 namespace pn {
 // fused cycle (fused.hip)
 int fused_setup(pennant_problem* pb);
+// forms of the fused cycle
+enum { FORM_FP64 = 0, FORM_MIXED = 1, FORM_LEARNED = 2 };
 // ev: NULL, or 5 events recorded before K1, K2, K3, K4 and after K4
-// mixed: the mixed-precision form instead of the plain FP64 one
-int fused_cycle(pennant_problem* pb, hipEvent_t* ev, bool mixed);
+int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);
 int launch_end_cycle(pennant_problem* pb, int compute_dtrec);

@@ -21,6 +21,7 @@ LIB_PATH = _HERE / "libpennant_hip.so"
 MODE_TASKS = 0
 MODE_FUSED = 1
 MODE_MIXED = 2     # fused, FP32 side/corner/QCS math, FP64 state, volumes, energy and dt
+MODE_LEARNED = 3   # the mixed form with the learned QCS coefficient (Problem.set_closure)
 EBADSIDES = -100
 EINVAL = -22
 
@@ -112,6 +113,22 @@ DRIVER_SIGNATURES = {
     "pennant_end_cycle": ([_P], ctypes.c_int),
     "pennant_write_xy": ([_P, ctypes.c_char_p], ctypes.c_int),
     "pennant_write_gold": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_mlp_num_params": ([], ctypes.c_int),
+    "pennant_mlp_create": ([ctypes.c_uint64], ctypes.c_void_p),
+    "pennant_mlp_destroy": ([_P], None),
+    "pennant_mlp_set_params": ([_P, _P, ctypes.c_int], ctypes.c_int),
+    "pennant_mlp_get_params": ([_P, _P, ctypes.c_int], ctypes.c_int),
+    "pennant_mlp_set_envelope": ([_P, _P, _P], ctypes.c_int),
+    "pennant_mlp_get_envelope": ([_P, _P, _P], ctypes.c_int),
+    "pennant_mlp_forward": ([_P, _P, ctypes.c_int64, _P, ctypes.c_int], ctypes.c_int),
+    "pennant_mlp_loss_grad": ([_P, _P, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "pennant_mlp_train": ([_P, _P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                           ctypes.c_uint64, _P], ctypes.c_int),
+    "pennant_set_closure": ([_P, _P], ctypes.c_int),
+    "pennant_closure_stats": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_set_capture": ([_P, _P, ctypes.c_int64], ctypes.c_int),
+    "pennant_capture_count": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_write_gold_host": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, _P, _P, _P, _P,
                                  ctypes.c_int32, _P, _P, ctypes.c_int32, _P], ctypes.c_int),
     "pennant_hip_reduce_workspace_bytes": ([], ctypes.c_size_t),
@@ -240,7 +257,7 @@ class Problem:
         tot = ctypes.c_double()
         ph = (ctypes.c_double * 4)() if phases else None
         self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
-        return tot.value, (list(ph) if phases and mode in (MODE_FUSED, MODE_MIXED) else None)
+        return tot.value, (list(ph) if phases and mode in (MODE_FUSED, MODE_MIXED, MODE_LEARNED) else None)
 
     def set_tuning(self, key: str, value: int):
         if lib().pennant_set_tuning(self._h, key.encode(), int(value)) != 0:
@@ -308,6 +325,36 @@ class Problem:
 
     def write_xy(self, path):
         self._check(lib().pennant_write_xy(self._h, str(path).encode()))
+
+    def set_closure(self, mlp):
+        """Attaches a closure network (pennant_amd.closure.Mlp) for MODE_LEARNED
+        cycles; None detaches.  The problem keeps a reference to it."""
+        self._closure = mlp
+        self._check(lib().pennant_set_closure(self._h, mlp._h if mlp is not None else None))
+
+    def closure_stats(self):
+        """(zones evaluated by the network, zones that fell back to exact QCS)."""
+        out = (ctypes.c_int64 * 2)()
+        self._check(lib().pennant_closure_stats(self._h, out))
+        return int(out[0]), int(out[1])
+
+    def set_capture(self, buf, cap: int):
+        """Records training tuples of MODE_MIXED cycles into a device buffer
+        (pennant_amd.device.DeviceBuffer of cap x 12 float32); None stops."""
+        self._capture = buf
+        self._check(lib().pennant_set_capture(self._h, buf.ptr if buf is not None else None,
+                                              int(cap) if buf is not None else 0))
+
+    def capture_count(self) -> int:
+        n = ctypes.c_int64()
+        self._check(lib().pennant_capture_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def energy(self):
+        """(internal, kinetic) energy of the current state."""
+        ei, ek = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().pennant_energy(self._h, ctypes.byref(ei), ctypes.byref(ek)))
+        return ei.value, ek.value
 
     def write_gold(self, base):
         """EnSight Gold files <base>.case/.geo/.zr/.ze/.zp of the current state."""
