@@ -450,6 +450,59 @@ int pennant_capture_count(pennant_problem* pb, int64_t* count);
  * as zm and zvol). */
 int pennant_energy(pennant_problem* pb, double* internal, double* kinetic);
 
+/* ======================================================================== */
+/* Batched problems.  No counterpart in the reference (its closest analogue  */
+/* is a Legion index launch over the pieces of one problem,                  */
+/* src/Hydro.cc:385): many independent problems of uneven sizes advance      */
+/* together, one launch per phase over all of them, each with its own dt,    */
+/* parameters and stop condition.  Modes PENNANT_MODE_FUSED, _MIXED and      */
+/* _LEARNED; zones must have 3 or 4 sides (rect and pie meshes).             */
+
+typedef struct pennant_batch pennant_batch;
+
+/* Creates and initialises every deck (as pennant_create does), then moves
+ * them into one arena on the current device.  NULL on error (message in
+ * errbuf, naming the deck). */
+pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbuf, int errlen);
+void pennant_batch_destroy(pennant_batch* b);
+int pennant_batch_count(pennant_batch* b);
+/* out3: nump, numz, nums of problem i. */
+int pennant_batch_sizes(pennant_batch* b, int i, int64_t* out3);
+const char* pennant_batch_error(pennant_batch* b);
+/* Queues ncycles cycles (asynchronous; problems that have stopped skip
+ * them).  Runs of 16 cycles replay a captured HIP graph unless the tuning
+ * key "graph" is 0. */
+int pennant_batch_launch_cycles(pennant_batch* b, int ncycles, int mode);
+/* Waits; PENNANT_EBADSIDES (message in pennant_batch_error) when a problem
+ * stopped on negative side volumes. */
+int pennant_batch_sync(pennant_batch* b);
+/* Waits; number of problems still running. */
+int pennant_batch_active(pennant_batch* b);
+int pennant_batch_time_cycles(pennant_batch* b, int ncycles, int mode, double* total_ms);
+/* The scalars of problem i (the layout of pennant_scalars). */
+int pennant_batch_scalars(pennant_batch* b, int i, double* out10);
+/* Field of problem i: "px", "pu" (interleaved x, y), "zr", "ze", "zp",
+ * "zetot", "zwrate", "zvol", "zm". */
+int pennant_batch_get_field(pennant_batch* b, int i, const char* name, double* out);
+int pennant_batch_set_closure(pennant_batch* b, pennant_mlp* m);
+int pennant_batch_closure_stats(pennant_batch* b, int64_t* out2);
+/* Training-tuple capture over all problems (see pennant_set_capture). */
+int pennant_batch_set_capture(pennant_batch* b, float* dev_buf, int64_t cap);
+int pennant_batch_capture_count(pennant_batch* b, int64_t* count);
+/* "max_blocks" (1..65535, default 2048), "xcd" (0/1), "graph" (0/1). */
+int pennant_batch_set_tuning(pennant_batch* b, const char* key, int value);
+int pennant_batch_get_tuning(pennant_batch* b, const char* key);
+/* Times `cycles` cycles at each grid shape (grid cap x XCD mapping) from the
+ * same snapshot of the state, keeps the fastest and restores the snapshot:
+ * the problems do not advance.  best_ms (may be NULL) receives its time. */
+int pennant_batch_autotune(pennant_batch* b, int mode, int cycles, double* best_ms);
+/* Parameters of an ensemble of `count` problems: for k even a Sedov blast on
+ * an n[k] x n[k] rect mesh with energy param[k] in [0.1, 1.0], for k odd a
+ * Noh implosion on an n[k] x n[k] pie mesh with inflow speed param[k] in
+ * [0.5, 2.0]; n log-uniform in [32, 512]; std::mt19937_64(seed), drawn in
+ * the order n, param for each k. */
+int pennant_ensemble_params(uint64_t seed, int count, int32_t* n, double* param);
+
 #ifdef __cplusplus
 }
 #endif

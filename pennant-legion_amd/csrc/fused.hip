@@ -43,13 +43,14 @@
 // one partial per reduction (no same-address atomics on the hot path).  The
 // host launches K1 and K3 once per run of consecutive zones with equal side
 // count (rect: one run of quads; pie: triangles, then quads).
+#include <cstring>
+
 #include "closure.h"
 #include "common.h"
 #include "problem.h"
 
 using namespace pn;
 
-#define PN_HEAVY 64          // points with more corners get a whole block in K2
 #define PN_FIN_BLOCK 1024    // finalize block size
 
 namespace {
@@ -951,45 +952,45 @@ template <> struct Corners<float> {
     __device__ static double wy(const FusedArgs& a, int s) { return (double)a.fsfwy[s]; }
 };
 
-// Point gathers, fixed BCs, acceleration and position update.  Blocks
-// [0, nheavy) each take one high-valence point (the pie origin has one
-// corner per centre triangle): the block stages the corner values in LDS and
-// one thread adds them in ascending order.
+// Point gathers, fixed BCs, acceleration and position update.
+//
+// A high-valence point (the pie origin has one corner per centre triangle)
+// takes a whole block: the block stages the corner values in LDS and one
+// thread adds them in ascending order.  Called by every thread of the block.
 template <typename T>
-__global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+__device__ __forceinline__ void point_heavy(const FusedArgs& a, int p, double dt) {
     using C = Corners<T>;
-    if (a.ctl->done) return;
-    const double dt = a.ctl->dt;
-    if ((int)blockIdx.x < a.nheavy) {
-        __shared__ double sm[PN_BLOCK], sx[PN_BLOCK], sy[PN_BLOCK];
-        const int p = a.heavy[blockIdx.x];
-        const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
-        double mass = 0.;
-        d2 f = mk(0., 0.);
-        for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
-            const int k = c0 + threadIdx.x;
-            if (k < ke) {
-                sm[threadIdx.x] = C::mass(a, k);
-                sx[threadIdx.x] = C::fx(a, k);
-                sy[threadIdx.x] = C::fy(a, k);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const int cnt = (ke - c0 < PN_BLOCK) ? ke - c0 : PN_BLOCK;
-                for (int i = 0; i < cnt; ++i) {
-                    mass += sm[i];
-                    f = add(f, mk(sx[i], sy[i]));
-                }
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) point_update(a, p, mass, f, dt);
-        return;
-    }
-    const int p = (blockIdx.x - a.nheavy) * PN_BLOCK + threadIdx.x;
-    if (p >= a.nump) return;
+    __shared__ double sm[PN_BLOCK], sx[PN_BLOCK], sy[PN_BLOCK];
     const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
-    if (ke - kb > PN_HEAVY) return;          // done by a heavy-point block
+    double mass = 0.;
+    d2 f = mk(0., 0.);
+    for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
+        const int k = c0 + threadIdx.x;
+        if (k < ke) {
+            sm[threadIdx.x] = C::mass(a, k);
+            sx[threadIdx.x] = C::fx(a, k);
+            sy[threadIdx.x] = C::fy(a, k);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = (ke - c0 < PN_BLOCK) ? ke - c0 : PN_BLOCK;
+            for (int i = 0; i < cnt; ++i) {
+                mass += sm[i];
+                f = add(f, mk(sx[i], sy[i]));
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) point_update(a, p, mass, f, dt);
+}
+
+// An ordinary point, one thread; points with more than PN_HEAVY corners are
+// left to point_heavy.
+template <typename T>
+__device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt) {
+    using C = Corners<T>;
+    const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
+    if (ke - kb > PN_HEAVY) return;
     double mass = 0.;
     d2 f = mk(0., 0.);
     for (int k = kb; k < ke; ++k) {
@@ -999,68 +1000,87 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
     point_update(a, p, mass, f, dt);
 }
 
+// K2: blocks [0, nheavy) take the heavy points, the rest one point per thread.
+template <typename T>
+__global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    if ((int)blockIdx.x < a.nheavy) {
+        point_heavy<T>(a, a.heavy[blockIdx.x], dt);
+        return;
+    }
+    const int p = (blockIdx.x - a.nheavy) * PN_BLOCK + threadIdx.x;
+    if (p < a.nump) point_light<T>(a, p, dt);
+}
+
 // ---------------------------------------------------------------------------
-// K3: corrector geometry, work, work rate, energy, density, dvol.
+// K3: corrector geometry, work, work rate, energy, density, dvol of zone z
+// (sides sb..sb+NS-1); folds the bad-side count and the dvol ratio.
+template <int NS, typename T>
+__device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, double dt, int& bad,
+                                          double& dvov) {
+    using C = Corners<T>;
+    const double dth = 0.5 * dt;
+    const double n = (double)NS;
+    int pp[NS];
+    d2 x[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        pp[k] = a.mapsp1[sb + k];
+        x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
+    }
+    d2 zx = mk(0., 0.);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) zx = add(zx, dvs(x[k], n));
+    const double third = 1. / 3.;
+    double zvol = 0.;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const d2 p1 = x[k], p2 = x[nxt<NS>(k)];
+        const double sa = 0.5 * cross(sub(p2, p1), sub(zx, p1));
+        const double sv = third * sa * (p1.x + p2.x + zx.x);
+        zvol += sv;
+        if (sv <= 0.) bad += 1;
+    }
+    // calcWork
+    double w = 0., et = a.zetot[z];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int p1 = pp[k], p2 = pp[nxt<NS>(k)];
+        const d2 sftot = mk(C::wx(a, sb + k), C::wy(a, sb + k));
+        const double sd1 = dot(sftot, mk(a.vsx[p1], a.vsy[p1]));
+        const double sd2 = dot(neg(sftot), mk(a.vsx[p2], a.vsy[p2]));
+        const double dwork = -dth * (sd1 * a.pxpx[p1] + sd2 * a.pxpx[p2]);
+        et += dwork;
+        w += dwork;
+    }
+    const double zvol0 = a.zvol[z];
+    const double zm = a.zm[z];
+    // calcWorkRate, calcEnergy, calcRho (corr), calcDvol
+    const double dtinv = 1. / dt;
+    const double dvol = zvol - zvol0;
+    a.zwrate[z] = (w + a.zp[z] * dvol) * dtinv;
+    a.zetot[z] = et;
+    const double fuzz = 1.e-99;
+    a.ze[z] = et / (zm + fuzz);
+    a.zr[z] = zm / zvol;
+    a.zvol[z] = zvol;
+    const double zdvov = fabs((zvol - zvol0) / zvol0);
+    dvov = (zdvov > dvov) ? zdvov : dvov;
+}
+
 template <int NS, typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int zend, int sbeg,
                                                    int poff) {
-    using C = Corners<T>;
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
-    const double dth = 0.5 * dt;
     double dvov = 1.e-99;
     int bad = 0;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
     for (Units wu = block_units(nunits, a.xcd); wu.u < wu.end; wu.u += wu.step) {
         const int z = zbeg + wu.u * PN_BLOCK + threadIdx.x;
         if (z >= zend) continue;
-        const int sb = sbeg + (z - zbeg) * NS;
-        const double n = (double)NS;
-        int pp[NS];
-        d2 x[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            pp[k] = a.mapsp1[sb + k];
-            x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
-        }
-        d2 zx = mk(0., 0.);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) zx = add(zx, dvs(x[k], n));
-        const double third = 1. / 3.;
-        double zvol = 0.;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const d2 p1 = x[k], p2 = x[nxt<NS>(k)];
-            const double sa = 0.5 * cross(sub(p2, p1), sub(zx, p1));
-            const double sv = third * sa * (p1.x + p2.x + zx.x);
-            zvol += sv;
-            if (sv <= 0.) bad += 1;
-        }
-        // calcWork
-        double w = 0., et = a.zetot[z];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int p1 = pp[k], p2 = pp[nxt<NS>(k)];
-            const d2 sftot = mk(C::wx(a, sb + k), C::wy(a, sb + k));
-            const double sd1 = dot(sftot, mk(a.vsx[p1], a.vsy[p1]));
-            const double sd2 = dot(neg(sftot), mk(a.vsx[p2], a.vsy[p2]));
-            const double dwork = -dth * (sd1 * a.pxpx[p1] + sd2 * a.pxpx[p2]);
-            et += dwork;
-            w += dwork;
-        }
-        const double zvol0 = a.zvol[z];
-        const double zm = a.zm[z];
-        // calcWorkRate, calcEnergy, calcRho (corr), calcDvol
-        const double dtinv = 1. / dt;
-        const double dvol = zvol - zvol0;
-        a.zwrate[z] = (w + a.zp[z] * dvol) * dtinv;
-        a.zetot[z] = et;
-        const double fuzz = 1.e-99;
-        a.ze[z] = et / (zm + fuzz);
-        a.zr[z] = zm / zvol;
-        a.zvol[z] = zvol;
-        const double zdvov = fabs((zvol - zvol0) / zvol0);
-        dvov = (zdvov > dvov) ? zdvov : dvov;
+        corr_zone<NS, T>(a, z, sbeg + (z - zbeg) * NS, dt, bad, dvov);
     }
     const double m = block_max(dvov);
     const int b = block_sum(bad);
@@ -1078,6 +1098,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
 // stop test and calcGlobalDt (src/Driver.cc:213-245) for the next cycle.
 __device__ __forceinline__ double blk_min_fin(double v) {
     __shared__ double part[PN_FIN_BLOCK / 64];
+    __syncthreads();   // a previous call's partials may still be read
     v = wave_min(v);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -1087,6 +1108,7 @@ __device__ __forceinline__ double blk_min_fin(double v) {
 }
 __device__ __forceinline__ double blk_max_fin(double v) {
     __shared__ double part[PN_FIN_BLOCK / 64];
+    __syncthreads();   // a previous call's partials may still be read
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -1096,6 +1118,7 @@ __device__ __forceinline__ double blk_max_fin(double v) {
 }
 __device__ __forceinline__ int blk_sum_fin(int v) {
     __shared__ int part[PN_FIN_BLOCK / 64];
+    __syncthreads();   // a previous call's partials may still be read
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -1104,22 +1127,24 @@ __device__ __forceinline__ int blk_sum_fin(int v) {
     return v;
 }
 
-__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
-                                                           double cflv, int cstop, double dtmax,
-                                                           double dtfac, double tstop) {
-    Ctl* c = a.ctl;
+// The end of the cycle for one problem, by one block of PN_FIN_BLOCK threads,
+// from npred predictor and ncorr corrector partials.
+__device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, const int* part_bp,
+                                               int npred, const double* part_dv, const int* part_bc,
+                                               int ncorr, double cflv, int cstop, double dtmax,
+                                               double dtfac, double tstop) {
     if (c->done) return;
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
     for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
-        const double v = a.part_dt[i];
+        const double v = part_dt[i];
         mn = (v < mn) ? v : mn;
-        bp += a.part_bp[i];
+        bp += part_bp[i];
     }
     for (int i = threadIdx.x; i < ncorr; i += PN_FIN_BLOCK) {
-        const double v = a.part_dv[i];
+        const double v = part_dv[i];
         mx = (v > mx) ? v : mx;
-        bc += a.part_bc[i];
+        bc += part_bc[i];
     }
     mn = blk_min_fin(mn);
     mx = blk_max_fin(mx);
@@ -1151,6 +1176,14 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npre
     if ((tstop - c->time) < dt) dt = tstop - c->time;
     if (c->dtrec < dt) dt = c->dtrec;
     c->dt = dt;
+}
+
+
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
+                                                           double cflv, int cstop, double dtmax,
+                                                           double dtfac, double tstop) {
+    finalize_block(a.ctl, a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, cflv, cstop,
+                   dtmax, dtfac, tstop);
 }
 
 FusedArgs make_args(pennant_problem* pb) {
@@ -1221,6 +1254,164 @@ void launch_corr(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, 
         hipLaunchKernelGGL((k_corr<NS, float>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else
         hipLaunchKernelGGL((k_corr<NS, double>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+}
+
+// ---------------------------------------------------------------------------
+// Batched problems: one launch per phase over the units of every problem.
+// Each zone unit lies in one side-count run of one problem, so a block works
+// on one problem (its dt, parameters and stop flag) at a time and writes one
+// partial per unit; K4 runs one block per problem over that problem's units.
+
+struct BatchArgs {
+    FusedArgs a;                   // arena arrays; per-problem scalars come from probs
+    const BUnit* units;
+    int nunits;
+    const PUnit* punits;
+    int npunits;
+    const BHeavy* heavy;
+    int nheavy;
+    const BProb* probs;
+    Ctl* ctls;
+};
+
+__device__ __forceinline__ void apply_prob(FusedArgs& a, const BProb& q) {
+    a.gamma = q.gamma; a.ssmin = q.ssmin; a.alfa = q.alfa; a.qgamma = q.qgamma;
+    a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
+    a.nbcs = q.nbcs;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { a.bcvx[i] = q.bcvx[i]; a.bcvy[i] = q.bcvy[i]; }
+}
+
+template <int NS, int FORM, int CL>
+__device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, double dt,
+                                          double& dtmin, int& bad, ClCount& cnt) {
+    const int z = U.zb + (int)threadIdx.x;
+    const bool valid = z < U.ze;
+    if constexpr (FORM == FORM_FP64) {
+        if (valid) {
+            const double d = zone_pred<NS>(a, z, U.sb + (z - U.zb) * NS, dt, bad);
+            dtmin = (d < dtmin) ? d : dtmin;
+        }
+    } else if constexpr (CL == CL_EXACT) {
+        if (valid) {
+            const double d = zone_pred_mixed<NS, CL_EXACT>(a, z, U.sb + (z - U.zb) * NS, dt, bad, true, cnt);
+            dtmin = (d < dtmin) ? d : dtmin;
+        }
+    } else {
+        const int zc = valid ? z : U.ze - 1;
+        const double d = zone_pred_mixed<NS, CL>(a, zc, U.sb + (zc - U.zb) * NS, dt, bad, valid, cnt);
+        if (valid) dtmin = (d < dtmin) ? d : dtmin;
+    }
+}
+
+template <int FORM, int CL>
+__global__ void __launch_bounds__(PN_BLOCK) k_pred_batch(BatchArgs B) {
+    for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
+        const BUnit U = B.units[w.u];
+        const Ctl* c = B.ctls + U.pid;
+        if (c->done) continue;
+        FusedArgs a = B.a;
+        apply_prob(a, B.probs[U.pid]);
+        const double dt = c->dt;
+        double dtmin = 1.e99;
+        int bad = 0;
+        ClCount cnt;
+        if (U.ns == 4) pred_unit<4, FORM, CL>(a, U, dt, dtmin, bad, cnt);
+        else pred_unit<3, FORM, CL>(a, U, dt, dtmin, bad, cnt);
+        const double m = block_min(dtmin);
+        const int nb = block_sum(bad);
+        if constexpr (CL == CL_LEARNED) {
+            const int nl = block_sum(cnt.learned);
+            const int nf = block_sum(cnt.fallback);
+            if (threadIdx.x == 0 && (nl | nf)) {
+                atomicAdd(a.cl_stats, (unsigned long long)nl);
+                atomicAdd(a.cl_stats + 1, (unsigned long long)nf);
+            }
+        }
+        if (threadIdx.x == 0) {
+            a.part_dt[w.u] = m;
+            a.part_bp[w.u] = nb;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PN_BLOCK) k_point_batch(BatchArgs B) {
+    if ((int)blockIdx.x < B.nheavy) {
+        const BHeavy h = B.heavy[blockIdx.x];
+        const Ctl* c = B.ctls + h.pid;
+        if (c->done) return;
+        FusedArgs a = B.a;
+        apply_prob(a, B.probs[h.pid]);
+        point_heavy<T>(a, h.p, c->dt);
+        return;
+    }
+    const PUnit U = B.punits[blockIdx.x - B.nheavy];
+    const Ctl* c = B.ctls + U.pid;
+    if (c->done) return;
+    FusedArgs a = B.a;
+    apply_prob(a, B.probs[U.pid]);
+    const int p = U.pb + (int)threadIdx.x;
+    if (p < U.pe) point_light<T>(a, p, c->dt);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PN_BLOCK) k_corr_batch(BatchArgs B) {
+    for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
+        const BUnit U = B.units[w.u];
+        const Ctl* c = B.ctls + U.pid;
+        if (c->done) continue;
+        const double dt = c->dt;
+        double dvov = 1.e-99;
+        int bad = 0;
+        const int z = U.zb + (int)threadIdx.x;
+        if (z < U.ze) {
+            if (U.ns == 4) corr_zone<4, T>(B.a, z, U.sb + (z - U.zb) * 4, dt, bad, dvov);
+            else corr_zone<3, T>(B.a, z, U.sb + (z - U.zb) * 3, dt, bad, dvov);
+        }
+        const double m = block_max(dvov);
+        const int nb = block_sum(bad);
+        if (threadIdx.x == 0) {
+            B.a.part_dv[w.u] = m;
+            B.a.part_bc[w.u] = nb;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_batch(BatchArgs B) {
+    const BProb& q = B.probs[blockIdx.x];
+    const int n = q.uend - q.ubeg;
+    finalize_block(B.ctls + blockIdx.x, B.a.part_dt + q.ubeg, B.a.part_bp + q.ubeg, n,
+                   B.a.part_dv + q.ubeg, B.a.part_bc + q.ubeg, n, q.cflv, q.cstop, q.dtmax,
+                   q.dtfac, q.tstop);
+}
+
+BatchArgs make_batch_args(pennant_batch* b) {
+    BatchArgs B;
+    FusedArgs& a = B.a;
+    std::memset(&a, 0, sizeof a);
+    a.mapsp1 = b->mapsp1; a.pcbeg = b->pcbeg; a.pcpos = b->pcpos; a.smf = b->smf;
+    a.nump = (int)b->nump;
+    a.pxx = b->pxx; a.pxy = b->pxy; a.pux = b->pux; a.puy = b->puy;
+    a.zr = b->zr; a.ze = b->ze; a.zetot = b->zetot; a.zwrate = b->zwrate; a.zvol = b->zvol;
+    a.zp = b->zp; a.zm = b->zm;
+    a.cfx = b->cfx; a.cfy = b->cfy; a.cmass = b->cmass; a.sfwx = b->sfwx; a.sfwy = b->sfwy;
+    a.fcfx = b->fcfx; a.fcfy = b->fcfy; a.fcmass = b->fcmass; a.fsfwx = b->fsfwx; a.fsfwy = b->fsfwy;
+    a.fsmf = b->fsmf;
+    a.vsx = b->vsx; a.vsy = b->vsy; a.pxpx = b->pxpx; a.pbc = b->pbc;
+    a.part_dt = b->part_d; a.part_dv = b->part_d + b->nunits;
+    a.part_bp = b->part_i; a.part_bc = b->part_i + b->nunits;
+    a.xcd = b->tune.xcd;
+    a.pack = (const ClosurePack*)b->closure;
+    a.cap_buf = b->cap_buf;
+    a.cap = b->cap;
+    a.cap_count = b->cl_counters;
+    a.cl_stats = b->cl_counters + 1;
+    B.units = b->units; B.nunits = b->nunits;
+    B.punits = b->punits; B.npunits = b->npunits;
+    B.heavy = b->heavy; B.nheavy = b->nheavy;
+    B.probs = b->probs; B.ctls = b->ctls;
+    return B;
 }
 
 }  // namespace
@@ -1361,6 +1552,37 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred, ncorr, q.cflv,
                        q.cstop, q.dtmax, q.dtfac, q.tstop);
     if (ev) (void)hipEventRecord(ev[4], s);
+    return (int)hipGetLastError();
+}
+
+int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_to_float, dim3(nblocks(n)), dim3(PN_BLOCK), 0, s, (int)n, src, dst);
+    return (int)hipGetLastError();
+}
+
+int batch_cycle(pennant_batch* b, int form) {
+    if (form == FORM_LEARNED && !b->closure) return PENNANT_EINVAL;
+    const BatchArgs B = make_batch_args(b);
+    hipStream_t s = b->stream;
+    const dim3 zg(grid_for(b->nunits, b->tune.max_blocks)), blk(PN_BLOCK);
+    if (form == FORM_FP64)
+        hipLaunchKernelGGL((k_pred_batch<FORM_FP64, CL_EXACT>), zg, blk, 0, s, B);
+    else if (form == FORM_LEARNED)
+        hipLaunchKernelGGL((k_pred_batch<FORM_MIXED, CL_LEARNED>), zg, blk, 0, s, B);
+    else if (b->cap_buf)
+        hipLaunchKernelGGL((k_pred_batch<FORM_MIXED, CL_CAPTURE>), zg, blk, 0, s, B);
+    else
+        hipLaunchKernelGGL((k_pred_batch<FORM_MIXED, CL_EXACT>), zg, blk, 0, s, B);
+    const dim3 pg(b->nheavy + b->npunits);
+    if (form == FORM_FP64) {
+        hipLaunchKernelGGL(k_point_batch<double>, pg, blk, 0, s, B);
+        hipLaunchKernelGGL(k_corr_batch<double>, zg, blk, 0, s, B);
+    } else {
+        hipLaunchKernelGGL(k_point_batch<float>, pg, blk, 0, s, B);
+        hipLaunchKernelGGL(k_corr_batch<float>, zg, blk, 0, s, B);
+    }
+    hipLaunchKernelGGL(k_finalize_batch, dim3(b->nprob), dim3(PN_FIN_BLOCK), 0, s, B);
     return (int)hipGetLastError();
 }
 

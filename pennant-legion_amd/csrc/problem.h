@@ -31,12 +31,14 @@ struct Ctl {
 };
 
 #define PN_MAX_GRID 4096   // largest grid of a fused K1/K3 launch
+#define PN_HEAVY 64        // points with more corners get a whole block in K2
 
 // Launch-shape choices of the fused cycle (pennant_set_tuning).
 struct Tuning {
     int k1_layout = 0;       // 0 zone per thread, 1 corner per lane
     int max_blocks = 2048;   // grid cap of the grid-stride K1/K3 launches
     int xcd = 1;             // XCD-aware unit mapping of the grid-stride launches
+    int graph = 1;           // batched cycles through a captured HIP graph
 };
 
 struct Params {
@@ -110,6 +112,70 @@ struct pennant_problem {
 };
 
 namespace pn {
+// Batched problems: zone units (at most PN_BLOCK zones of one side-count run
+// of one problem), point units (at most PN_BLOCK points of one problem),
+// heavy points and the per-problem parameters.
+struct BUnit {
+    int zb, ze, sb, ns, pid, pad0, pad1, pad2;
+};
+struct PUnit {
+    int pb, pe, pid, pad;
+};
+struct BHeavy {
+    int p, pid;
+};
+struct BProb {
+    double gamma, ssmin, alfa, qgamma, q1, q2, cfl, cflv, dtmax, dtfac, tstop;
+    double bcvx[8], bcvy[8];
+    int cstop, nbcs, ubeg, uend;
+};
+}  // namespace pn
+
+struct pennant_batch {
+    int nprob = 0;
+    int64_t numz = 0, nums = 0, nump = 0;
+    std::vector<int64_t> zoff, soff, poff;   // per problem, plus the totals
+    std::vector<void*> allocs;
+    // state (arena: all problems concatenated; maps hold arena indices)
+    double *pxx = nullptr, *pxy = nullptr, *pux = nullptr, *puy = nullptr;
+    double *zr = nullptr, *ze = nullptr, *zetot = nullptr, *zwrate = nullptr, *zvol = nullptr;
+    double *zp = nullptr, *zm = nullptr, *smf = nullptr;
+    float* fsmf = nullptr;
+    int *mapsp1 = nullptr, *pcbeg = nullptr, *pcpos = nullptr;
+    unsigned char* pbc = nullptr;
+    // fused intermediates
+    double *cfx = nullptr, *cfy = nullptr, *cmass = nullptr, *sfwx = nullptr, *sfwy = nullptr;
+    double *vsx = nullptr, *vsy = nullptr, *pxpx = nullptr;
+    float *fcfx = nullptr, *fcfy = nullptr, *fcmass = nullptr, *fsfwx = nullptr, *fsfwy = nullptr;
+    // work tables
+    pn::BUnit* units = nullptr;
+    int nunits = 0;
+    pn::PUnit* punits = nullptr;
+    int npunits = 0;
+    pn::BHeavy* heavy = nullptr;
+    int nheavy = 0;
+    pn::BProb* probs = nullptr;
+    pn::Ctl* ctls = nullptr;
+    pn::Ctl* ctls_host = nullptr;                // pinned mirror
+    double* part_d = nullptr;                    // per unit: dt minima, then dvol maxima
+    int* part_i = nullptr;                       // per unit: bad sides (pred, corr)
+    // closure and capture
+    const void* closure = nullptr;
+    float* cap_buf = nullptr;
+    int64_t cap = 0;
+    unsigned long long* cl_counters = nullptr;
+    pn::Tuning tune;
+    hipStream_t stream = nullptr;
+    // captured cycles: graph_cycles cycles of form graph_form
+    hipGraphExec_t gexec = nullptr;
+    int graph_form = -1, graph_cycles = 0;
+    std::string err;
+};
+
+namespace pn {
+// one cycle of every unfinished problem of a batch (fused.hip)
+int batch_cycle(pennant_batch* b, int form);
+int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 // fused cycle (fused.hip)
 int fused_setup(pennant_problem* pb);
 // forms of the fused cycle

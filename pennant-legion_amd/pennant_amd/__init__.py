@@ -129,6 +129,27 @@ DRIVER_SIGNATURES = {
     "pennant_set_capture": ([_P, _P, ctypes.c_int64], ctypes.c_int),
     "pennant_capture_count": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "pennant_energy": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_batch_create": ([ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+                             ctypes.c_void_p),
+    "pennant_batch_destroy": ([_P], None),
+    "pennant_batch_count": ([_P], ctypes.c_int),
+    "pennant_batch_sizes": ([_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_batch_error": ([_P], ctypes.c_char_p),
+    "pennant_batch_launch_cycles": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pennant_batch_sync": ([_P], ctypes.c_int),
+    "pennant_batch_active": ([_P], ctypes.c_int),
+    "pennant_batch_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                                  ctypes.c_int),
+    "pennant_batch_scalars": ([_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_batch_get_field": ([_P, ctypes.c_int, ctypes.c_char_p, _P], ctypes.c_int),
+    "pennant_batch_set_closure": ([_P, _P], ctypes.c_int),
+    "pennant_batch_closure_stats": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_batch_set_capture": ([_P, _P, ctypes.c_int64], ctypes.c_int),
+    "pennant_batch_capture_count": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_batch_set_tuning": ([_P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "pennant_batch_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_batch_autotune": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_ensemble_params": ([ctypes.c_uint64, ctypes.c_int, _P, _P], ctypes.c_int),
     "pennant_write_gold_host": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, _P, _P, _P, _P,
                                  ctypes.c_int32, _P, _P, ctypes.c_int32, _P], ctypes.c_int),
     "pennant_hip_reduce_workspace_bytes": ([], ctypes.c_size_t),
@@ -371,3 +392,122 @@ def write_gold_host(base, cycle: int, time: float, zr, ze, zp, znump, px, py, ma
                                        f64[4].ctypes.data, int(f64[3].size), i32[1].ctypes.data)
     if rc != 0:
         raise PennantError(f"cannot write {base}.*")
+
+
+class Batch:
+    """Many independent problems advanced together (one launch per phase
+    over all of them); each keeps its own dt, parameters and stop test."""
+
+    def __init__(self, deck_texts):
+        texts = [t.encode() for t in deck_texts]
+        arr = (ctypes.c_char_p * len(texts))(*texts)
+        err = ctypes.create_string_buffer(1024)
+        h = lib().pennant_batch_create(arr, len(texts), err, len(err))
+        if not h:
+            raise PennantError(err.value.decode())
+        self._h = ctypes.c_void_p(h)
+        self.count = len(texts)
+
+    def _check(self, rc):
+        if rc == EBADSIDES:
+            raise PennantError(lib().pennant_batch_error(self._h).decode())
+        if rc != 0:
+            raise PennantError(f"HIP/pennant error {rc}")
+
+    def sizes(self, i: int):
+        """(nump, numz, nums) of problem i."""
+        out = (ctypes.c_int64 * 3)()
+        self._check(lib().pennant_batch_sizes(self._h, int(i), out))
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def launch(self, ncycles: int, mode: int = MODE_MIXED):
+        self._check(lib().pennant_batch_launch_cycles(self._h, int(ncycles), int(mode)))
+
+    def sync(self):
+        self._check(lib().pennant_batch_sync(self._h))
+
+    def active(self) -> int:
+        n = lib().pennant_batch_active(self._h)
+        if n < 0:
+            self._check(n)
+        return n
+
+    def run(self, ncycles: int, mode: int = MODE_MIXED):
+        self.launch(ncycles, mode)
+        self.sync()
+
+    def run_to_end(self, mode: int = MODE_MIXED, chunk: int = 256, max_cycles: int = 10 ** 7):
+        done = 0
+        while done < max_cycles and self.active() > 0:
+            self.run(chunk, mode)
+            done += chunk
+        return done
+
+    def time_cycles(self, ncycles: int, mode: int = MODE_MIXED) -> float:
+        ms = ctypes.c_double()
+        self._check(lib().pennant_batch_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(ms)))
+        return ms.value
+
+    def scalars(self, i: int) -> dict:
+        out = (ctypes.c_double * 10)()
+        self._check(lib().pennant_batch_scalars(self._h, int(i), out))
+        keys = ("time", "dt", "dtrec", "cycle", "dtnew", "dvovmax", "numsbad_pred", "numsbad_corr",
+                "dtnext", "done")
+        d = dict(zip(keys, list(out)))
+        d["cycle"] = int(d["cycle"])
+        d["done"] = int(d["done"])
+        return d
+
+    def get(self, i: int, name: str) -> np.ndarray:
+        nump, numz, _ = self.sizes(i)
+        if name in ("px", "pu"):
+            out = np.empty((nump, 2), dtype=np.float64)
+        else:
+            out = np.empty(numz, dtype=np.float64)
+        self._check(lib().pennant_batch_get_field(self._h, int(i), name.encode(), out.ctypes.data))
+        return out
+
+    def set_closure(self, mlp):
+        self._closure = mlp
+        self._check(lib().pennant_batch_set_closure(self._h, mlp._h if mlp is not None else None))
+
+    def closure_stats(self):
+        out = (ctypes.c_int64 * 2)()
+        self._check(lib().pennant_batch_closure_stats(self._h, out))
+        return int(out[0]), int(out[1])
+
+    def set_capture(self, buf, cap: int):
+        self._capture = buf
+        self._check(lib().pennant_batch_set_capture(self._h, buf.ptr if buf is not None else None,
+                                                    int(cap) if buf is not None else 0))
+
+    def capture_count(self) -> int:
+        n = ctypes.c_int64()
+        self._check(lib().pennant_batch_capture_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_tuning(self, key: str, value: int):
+        if lib().pennant_batch_set_tuning(self._h, key.encode(), int(value)) != 0:
+            raise ValueError(f"bad tuning {key}={value}")
+
+    def get_tuning(self, key: str) -> int:
+        return lib().pennant_batch_get_tuning(self._h, key.encode())
+
+    def autotune(self, mode: int = MODE_MIXED, cycles: int = 8) -> float:
+        ms = ctypes.c_double()
+        self._check(lib().pennant_batch_autotune(self._h, int(mode), int(cycles), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pennant_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()

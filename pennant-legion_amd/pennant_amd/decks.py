@@ -85,3 +85,26 @@ def leblanc_4096x256(cstop: int | None = None) -> str:
 def sedov_90(cstop: int | None = None) -> str:
     """The CPU-sized Sedov case: 90 x 90, einitsub 40222.1, dtinit 2.5e-5."""
     return sedov_rect(90, cstop=cstop, dtinit=2.5e-5)
+
+
+def ensemble(count: int = 256, seed: int = 20261202, cstop: int | None = None):
+    """The batched ensemble: for k even a Sedov blast on an n x n rect mesh
+    with energy E, for k odd a Noh implosion on an n x n pie mesh with inflow
+    speed s; n log-uniform in [32, 512], E in [0.1, 1.0], s in [0.5, 2.0]
+    (std::mt19937_64(seed), pennant_ensemble_params).  Returns (decks, n, param)."""
+    import ctypes
+
+    import numpy as np
+
+    from . import lib
+    n = np.zeros(count, dtype=np.int32)
+    par = np.zeros(count, dtype=np.float64)
+    if lib().pennant_ensemble_params(ctypes.c_uint64(seed), int(count), n.ctypes.data, par.ctypes.data) != 0:
+        raise ValueError("bad ensemble size")
+    decks = []
+    for k in range(count):
+        if k % 2 == 0:
+            decks.append(sedov_rect(int(n[k]), energy=float(par[k]), cstop=cstop))
+        else:
+            decks.append(noh_pie(int(n[k]), int(n[k]), speed=float(par[k]), cstop=cstop))
+    return decks, n, par
