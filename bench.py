@@ -1,8 +1,12 @@
 """PENNANT hydro cycle benchmark on one MI355X.
 
 Workload (default): Sedov blast on a 1440 x 1440 rect mesh (2,073,600 zones,
-8,294,400 sides), plain FP64, fused cycle.  A step is one hydro cycle over
-the whole mesh; the state is resident in HBM before the timed region starts.
+8,294,400 sides), fused cycle in mixed precision with the learned QCS
+closure (trained weights loaded from the package; training is not timed),
+reported against the plain FP64 fused cycle (bit-identical to the CPU
+oracle) and the mixed form with the exact QCS on the same workload.  A step
+is one hydro cycle over the whole mesh; the state is resident in HBM before
+the timed region starts.
 
 Prints ONE JSON line on stdout:
   value       zone-cycles/s = zones * cycles / wall (HIP events on the
@@ -72,9 +76,43 @@ def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False):
     }
 
 
+def physics_check(pa, decks, mlp, n):
+    """Sedov n x n to t = 1 in the plain FP64, mixed and learned forms: density
+    L1 error, shock radius (densest zone) and total energy against FP64."""
+    from pennant_amd.accuracy import rel_l1
+    from pennant_amd.diagnostics import shock_radius, zone_centroids
+    text = decks.sedov_rect(n)
+    hm = pa.HostMesh(text)
+    znump, mapsp1 = hm.get("znump"), hm.get("mapsp1")
+    res = {}
+    for name, mode in (("fp64", pa.MODE_FUSED), ("mixed", pa.MODE_MIXED), ("learned", pa.MODE_LEARNED)):
+        with pa.Problem(text) as pb:
+            if mode == pa.MODE_LEARNED:
+                pb.set_closure(mlp)
+            cyc = 0
+            while not pb.scalars()["done"]:
+                cyc += pb.run(4096, mode)
+            zr, px = pb.get("zr"), pb.get("px")
+            ei, ek = pb.energy()
+            res[name] = (zr, shock_radius(zr, zone_centroids(znump, mapsp1, px)), ei + ek, cyc,
+                         pb.closure_stats() if mode == pa.MODE_LEARNED else None)
+    e0 = res["fp64"][2]
+    out = {"workload": f"sedov-rect-{n}x{n} to t=1", "cycles": res["fp64"][3],
+           "shock_radius_fp64": round(res["fp64"][1], 6), "energy_fp64": e0}
+    for name in ("mixed", "learned"):
+        zr, rs, et, cyc, st = res[name]
+        out[name] = {"rel_l1_zr": float(f"{rel_l1(zr, res['fp64'][0]):.3e}"), "shock_radius": round(rs, 6),
+                     "rel_energy_diff": float(f"{(et - e0) / e0:.3e}"), "cycles": cyc}
+        if st is not None:
+            out[name]["closure_zone_evaluations"] = {"learned": st[0], "exact_fallback": st[1]}
+    log(f"physics check {out}")
+    return out
+
+
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
     "mixed": {"zone_pred": "k_pred_mixed<4>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
+    "learned": {"zone_pred": "k_pred_cl<4, 1>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
 }
 
 
@@ -119,10 +157,17 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sedov1440")
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "fused", "tasks"],
-                    help="mixed: fused cycle in mixed precision (reported against the plain FP64 "
-                         "fused cycle on the same workload); fused: plain FP64 fused cycle; "
-                         "tasks: one launch per task")
+    ap.add_argument("--mode", default="learned", choices=["learned", "mixed", "fused", "tasks"],
+                    help="learned: mixed precision with the learned QCS closure; mixed: mixed "
+                         "precision with the exact QCS (both reported against the plain FP64 fused "
+                         "cycle on the same workload); fused: plain FP64 fused cycle; tasks: one "
+                         "launch per task")
+    ap.add_argument("--closure", default=None,
+                    help="trained closure (.npz from pennant_amd.train_closure); default: the "
+                         "package's closure_ensemble.npz")
+    ap.add_argument("--physics-n", type=int, default=360,
+                    help="Sedov n x n run to t=1 comparing shock radius and energy of the forms "
+                         "(0 disables)")
     ap.add_argument("--copy-mb", type=int, default=512,
                     help="size of the streaming-copy calibration (0 disables)")
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
@@ -141,11 +186,22 @@ def main():
     from pennant_amd.accuracy import field_errors
 
     text, wname = workload_deck(args.workload, decks)
-    modes = {"mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED, "tasks": pa.MODE_TASKS}
+    modes = {"learned": pa.MODE_LEARNED, "mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED,
+             "tasks": pa.MODE_TASKS}
+    mlp = None
+    if args.mode == "learned":
+        from pennant_amd.closure import Mlp
+        from pennant_amd.train_closure import DEFAULT_OUT
+        path = Path(args.closure) if args.closure else DEFAULT_OUT
+        if not path.exists():
+            raise SystemExit(f"no trained closure at {path}: run python -m pennant_amd.train_closure")
+        mlp = Mlp.load(path)
 
     def measure(mode):
         t0 = time.perf_counter()
         pb = pa.Problem(text)
+        if mode == pa.MODE_LEARNED:
+            pb.set_closure(mlp)
         if args.k1 is not None:
             pb.set_tuning("k1_layout", args.k1)
         if args.max_blocks is not None:
@@ -159,6 +215,8 @@ def main():
         sc = pb.scalars()
         assert sc["cycle"] == args.warmup + args.steps, sc
         state = {f: pb.get(f) for f in ("zr", "ze", "zp")}
+        if mode == pa.MODE_LEARNED:
+            state["closure_stats"] = pb.closure_stats()
         dims = (pb.numz, pb.nums, pb.nump)
         pb.close()
         return total_ms, phase_ms, sc, state, dims
@@ -171,7 +229,7 @@ def main():
         f"{value:.4e} zone-cycles/s; t={sc['time']:.6e}")
 
     accuracy = None
-    if mode == pa.MODE_MIXED:
+    if mode in (pa.MODE_MIXED, pa.MODE_LEARNED):
         # the plain FP64 form on the same workload, same cycle count: its rate
         # is the denominator of the speedup and its state the accuracy reference
         p_ms, _, p_sc, p_state, _ = measure(pa.MODE_FUSED)
@@ -187,10 +245,21 @@ def main():
                     "rel_linf": {k: float(f"{v[1]:.3e}") for k, v in err.items()},
                     "plain_fp64_value": round(plain, 1),
                     "speedup_vs_plain_fp64": round(value / plain, 4)}
+        if mode == pa.MODE_LEARNED:
+            m_ms, _, _, m_state, _ = measure(pa.MODE_MIXED)
+            mixed_v = Z * args.steps / (m_ms * 1e-3)
+            nl, nf = state["closure_stats"]
+            log(f"mixed exact QCS: {m_ms / args.steps * 1e3:.1f} us/cycle, {mixed_v:.4e} zone-cycles/s; "
+                f"closure zones learned {nl} fallback {nf}")
+            accuracy["mixed_exact_qcs_value"] = round(mixed_v, 1)
+            accuracy["closure_zone_evaluations"] = {"learned": nl, "exact_fallback": nf}
+            accuracy["rel_l1_learned_vs_mixed_exact"] = float(f"{field_errors(state, m_state)['zr'][0]:.3e}")
+            if args.physics_n > 0:
+                accuracy["physics"] = physics_check(pa, decks, mlp, args.physics_n)
 
     roofline = None
     if phase_ms is not None:
-        mixed = (mode == pa.MODE_MIXED)
+        mixed = (mode in (pa.MODE_MIXED, pa.MODE_LEARNED))
         kb = kernel_bytes(S, Z, P, mixed)
         names = ("zone_pred", "point", "zone_corr", "finalize")
         per = {n: phase_ms[i] / args.steps for i, n in enumerate(names)}
@@ -203,8 +272,8 @@ def main():
         # the reference's task decomposition moves 688 S + 536 Z + 512 P bytes
         # per cycle; at the fused cycle's rate that is this effective bandwidth
         b_tasks = 688 * S + 536 * Z + 512 * P
-        traffic = (profiled_traffic(args.workload, dom, "mixed" if mixed else "fp64")
-                   if args.k1 in (None, 0) else None)
+        pform = {pa.MODE_LEARNED: "learned", pa.MODE_MIXED: "mixed"}.get(mode, "fp64")
+        traffic = profiled_traffic(args.workload, dom, pform) if args.k1 in (None, 0) else None
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
@@ -232,13 +301,15 @@ def main():
                          f"variant {v_ser:.4g} zone-cycles/s on 1 core over {n_ser} cycles",
                "serial_value": round(v_ser, 1)}
 
-    form = {"mixed": "mixed-fp32-qcs-fused", "fused": "plain-fp64-fused", "tasks": "plain-fp64-tasks"}
+    form = {"learned": "mixed-fp32-learned-qcs-fused", "mixed": "mixed-fp32-qcs-fused",
+            "fused": "plain-fp64-fused", "tasks": "plain-fp64-tasks"}
     line = {
         "metric": BASELINE_METRIC,
         "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if mode == pa.MODE_MIXED else "f64", "data": "synthetic",
+        "dtype": {pa.MODE_LEARNED: "bf16+f32+f64", pa.MODE_MIXED: "f32+f64"}.get(mode, "f64"),
+        "data": "synthetic",
         "config": {"workload": wname, "form": form[args.mode], "zones": Z, "sides": S, "points": P},
         "roofline": roofline, "cpu_baseline": cpu,
     }

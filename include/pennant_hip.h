@@ -362,6 +362,11 @@ int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
  * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
 int pennant_get_tuning(pennant_problem* pb, const char* key);
+/* Autotuner: times `cycles` cycles of `mode` at every launch shape (K1
+ * layout for the FP64 form, grid cap 256..4096, XCD mapping) from one
+ * snapshot of the state, keeps the fastest and restores the snapshot (the
+ * problem does not advance).  best_ms (may be NULL) receives its time. */
+int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms);
 
 /* Writes <path> in the reference's .xy format (src/WriteXY.cc:28-53). */
 int pennant_write_xy(pennant_problem* pb, const char* path);

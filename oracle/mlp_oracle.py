@@ -96,6 +96,12 @@ def loss_grad(p, tuples, rounding=True):
     return loss, g
 
 
+def slog(x):
+    """sign(x) log(1 + |x|)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.log1p(np.abs(x))
+
+
 def exact_targets(q1, G, zss, du):
     """Linear and quadratic coefficients whose combination q1 y0 zss + G y1 du
     is the Kurapatenko coefficient zkur = t2 + sqrt(t1^2 + t2^2)."""
@@ -127,16 +133,16 @@ def qcs_tuples(fields: dict, maps: dict, q1: float, q2: float, qgamma: float):
     du = fields["cdu"][c]
     el = fields["elen"]
     f = np.zeros((c.size, NF + NO))
-    f[:, 0:2] = (pu[p] - pu[p1]) / zss[:, None]
-    f[:, 2:4] = (pu[p2] - pu[p]) / zss[:, None]
-    f[:, 4] = cdiv[c] * evol / zss
+    f[:, 0:2] = slog((pu[p] - pu[p1]) / zss[:, None])
+    f[:, 2:4] = slog((pu[p2] - pu[p]) / zss[:, None])
+    f[:, 4] = slog(cdiv[c] * evol / zss)
     f[:, 5] = evol / np.minimum(el[s3], el[c])
     up1 = 0.5 * (pu[p] + pu[p2])
     up3 = 0.5 * (pu[p1] + pu[p])
     duav = 0.25 * (pu[p] + up1 + fields["zuc"][z] + up3)
     f[:, 6] = zss / (zss + np.sqrt(np.sum(duav * duav, axis=1)))
     f[:, 7] = fields["zrp"][z] / fields["zr"][z]
-    f[:, 8] = du / zss
+    f[:, 8] = slog(du / zss)
     tl, tq = q1 * zss, G * du
     den = tl + tq
     f[:, 9] = np.where(den > 0, tq / np.where(den > 0, den, 1.0), 0.0)

@@ -12,11 +12,12 @@
 // with zss the zone sound speed, u the corner point velocities (k the
 // corner, km/kp the previous/next corner of the zone), evol the corner
 // characteristic length, du the corner velocity jump (setCornerDiv,
-// src/QCS.cc:164-241) and G = q2 (qgamma + 1) / 4:
-//   f0,f1 = (u_k - u_km) / zss          f2,f3 = (u_kp - u_k) / zss
-//   f4 = cdiv * evol / zss              f5 = evol / min(elen_km, elen_k)
+// src/QCS.cc:164-241), G = q2 (qgamma + 1) / 4 and slog(x) = sign(x)
+// log(1 + |x|) (which keeps the unbounded ratios within a few units):
+//   f0,f1 = slog((u_k - u_km) / zss)    f2,f3 = slog((u_kp - u_k) / zss)
+//   f4 = slog(cdiv * evol / zss)        f5 = evol / min(elen_km, elen_k)
 //   f6 = zss / (zss + |duav|)           f7 = zrp / zr
-//   f8 = du / zss                       f9 = G du / (q1 zss + G du)
+//   f8 = slog(du / zss)                 f9 = G du / (q1 zss + G du)
 // Outputs: the linear and quadratic viscosity coefficients, scaled by q1 and
 // G, so that the Kurapatenko coefficient of setQCnForce (src/QCS.cc:285-296)
 // becomes  zkur = q1 y0 zss + G y1 du.  The exact values the network is
@@ -64,6 +65,9 @@ struct ClosurePack {
     float hi[PN_MLP_NF];
     float pad[10];
 };
+
+// sign(x) log(1 + |x|)
+__device__ __forceinline__ float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
 
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
     const __bf16 x = (__bf16)a, y = (__bf16)b;

@@ -545,6 +545,59 @@ int pennant_sync(pennant_problem* pb) {
     return bad_sides_error(pb);
 }
 
+// Times `cycles` cycles (after one untimed cycle) at every launch shape from
+// the same snapshot of the state, keeps the fastest shape and restores the
+// snapshot, so the problem does not advance.  Results do not depend on the
+// shape, only the time does.
+int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms) {
+    const int form = fused_form(mode);
+    if (form < 0 || cycles < 1) return PENNANT_EINVAL;
+    if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
+    const size_t Z = (size_t)pb->dm.numz * 8, P = (size_t)pb->dm.nump * 8;
+    const std::pair<void*, size_t> st[] = {
+        {pb->f("px", 0), P}, {pb->f("px", 1), P}, {pb->f("pu", 0), P}, {pb->f("pu", 1), P},
+        {pb->f("zr"), Z}, {pb->f("ze"), Z}, {pb->f("zetot"), Z}, {pb->f("zwrate"), Z},
+        {pb->f("zvol"), Z}, {pb->f("zp"), Z}, {pb->ctl, sizeof(Ctl)}};
+    std::vector<void*> copy;
+    int rc = 0;
+    for (const auto& a : st) {
+        void* c = nullptr;
+        if (!rc) rc = (int)hipMalloc(&c, a.second);
+        if (!rc) copy.push_back(c);
+        if (!rc) rc = (int)hipMemcpyAsync(c, a.first, a.second, hipMemcpyDeviceToDevice, pb->stream);
+    }
+    auto restore = [&]() {
+        int r = 0;
+        for (size_t i = 0; i < copy.size() && !r; ++i)
+            r = (int)hipMemcpyAsync(st[i].first, copy[i], st[i].second, hipMemcpyDeviceToDevice, pb->stream);
+        if (!r) r = (int)hipStreamSynchronize(pb->stream);
+        return r;
+    };
+    const Tuning keep = pb->tune;
+    Tuning best = keep;
+    double bms = 1e300;
+    const int grids[] = {256, 512, 1024, 2048, 4096};
+    for (int layout = 0; layout <= (form == FORM_FP64 ? 1 : 0) && !rc; ++layout)
+        for (int xcd = 0; xcd <= 1 && !rc; ++xcd)
+            for (int g : grids) {
+                if (rc) break;
+                pb->tune.k1_layout = layout;
+                pb->tune.xcd = xcd;
+                pb->tune.max_blocks = g;
+                rc = restore();
+                if (!rc) rc = pennant_launch_cycles(pb, 1, mode);
+                double ms = 0.;
+                if (!rc) rc = pennant_time_cycles(pb, cycles, mode, &ms, nullptr);
+                if (rc == PENNANT_EBADSIDES) rc = 0;
+                if (!rc && ms < bms) { bms = ms; best = pb->tune; }
+            }
+    pb->tune = (bms < 1e300) ? best : keep;
+    const int rc2 = restore();
+    for (void* c : copy) (void)hipFree(c);
+    if (best_ms) *best_ms = bms;
+    return rc ? rc : rc2;
+}
+
 int pennant_run(pennant_problem* pb, int ncycles, int mode) {
     CK(copy_ctl_to_host(pb));
     const int c0 = pb->ctl_host->cycle;

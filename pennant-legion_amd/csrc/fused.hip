@@ -342,8 +342,10 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
 // candidate.  FP32: edge lengths, the per-side pressure/TTS force products
 // and the whole QCS chain (corner divergence, evolution factor, Kurapatenko
 // coefficient, corner weights, edge viscous force).  Coordinate and velocity
-// differences are formed in FP64 and rounded once.  Contraction into fused
-// multiply-adds is allowed here.
+// differences are formed in FP64 and rounded once.  Like every kernel here it
+// is compiled without contraction, so each operation rounds on its own and
+// the results do not depend on how the function is inlined (a batched
+// problem gives the same bits as its own run).
 
 struct f2 {
     float x, y;
@@ -377,7 +379,6 @@ struct ClCount {
 template <int NS, int CL>
 __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
                                                   int& bad, bool valid, ClCount& cnt) {
-#pragma clang fp contract(fast)
     const double dth = 0.5 * dt;
     const double n = (double)NS;
     d2 u0[NS], xp[NS];
@@ -507,15 +508,15 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
             // network features of this corner (closure.h)
             const float inv = 1.f / zssf;
             float f[PN_MLP_NF];
-            f[0] = (ul[k].x - ul[km].x) * inv;
-            f[1] = (ul[k].y - ul[km].y) * inv;
-            f[2] = (ul[kp].x - ul[k].x) * inv;
-            f[3] = (ul[kp].y - ul[k].y) * inv;
-            f[4] = div * evol * inv;
+            f[0] = slog((ul[k].x - ul[km].x) * inv);
+            f[1] = slog((ul[k].y - ul[km].y) * inv);
+            f[2] = slog((ul[kp].x - ul[k].x) * inv);
+            f[3] = slog((ul[kp].y - ul[k].y) * inv);
+            f[4] = slog(div * evol * inv);
             f[5] = evol / minelen;
             f[6] = zssf / (zssf + flen(duav));
             f[7] = zrpf / (float)r;
-            f[8] = du * inv;
+            f[8] = slog(du * inv);
             const float tl = q1f * zssf, tq = gq * du;
             f[9] = (tl + tq > 0.f) ? tq / (tl + tq) : 0.f;
             if (div < 0.f) {
@@ -660,8 +661,8 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_mixed(FusedArgs a, int zbeg, 
 // zone and store nothing), because the network and the capture slots are
 // computed across the wave.
 template <int NS, int CL>
-__global__ void __launch_bounds__(PN_BLOCK) k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg,
-                                                      int poff) {
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg, int poff) {
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
@@ -1305,7 +1306,8 @@ __device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, do
 }
 
 template <int FORM, int CL>
-__global__ void __launch_bounds__(PN_BLOCK) k_pred_batch(BatchArgs B) {
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_pred_batch(BatchArgs B) {
     for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
         const BUnit U = B.units[w.u];
         const Ctl* c = B.ctls + U.pid;

@@ -490,8 +490,10 @@ int pennant_mlp_train(pennant_mlp* m, const float* tuples, int64_t n, int steps,
         MCK(fwd_bwd(m, batch));
         m->adam_t += 1;
         const float c1 = 1.f - std::pow(b1, (float)m->adam_t), c2 = 1.f - std::pow(b2, (float)m->adam_t);
+        // cosine decay from lr to lr/50 over the call's steps
+        const float lrt = lr * (0.02f + 0.98f * 0.5f * (1.f + std::cos(3.14159265f * (float)st / (float)steps)));
         hipLaunchKernelGGL(k_adam, dim3(nb(PN_MLP_NPARAM)), dim3(256), 0, m->stream, PN_MLP_NPARAM, m->p, m->g,
-                           m->m, m->v, lr, b1, b2, eps, c1, c2);
+                           m->m, m->v, lrt, b1, b2, eps, c1, c2);
         hipLaunchKernelGGL(k_pack, dim3(8), dim3(256), 0, m->stream, m->p, m->w1b, m->w2b, m->w3b, m->pack);
         if (loss_out) MCK(hipMemcpyAsync(loss_out + st, m->lossp + nb(batch), 4, hipMemcpyDeviceToHost, m->stream));
     }

@@ -160,6 +160,7 @@ DRIVER_SIGNATURES = {
     "pennant_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_set_tuning": ([_P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "pennant_autotune": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_bench_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                            ctypes.c_int),
     "pennant_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
@@ -283,6 +284,12 @@ class Problem:
     def set_tuning(self, key: str, value: int):
         if lib().pennant_set_tuning(self._h, key.encode(), int(value)) != 0:
             raise ValueError(f"bad tuning {key}={value}")
+
+    def autotune(self, mode: int = MODE_FUSED, cycles: int = 4) -> float:
+        """Picks the fastest launch shape for `mode` (the state is restored)."""
+        ms = ctypes.c_double()
+        self._check(lib().pennant_autotune(self._h, int(mode), int(cycles), ctypes.byref(ms)))
+        return ms.value
 
     def get_tuning(self, key: str) -> int:
         return lib().pennant_get_tuning(self._h, key.encode())

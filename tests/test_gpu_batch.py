@@ -61,19 +61,19 @@ def test_batch_fp64_bit_identical_to_single_runs(graph):
                 assert np.array_equal(bits(b.get(i, f)), bits(want[f])), (i, f)
 
 
-def test_batch_mixed_matches_single_mixed_runs():
+def test_batch_mixed_bit_identical_to_single_mixed_runs():
+    """The mixed form is compiled without contraction, so a batched problem
+    gives the same bits as its own mixed run."""
     decks = mixed_decks()
     with pa.Batch(decks) as b:
         b.run(100, pa.MODE_MIXED)
-        exact = 0
         for i, text in enumerate(decks):
             _, sc, want = single(text, pa.MODE_MIXED)
             assert b.scalars(i)["cycle"] == sc["cycle"]
-            same = all(np.array_equal(bits(b.get(i, f)), bits(want[f])) for f in STATE)
-            exact += same
-            for f in ("zr", "ze", "zp"):
-                assert rel_l1(b.get(i, f), want[f]) <= 1e-9, (i, f)
-        say(f"[batch mixed] {exact} of {len(decks)} problems bit-identical to their own runs")
+            assert bits(b.scalars(i)["time"]) == bits(sc["time"]), i
+            for f in STATE:
+                assert np.array_equal(bits(b.get(i, f)), bits(want[f])), (i, f)
+            assert rel_l1(b.get(i, "zr"), want["zr"]) == 0.0
 
 
 @pytest.mark.parametrize("maxb,xcd", [(1, 0), (7, 1), (64, 1), (65535, 0)])

@@ -277,3 +277,29 @@ def test_task_abi_on_caller_owned_buffers():
         synchronize()
         want = np.min(a * 0.6 / np.maximum(u, np.maximum(s, 1e-99)))
         assert out.numpy()[0] == want
+
+
+@pytest.mark.parametrize("mode", [pa.MODE_FUSED, pa.MODE_MIXED])
+def test_autotune_restores_state_and_keeps_results(mode):
+    """The autotuner times launch shapes from a snapshot and restores it: the
+    problem does not advance, and the run afterwards matches an untuned run
+    bit for bit (the FP64 form also matches the oracle)."""
+    text = wd.noh_pie(60, 40, cstop=30, dtinit=1e-4)
+    with pa.Problem(text) as ref, pa.Problem(text) as pb:
+        ref.run(5, mode)
+        pb.run(5, mode)
+        pb.autotune(mode, cycles=2)
+        say(f"[autotune] mode {mode}: k1 {pb.get_tuning('k1_layout')} max_blocks "
+            f"{pb.get_tuning('max_blocks')} xcd {pb.get_tuning('xcd')}")
+        assert pb.scalars()["cycle"] == 5
+        assert ref.run(100, mode) == pb.run(100, mode) == 25
+        for f in STATE:
+            assert_bits_equal(f, pb.get(f), ref.get(f))
+    if mode == pa.MODE_FUSED:
+        o = Oracle(text)
+        o.run(100)
+        with pa.Problem(text) as pb:
+            pb.autotune(mode, cycles=2)
+            pb.run(100, mode)
+            for f in STATE:
+                assert_bits_equal(f, pb.get(f), o.get(f))

@@ -8,9 +8,9 @@
     the hydro oracle's own crmu (src/QCS.cc:285-296) on Sedov and Noh
     states: crmu = (q1 y0 zss + G y1 du) zrp cevol for every compressing
     corner;
-  * features: f8 = du/zss and f9 are consistent with the targets (y0, y1 are
-    functions of f9 alone), and compressing corners are exactly those with
-    cdiv < 0.
+  * features: f8 = log(1 + du/zss) and f9 are consistent with the targets
+    (y0, y1 are functions of f9 alone), and compressing corners are exactly
+    those with cdiv < 0.
 """
 import sys
 from pathlib import Path
@@ -112,7 +112,7 @@ def test_targets_reproduce_reference_qcs(deck, ncyc):
     nrm = np.sqrt((1 - r) ** 2 + r ** 2)
     np.testing.assert_allclose(tup[:, 10], (1 - r) / nrm, rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(tup[:, 11], 1 + r / nrm, rtol=1e-12, atol=1e-14)
-    np.testing.assert_allclose(tup[:, 8], du / zss, rtol=1e-14)
+    np.testing.assert_allclose(tup[:, 8], np.log1p(du / zss), rtol=1e-14)
     assert np.all(tup[:, 6] > 0) and np.all(tup[:, 6] <= 1)
 
 
