@@ -90,6 +90,9 @@ struct FusedArgs {
     long long cap;
     unsigned long long* cap_count;
     unsigned long long* cl_stats;
+    int* zlist;                // zones for the learned pass, run r at [zbeg_r, zend_r)
+    unsigned* zcount;          // per run
+    int nzruns;
 };
 
 // Work units of a grid-stride launch.  MI355X dispatches workgroups to its 8
@@ -361,15 +364,28 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
-// Closure kinds of the mixed K1: the exact QCS coefficient, the learned one
-// (exact outside the network's envelope), or the exact one while recording
-// training tuples.
-enum { CL_EXACT = 0, CL_LEARNED = 1, CL_CAPTURE = 2 };
+// Closure kinds of the mixed K1:
+//   CL_EXACT    the exact QCS coefficient;
+//   CL_FLAG     the exact coefficient, and the zone is reported (ClCount.flag)
+//               when one of its corners passes the network's gate;
+//   CL_LEARNED  the network's coefficient on the corners that pass the gate
+//               (the exact one elsewhere, and on zones outside the envelope);
+//   CL_RECOMP   CL_LEARNED for a zone that a CL_FLAG pass has already handled:
+//               stores only what the coefficient changes (corner and work
+//               forces);
+//   CL_CAPTURE  the exact coefficient, recording training tuples.
+// The gate: a compressing corner (cdiv < 0) whose quadratic term matters,
+// G du > PN_GATE (q1 zss + G du), i.e. f9 > PN_GATE.  Below it the exact
+// coefficient equals its acoustic limit q1 zss + G du to within 0.1%; the
+// round-off velocities of a quiescent region stay far below it.
+enum { CL_EXACT = 0, CL_LEARNED = 1, CL_CAPTURE = 2, CL_FLAG = 3, CL_RECOMP = 4 };
+#define PN_GATE 1.e-3f
 
 // Zone counters of the learned form, summed per thread.
 struct ClCount {
-    int learned = 0;    // zones with compressing corners evaluated by the network
+    int learned = 0;    // zones with gated corners evaluated by the network
     int fallback = 0;   // such zones outside the envelope (exact coefficient)
+    bool flag = false;  // CL_FLAG: this zone has a gated corner
 };
 
 // The mixed-precision predictor half of zone z (sides sb..sb+NS-1).  With
@@ -445,7 +461,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     const double denom = 1. + 0.5 * per * dv;
     const double src = a.zwrate[z] * dth * minv;
     const double zp = pz + (per * src - r * bulk * dv) / denom;
-    if (valid) a.zp[z] = zp;
+    if (valid && CL != CL_RECOMP) a.zp[z] = zp;
     // pressure and TTS side forces
     const float zpf = (float)zp;
     double sstmp = smax(zss, a.ssmin);
@@ -466,11 +482,14 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     const float zssf = (float)zss;
     const float zt1 = (float)(a.q1 * zss);
     const float zrpf = (float)zrp;
+    // gate threshold on du / zss (no corner passes when G = 0)
+    const float gthr = (gq > 0.f) ? PN_GATE * q1f / ((1.f - PN_GATE) * gq) : 3.4e38f;
     float carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
     u32x4 flo[NS];            // packed network inputs, slots 0-7
     unsigned fhi[NS];         // slots 8-9
-    bool inenv = true;        // every compressing corner inside the envelope
-    bool comp = false;        // some corner compressing
+    bool inenv = true;        // every gated corner inside the envelope
+    bool comp = false;        // some corner passes the gate
+    unsigned gated = 0;       // bit k: corner k passes the gate
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int km = prv<NS>(k), kp = nxt<NS>(k);
@@ -504,7 +523,14 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const float du = __builtin_sqrtf(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
         cevol[k] = (div < 0.f) ? evol : 0.f;
         cdu[k] = (div < 0.f) ? du : 0.f;
-        if constexpr (CL != CL_EXACT) {
+        if constexpr (CL == CL_FLAG || CL == CL_LEARNED || CL == CL_RECOMP) {
+            // G du > PN_GATE (q1 zss + G du)  <=>  du > gthr zss
+            if (div < 0.f && du > gthr * zssf) {
+                gated |= 1u << k;
+                comp = true;
+            }
+        }
+        if constexpr (CL == CL_LEARNED || CL == CL_RECOMP || CL == CL_CAPTURE) {
             // network features of this corner (closure.h)
             const float inv = 1.f / zssf;
             float f[PN_MLP_NF];
@@ -519,15 +545,14 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
             f[8] = slog(du * inv);
             const float tl = q1f * zssf, tq = gq * du;
             f[9] = (tl + tq > 0.f) ? tq / (tl + tq) : 0.f;
-            if (div < 0.f) {
-                comp = true;
-                if constexpr (CL == CL_LEARNED) {
+            if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
+                if (gated & (1u << k)) {
 #pragma unroll
                     for (int j = 0; j < PN_MLP_NF; ++j)
                         inenv = inenv && (f[j] >= a.pack->lo[j]) && (f[j] <= a.pack->hi[j]);
                 }
             }
-            if constexpr (CL == CL_LEARNED) {
+            if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
                 u32x4 hi;
                 pack_features(f, flo[k], hi);
                 fhi[k] = hi[0];
@@ -560,7 +585,8 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const float zkur = zt2 + __builtin_sqrtf(zt2 * zt2 + zt1 * zt1);
         crmu[k] = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
     }
-    if constexpr (CL == CL_LEARNED) {
+    if constexpr (CL == CL_FLAG) cnt.flag = comp;
+    if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
         const bool use = valid && comp && inenv;
         if (valid && comp) {
             if (inenv) cnt.learned += 1;
@@ -568,7 +594,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const bool need = use && cdiv[k] < 0.f;
+            const bool need = use && ((gated >> k) & 1u);
             const uint64_t m = __ballot(need);
             if (m) {
                 u32x4 hi;
@@ -625,34 +651,90 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
             const d2 cf = sub(tot[k], tot[km]);
             a.fcfx[slot] = (float)cf.x;
             a.fcfy[slot] = (float)cf.y;
-            a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+            if constexpr (CL != CL_RECOMP)
+                a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
         }
     }
     const double cdt = smax(zdu, smax(zss, 1.e-99));
     return (double)zdl * a.cfl / cdt;
 }
 
-template <int NS>
-__global__ void __launch_bounds__(PN_BLOCK) k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg,
-                                                         int poff) {
+// K1, mixed precision.  With FLAG, zones with a gated corner are appended to
+// the run's zone list (a.zlist[zbeg ..], count a.zcount[run]) for the
+// learned pass; their order in the list is not fixed, their results are.
+template <int NS, bool FLAG>
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
     int bad = 0;
-    ClCount cnt;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
     for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
         const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
-        if (z >= zend) continue;
-        const double zdthyd =
-            zone_pred_mixed<NS, CL_EXACT>(a, z, sbeg + (z - zbeg) * NS, dt, bad, true, cnt);
-        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+        ClCount cnt;
+        if (z < zend) {
+            const double zdthyd = zone_pred_mixed<NS, FLAG ? CL_FLAG : CL_EXACT>(
+                a, z, sbeg + (z - zbeg) * NS, dt, bad, true, cnt);
+            dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+        }
+        if constexpr (FLAG) {
+            const uint64_t m = __ballot(cnt.flag);
+            if (m) {
+                unsigned base = 0;
+                if ((threadIdx.x & 63) == 0) base = atomicAdd(a.zcount + run, (unsigned)__popcll(m));
+                base = __shfl(base, 0, 64);
+                if (cnt.flag) {
+                    const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
+                    a.zlist[zbeg + base + (unsigned)__popcll(below)] = z;
+                }
+            }
+        }
     }
     const double m = block_min(dtmin);
     const int b = block_sum(bad);
     if (threadIdx.x == 0) {
         a.part_dt[poff + blockIdx.x] = m;
         a.part_bp[poff + blockIdx.x] = b;
+    }
+}
+
+// Learned pass over the zones a flagging K1 listed for this run: recomputes
+// each listed zone with the network's coefficient on its gated corners and
+// rewrites its corner and work forces.  The packed weights are staged in LDS
+// once per block.  Every lane takes part in every unit (the network runs
+// across the wave); lanes past the list's end repeat its first zone and
+// store nothing.
+template <int NS>
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
+    if (a.ctl->done) return;
+    const int count = (int)a.zcount[run];
+    if (count == 0) return;
+    __shared__ ClosurePack wp;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(a.pack);
+        uint4* dst = reinterpret_cast<uint4*>(&wp);
+        for (int i = threadIdx.x; i < (int)(sizeof(ClosurePack) / 16); i += PN_BLOCK) dst[i] = src[i];
+    }
+    __syncthreads();
+    FusedArgs b = a;
+    b.pack = &wp;
+    const double dt = a.ctl->dt;
+    int bad = 0;
+    ClCount cnt;
+    const int nunits = (count + PN_BLOCK - 1) / PN_BLOCK;
+    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+        const int i = w.u * PN_BLOCK + threadIdx.x;
+        const bool valid = i < count;
+        const int z = a.zlist[zbeg + (valid ? i : 0)];
+        (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, dt, bad, valid, cnt);
+    }
+    const int nl = block_sum(cnt.learned);
+    const int nf = block_sum(cnt.fallback);
+    if (threadIdx.x == 0 && (nl | nf)) {
+        atomicAdd(a.cl_stats, (unsigned long long)nl);
+        atomicAdd(a.cl_stats + 1, (unsigned long long)nf);
     }
 }
 
@@ -1183,6 +1265,8 @@ __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, co
 __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
                                                            double cflv, int cstop, double dtmax,
                                                            double dtfac, double tstop) {
+    // the learned pass of this cycle has consumed its zone lists
+    if (a.zcount && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
     finalize_block(a.ctl, a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, cflv, cstop,
                    dtmax, dtfac, tstop);
 }
@@ -1213,6 +1297,9 @@ FusedArgs make_args(pennant_problem* pb) {
     a.cap = pb->cap;
     a.cap_count = pb->cl_counters;
     a.cl_stats = pb->cl_counters + 1;
+    a.zlist = pb->zlist;
+    a.zcount = pb->zcount;
+    a.nzruns = (int)pb->zruns.size() / 3;
     return a;
 }
 
@@ -1234,14 +1321,14 @@ int corr_grid(const pennant_problem* pb, int nz) {
 
 template <int NS>
 void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, int sb, int poff,
-                 int form, hipStream_t s) {
+                 int form, int run, hipStream_t s) {
     const dim3 grid(pred_grid(pb, ze - zb, NS, form));
     if (form == FORM_LEARNED)
-        hipLaunchKernelGGL((k_pred_cl<NS, CL_LEARNED>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+        hipLaunchKernelGGL((k_pred_mixed<NS, true>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff, run);
     else if (form == FORM_MIXED && a.cap_buf)
         hipLaunchKernelGGL((k_pred_cl<NS, CL_CAPTURE>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else if (form == FORM_MIXED)
-        hipLaunchKernelGGL(k_pred_mixed<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+        hipLaunchKernelGGL((k_pred_mixed<NS, false>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff, run);
     else if (pb->tune.k1_layout == 1)
         hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
     else
@@ -1491,7 +1578,9 @@ int fused_setup(pennant_problem* pb) {
     pb->npart_pred = nruns * PN_MAX_GRID;
     pb->npart_corr = nruns * PN_MAX_GRID;
     pb->cl_counters = (unsigned long long*)al(3, 8);
-    if (!pb->cl_counters) return (int)hipErrorOutOfMemory;
+    pb->zlist = (int*)al((size_t)h.numz, 4);
+    pb->zcount = (unsigned*)al(pb->zruns.size() / 3 + 1, 4);
+    if (!pb->cl_counters || !pb->zlist || !pb->zcount) return (int)hipErrorOutOfMemory;
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
     if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
@@ -1516,17 +1605,34 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
         switch (ns) {
-            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, form, s); break;
-            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, form, s); break;
-            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, form, s); break;
-            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, form, s); break;
-            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, form, s); break;
-            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, form, s); break;
+            case 3: launch_pred<3>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
+            case 4: launch_pred<4>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
+            case 5: launch_pred<5>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
+            case 6: launch_pred<6>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
+            case 7: launch_pred<7>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
+            case 8: launch_pred<8>(pb, a, zb, ze, sb, poff, form, (int)(r / 3), s); break;
             default: return PENNANT_EINVAL;
         }
         poff += pred_grid(pb, ze - zb, ns, form);
     }
     const int npred = poff;
+    if (form == FORM_LEARNED) {
+        for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+            const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+            const int sb = h.zsbeg[zb];
+            const dim3 grid(pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
+            const int run = (int)(r / 3);
+            switch (ns) {
+                case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, a, zb, sb, run); break;
+                default: return PENNANT_EINVAL;
+            }
+        }
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
     const dim3 pgrid(pb->nheavy + nblocks(h.nump));
     if (mixed)

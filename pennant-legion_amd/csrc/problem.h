@@ -101,6 +101,8 @@ struct pennant_problem {
     float* cap_buf = nullptr;
     int64_t cap = 0;
     unsigned long long* cl_counters = nullptr;   // [0] tuples, [1] learned zones, [2] fallback zones
+    int* zlist = nullptr;            // zones for the learned pass (per run, at the run's zone range)
+    unsigned* zcount = nullptr;      // per run
     pn::Tuning tune;
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;

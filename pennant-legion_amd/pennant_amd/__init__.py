@@ -430,8 +430,23 @@ class Batch:
     def launch(self, ncycles: int, mode: int = MODE_MIXED):
         self._check(lib().pennant_batch_launch_cycles(self._h, int(ncycles), int(mode)))
 
-    def sync(self):
-        self._check(lib().pennant_batch_sync(self._h))
+    def sync(self, raise_on_error: bool = True):
+        """Waits.  A problem that stopped on negative side volumes raises
+        PennantError unless raise_on_error is False (the other problems run
+        on regardless; stopped() lists them)."""
+        rc = lib().pennant_batch_sync(self._h)
+        if rc == EBADSIDES and not raise_on_error:
+            return
+        self._check(rc)
+
+    def stopped(self):
+        """Indices of the problems that stopped on negative side volumes."""
+        out = []
+        for i in range(self.count):
+            s = self.scalars(i)
+            if s["numsbad_pred"] > 0 or s["numsbad_corr"] > 0:
+                out.append(i)
+        return out
 
     def active(self) -> int:
         n = lib().pennant_batch_active(self._h)
@@ -439,9 +454,9 @@ class Batch:
             self._check(n)
         return n
 
-    def run(self, ncycles: int, mode: int = MODE_MIXED):
+    def run(self, ncycles: int, mode: int = MODE_MIXED, raise_on_error: bool = True):
         self.launch(ncycles, mode)
-        self.sync()
+        self.sync(raise_on_error)
 
     def run_to_end(self, mode: int = MODE_MIXED, chunk: int = 256, max_cycles: int = 10 ** 7):
         done = 0

@@ -40,9 +40,9 @@ def capture_ensemble(count: int, seed: int, cycles: int, every: int, cap: int, m
     with Batch(decks) as b:
         done = 0
         while done < cycles:
-            b.run(every - 1, MODE_MIXED)
+            b.run(every - 1, MODE_MIXED, raise_on_error=False)
             b.set_capture(buf, cap)
-            b.run(1, MODE_MIXED)
+            b.run(1, MODE_MIXED, raise_on_error=False)
             k = min(b.capture_count(), cap)
             b.set_capture(None, 0)
             done += every
@@ -51,6 +51,9 @@ def capture_ensemble(count: int, seed: int, cycles: int, every: int, cap: int, m
                 if k > per:
                     t = t[np.sort(rng.choice(k, per, replace=False))]
                 out.append(t)
+        st = b.stopped()
+        if st:
+            log(f"problems stopped on negative side volumes (their earlier tuples are kept): {st}")
     tup = np.concatenate(out) if out else np.zeros((0, TUPLE), np.float32)
     log(f"captured {tup.shape[0]} tuples in {time.perf_counter() - t0:.1f}s")
     return tup

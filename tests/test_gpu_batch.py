@@ -128,3 +128,30 @@ def test_ensemble_runs_and_captures_tuples():
             sc = b.scalars(i)
             assert sc["cycle"] == 33
             assert np.all(np.isfinite(b.get(i, "zr")))
+
+
+def test_batch_learned_bit_identical_to_single_learned_runs():
+    """The batch evaluates the closure in one pass, a single problem in two
+    (flag, then recompute the flagged zones); both apply the network to the
+    same gated corners and give the same bits."""
+    from pennant_amd.closure import Mlp
+    from pennant_amd.train_closure import DEFAULT_OUT
+    m = Mlp.load(DEFAULT_OUT)
+    decks = mixed_decks()[:6]
+    with pa.Batch(decks) as b:
+        b.set_closure(m)
+        b.run(60, pa.MODE_LEARNED)
+        nb = b.closure_stats()
+        tot = [0, 0]
+        for i, text in enumerate(decks):
+            with pa.Problem(text) as pb:
+                pb.set_closure(m)
+                pb.run(60, pa.MODE_LEARNED)
+                st = pb.closure_stats()
+                tot[0] += st[0]
+                tot[1] += st[1]
+                for f in STATE:
+                    assert np.array_equal(bits(b.get(i, f)), bits(pb.get(f))), (i, f)
+        say(f"[batch learned] zone evaluations batch {nb}, single runs {tuple(tot)}")
+        assert nb == tuple(tot) and nb[0] > 0
+    m.close()
