@@ -76,6 +76,45 @@ def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False):
     }
 
 
+def bench_ensemble(args, pa, decks):
+    """The batched ensemble: 256 Sedov (rect) and Noh (pie) problems of
+    log-uniform sizes 32^2..512^2 advanced together, one launch per phase,
+    16-cycle HIP graphs, autotuned grid.  value = zone-cycles actually
+    advanced per second (problems that stop early stop counting)."""
+    mode = {"learned": pa.MODE_LEARNED, "mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED}[args.mode]
+    texts, n, _ = decks.ensemble(256, cstop=10 ** 7)
+    t0 = time.perf_counter()
+    b = pa.Batch(texts)
+    log(f"ensemble setup {time.perf_counter() - t0:.1f}s: 256 problems, {int((n.astype('int64') ** 2).sum())} zones")
+    if mode == pa.MODE_LEARNED:
+        from pennant_amd.closure import Mlp
+        from pennant_amd.train_closure import DEFAULT_OUT
+        b.set_closure(Mlp.load(Path(args.closure) if args.closure else DEFAULT_OUT))
+    ms_tune = b.autotune(mode, cycles=4)
+    log(f"autotune: max_blocks {b.get_tuning('max_blocks')} xcd {b.get_tuning('xcd')} ({ms_tune:.3f} ms / 4 cycles)")
+    b.run(args.warmup, mode, raise_on_error=False)
+    zones = [b.sizes(i)[1] for i in range(256)]
+    c0 = [b.scalars(i)["cycle"] for i in range(256)]
+    total_ms = b.time_cycles(args.steps, mode, raise_on_error=False)
+    c1 = [b.scalars(i)["cycle"] for i in range(256)]
+    zc = sum(z * (e - s) for z, s, e in zip(zones, c0, c1))
+    value = zc / (total_ms * 1e-3)
+    stopped = b.stopped()
+    log(f"ensemble: {args.steps} cycles in {total_ms:.2f} ms, {zc:.4g} zone-cycles -> {value:.4e} zone-cycles/s; "
+        f"stopped on bad sides: {stopped}")
+    line = {"metric": "zone-cycles/s on 1 MI355X, batched ensemble of 256 Sedov/Noh problems",
+            "value": round(value, 1), "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(total_ms / args.steps, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": {pa.MODE_LEARNED: "bf16+f32+f64", pa.MODE_MIXED: "f32+f64"}.get(mode, "f64"),
+            "data": "synthetic",
+            "config": {"workload": "ensemble-256 (std::mt19937_64 20261202)", "form": args.mode,
+                       "zones": int(sum(zones)), "problems": 256, "stopped_on_bad_sides": len(stopped),
+                       "max_blocks": b.get_tuning("max_blocks"), "xcd": b.get_tuning("xcd"), "graph": 1}}
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
 def physics_check(pa, decks, mlp, n):
     """Sedov n x n to t = 1 in the plain FP64, mixed and learned forms: density
     L1 error, shock radius (densest zone) and total energy against FP64."""
@@ -185,6 +224,9 @@ def main():
     from pennant_amd import decks
     from pennant_amd.accuracy import field_errors
 
+    if args.workload == "ensemble256":
+        bench_ensemble(args, pa, decks)
+        return
     text, wname = workload_deck(args.workload, decks)
     modes = {"learned": pa.MODE_LEARNED, "mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED,
              "tasks": pa.MODE_TASKS}

@@ -359,6 +359,8 @@ int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
  *   "max_blocks" grid cap of the grid-stride zone kernels, 1..4096 (2048)
  *   "xcd"        1: each of the 8 XCDs takes a contiguous band of zones (default),
  *                0: plain grid-stride order
+ *   "graph"      1: pennant_launch_cycles replays runs of 16 fused cycles as one
+ *                captured HIP graph (default), 0: direct launches
  * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
 int pennant_get_tuning(pennant_problem* pb, const char* key);

@@ -347,6 +347,8 @@ int bad_sides_error(pennant_problem* pb) {
 
 }  // namespace
 
+static void drop_graph(pennant_problem* pb);
+
 // ===========================================================================
 extern "C" {
 
@@ -439,6 +441,7 @@ pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen)
 void pennant_destroy(pennant_problem* pb) {
     if (!pb) return;
     if (pb->stream) (void)hipStreamSynchronize(pb->stream);
+    drop_graph(pb);
     for (void* p : pb->allocs) (void)hipFree(p);
     if (pb->ctl_host) (void)hipHostFree(pb->ctl_host);
     if (pb->stream) (void)hipStreamDestroy(pb->stream);
@@ -449,6 +452,34 @@ const char* pennant_error(pennant_problem* pb) { return pb->err.c_str(); }
 
 void pennant_sizes(pennant_problem* pb, int64_t* out4) {
     out4[0] = pb->dm.nump; out4[1] = pb->dm.numz; out4[2] = pb->dm.nums; out4[3] = (int64_t)pb->bcs.size();
+}
+
+#define PN_GRAPH_CYCLES 16
+
+static void drop_graph(pennant_problem* pb) {
+    if (pb->gexec) (void)hipGraphExecDestroy(pb->gexec);
+    pb->gexec = nullptr;
+    pb->graph_form = -1;
+}
+
+// Captures PN_GRAPH_CYCLES fused cycles of `form` as one HIP graph (kernel
+// nodes only: the fused cycle issues no memset and no host work).
+static int build_graph(pennant_problem* pb, int form) {
+    drop_graph(pb);
+    hipGraph_t g = nullptr;
+    CK(hipStreamBeginCapture(pb->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < PN_GRAPH_CYCLES && !rc; ++k) rc = fused_cycle(pb, nullptr, form);
+    const int rc2 = (int)hipStreamEndCapture(pb->stream, &g);
+    if (rc || rc2) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rc2;
+    }
+    rc = (int)hipGraphInstantiate(&pb->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (rc) { pb->gexec = nullptr; return rc; }
+    pb->graph_form = form;
+    return 0;
 }
 
 // fused-cycle form of a driver mode, or -1 for task mode / unknown modes
@@ -464,7 +495,13 @@ static int fused_form(int mode) {
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
     const int form = fused_form(mode);
     if (form >= 0) {
-        for (int k = 0; k < ncycles; ++k) CK(fused_cycle(pb, nullptr, form));
+        if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
+        int left = ncycles;
+        if (pb->tune.graph && left >= PN_GRAPH_CYCLES) {
+            if (pb->graph_form != form || !pb->gexec) CK(build_graph(pb, form));
+            for (; left >= PN_GRAPH_CYCLES; left -= PN_GRAPH_CYCLES) CK(hipGraphLaunch(pb->gexec, pb->stream));
+        }
+        for (; left > 0; --left) CK(fused_cycle(pb, nullptr, form));
         return 0;
     }
     if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
@@ -515,6 +552,12 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
 }
 
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
+    drop_graph(pb);
+    if (!strcmp(key, "graph")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        pb->tune.graph = value;
+        return 0;
+    }
     if (!strcmp(key, "k1_layout")) {
         if (value != 0 && value != 1) return PENNANT_EINVAL;
         pb->tune.k1_layout = value;
@@ -537,6 +580,7 @@ int pennant_get_tuning(pennant_problem* pb, const char* key) {
     if (!strcmp(key, "k1_layout")) return pb->tune.k1_layout;
     if (!strcmp(key, "max_blocks")) return pb->tune.max_blocks;
     if (!strcmp(key, "xcd")) return pb->tune.xcd;
+    if (!strcmp(key, "graph")) return pb->tune.graph;
     return PENNANT_EINVAL;
 }
 
@@ -576,6 +620,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     const Tuning keep = pb->tune;
     Tuning best = keep;
     double bms = 1e300;
+    drop_graph(pb);
     const int grids[] = {256, 512, 1024, 2048, 4096};
     for (int layout = 0; layout <= (form == FORM_FP64 ? 1 : 0) && !rc; ++layout)
         for (int xcd = 0; xcd <= 1 && !rc; ++xcd)
@@ -584,6 +629,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
                 pb->tune.k1_layout = layout;
                 pb->tune.xcd = xcd;
                 pb->tune.max_blocks = g;
+                pb->tune.graph = 0;               // direct launches while shapes change
                 rc = restore();
                 if (!rc) rc = pennant_launch_cycles(pb, 1, mode);
                 double ms = 0.;
@@ -592,6 +638,8 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
                 if (!rc && ms < bms) { bms = ms; best = pb->tune; }
             }
     pb->tune = (bms < 1e300) ? best : keep;
+    pb->tune.graph = keep.graph;
+    drop_graph(pb);
     const int rc2 = restore();
     for (void* c : copy) (void)hipFree(c);
     if (best_ms) *best_ms = bms;
@@ -848,6 +896,7 @@ const void* pennant_mlp_device_pack(pennant_mlp* m);
 
 int pennant_set_closure(pennant_problem* pb, pennant_mlp* m) {
     pb->closure = m ? pennant_mlp_device_pack(m) : nullptr;
+    drop_graph(pb);
     return 0;
 }
 
@@ -865,6 +914,7 @@ int pennant_set_capture(pennant_problem* pb, float* dev_buf, int64_t cap) {
     CK(hipStreamSynchronize(pb->stream));
     pb->cap_buf = dev_buf;
     pb->cap = cap;
+    drop_graph(pb);
     const unsigned long long zero = 0;
     return (int)hipMemcpy(pb->cl_counters, &zero, sizeof zero, hipMemcpyHostToDevice);
 }

@@ -104,6 +104,8 @@ struct pennant_problem {
     int* zlist = nullptr;            // zones for the learned pass (per run, at the run's zone range)
     unsigned* zcount = nullptr;      // per run
     pn::Tuning tune;
+    hipGraphExec_t gexec = nullptr;  // captured fused cycles of form graph_form
+    int graph_form = -1;
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
     double bcvx[8], bcvy[8];

@@ -465,9 +465,11 @@ class Batch:
             done += chunk
         return done
 
-    def time_cycles(self, ncycles: int, mode: int = MODE_MIXED) -> float:
+    def time_cycles(self, ncycles: int, mode: int = MODE_MIXED, raise_on_error: bool = True) -> float:
         ms = ctypes.c_double()
-        self._check(lib().pennant_batch_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(ms)))
+        rc = lib().pennant_batch_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(ms))
+        if not (rc == EBADSIDES and not raise_on_error):
+            self._check(rc)
         return ms.value
 
     def scalars(self, i: int) -> dict:

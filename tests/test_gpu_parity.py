@@ -303,3 +303,28 @@ def test_autotune_restores_state_and_keeps_results(mode):
             pb.run(100, mode)
             for f in STATE:
                 assert_bits_equal(f, pb.get(f), o.get(f))
+
+
+@pytest.mark.parametrize("mode", [pa.MODE_FUSED, pa.MODE_MIXED])
+def test_graph_replay_matches_direct_launches(mode):
+    """Runs of 16 cycles replay one captured HIP graph; the result equals
+    direct launches bit for bit (and the oracle in the FP64 form), also when
+    the problem stops inside a replayed run."""
+    text = wd.sedov_rect(50, cstop=45)
+    with pa.Problem(text) as g, pa.Problem(text) as d:
+        d.set_tuning("graph", 0)
+        g.launch(64, mode)
+        d.launch(64, mode)
+        g.sync()
+        d.sync()
+        assert g.scalars()["cycle"] == d.scalars()["cycle"] == 45
+        for f in STATE:
+            assert_bits_equal(f, g.get(f), d.get(f))
+    if mode == pa.MODE_FUSED:
+        o = Oracle(text)
+        o.run(100)
+        with pa.Problem(text) as g:
+            g.launch(48, mode)
+            g.sync()
+            for f in STATE:
+                assert_bits_equal(f, g.get(f), o.get(f))
