@@ -82,7 +82,7 @@ struct FusedArgs {
     // physics parameters
     double gamma, ssmin, alfa, qgamma, q1, q2, cfl;
     int nbcs;
-    double bcvx[8], bcvy[8];
+    const double* bcv;         // BC plane normals: x components [0, 8), y components [8, 16)
     int xcd;   // XCD-aware unit mapping in the grid-stride kernels
     // learned closure and training-tuple capture
     const ClosurePack* pack;
@@ -710,7 +710,9 @@ __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2
 k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
     if (a.ctl->done) return;
     const int count = (int)a.zcount[run];
-    if (count == 0) return;
+    const int nunits = (count + PN_BLOCK - 1) / PN_BLOCK;
+    Units w = block_units(nunits, a.xcd);
+    if (w.u >= w.end) return;             // no listed zones for this block
     __shared__ ClosurePack wp;
     {
         const uint4* src = reinterpret_cast<const uint4*>(a.pack);
@@ -723,8 +725,7 @@ k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
     const double dt = a.ctl->dt;
     int bad = 0;
     ClCount cnt;
-    const int nunits = (count + PN_BLOCK - 1) / PN_BLOCK;
-    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
+    for (; w.u < w.end; w.u += w.step) {
         const int i = w.u * PN_BLOCK + threadIdx.x;
         const bool valid = i < count;
         const int z = a.zlist[zbeg + (valid ? i : 0)];
@@ -997,7 +998,7 @@ __device__ __forceinline__ void point_update(const FusedArgs& a, int p, double m
     if (bits) {
         for (int b = 0; b < a.nbcs; ++b) {
             if (bits & (1u << b)) {
-                const d2 v = mk(a.bcvx[b], a.bcvy[b]);
+                const d2 v = mk(a.bcv[b], a.bcv[8 + b]);
                 u0 = project(u0, v);
                 f = project(f, v);
             }
@@ -1290,7 +1291,7 @@ FusedArgs make_args(pennant_problem* pb) {
     a.gamma = q.gamma; a.ssmin = q.ssmin; a.alfa = q.alfa; a.qgamma = q.qgamma;
     a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
     a.nbcs = pb->nbcs;
-    for (int i = 0; i < 8; ++i) { a.bcvx[i] = pb->bcvx[i]; a.bcvy[i] = pb->bcvy[i]; }
+    a.bcv = pb->bcv_dev;
     a.xcd = pb->tune.xcd;
     a.pack = (const ClosurePack*)pb->closure;
     a.cap_buf = pb->cap_buf;
@@ -1366,8 +1367,7 @@ __device__ __forceinline__ void apply_prob(FusedArgs& a, const BProb& q) {
     a.gamma = q.gamma; a.ssmin = q.ssmin; a.alfa = q.alfa; a.qgamma = q.qgamma;
     a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
     a.nbcs = q.nbcs;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) { a.bcvx[i] = q.bcvx[i]; a.bcvy[i] = q.bcvy[i]; }
+    a.bcv = q.bcvx;            // q.bcvy follows q.bcvx in memory
 }
 
 template <int NS, int FORM, int CL>
@@ -1393,7 +1393,8 @@ __device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, do
 }
 
 template <int FORM, int CL>
-__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(PN_BLOCK)
+__attribute__((amdgpu_waves_per_eu((CL == CL_LEARNED) ? 2 : ((FORM == FORM_MIXED && CL == CL_EXACT) ? 4 : 1))))
 k_pred_batch(BatchArgs B) {
     for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
         const BUnit U = B.units[w.u];
@@ -1589,6 +1590,11 @@ int fused_setup(pennant_problem* pb) {
     if (!heavy.empty() &&
         hipMemcpy(pb->heavy, heavy.data(), heavy.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return (int)hipErrorUnknown;
+    double bcv[16];
+    for (int i = 0; i < 8; ++i) { bcv[i] = pb->bcvx[i]; bcv[8 + i] = pb->bcvy[i]; }
+    pb->bcv_dev = (double*)al(16, 8);
+    if (!pb->bcv_dev || hipMemcpy(pb->bcv_dev, bcv, sizeof bcv, hipMemcpyHostToDevice) != hipSuccess)
+        return (int)hipErrorOutOfMemory;
     return (int)hipMemcpy(pb->pbc, bits.data(), bits.size(), hipMemcpyHostToDevice);
 }
 

@@ -109,6 +109,7 @@ struct pennant_problem {
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
     double bcvx[8], bcvy[8];
+    double* bcv_dev = nullptr;       // device copy: bcvx then bcvy
 
     double* f(const char* name, int comp = 0) { return fields.at(name).c[comp]; }
     pennant_cvec cv(const char* name) { pn::DevField& d = fields.at(name); pennant_cvec v = {d.c[0], d.c[1]}; return v; }
