@@ -151,7 +151,7 @@ def physics_check(pa, decks, mlp, n):
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
     "mixed": {"zone_pred": "k_pred_mixed<4>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
-    "learned": {"zone_pred": "k_pred_cl<4, 1>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
+    "learned": {"zone_pred": "k_pred_mixed<4, true>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
 }
 
 

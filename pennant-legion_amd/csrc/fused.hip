@@ -359,7 +359,9 @@ __device__ __forceinline__ f2 fadd(f2 a, f2 b) { return fmk(a.x + b.x, a.y + b.y
 __device__ __forceinline__ f2 fscl(float r, f2 v) { return fmk(v.x * r, v.y * r); }
 __device__ __forceinline__ float fdot(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
 __device__ __forceinline__ float fcross(f2 a, f2 b) { return a.x * b.y - a.y * b.x; }
-__device__ __forceinline__ float flen(f2 v) { return __builtin_sqrtf(v.x * v.x + v.y * v.y); }
+// the hardware square root (1 ulp), without the correctly rounded expansion
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float flen(f2 v) { return fsqrt(v.x * v.x + v.y * v.y); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
@@ -451,9 +453,9 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     const double r = a.zr[z];
     const double e = smax(a.ze[z], 0.);
     const double pz = gm1 * r * e;
-    const double pre = gm1 * e;
     const double per = gm1 * r;
-    const double csqd = smax(ssmin2, pre + per * pz / (r * r));
+    // pre + per pz / r^2 = (gamma - 1) e + (gamma - 1)^2 e = gamma (gamma - 1) e
+    const double csqd = smax(ssmin2, a.gamma * gm1 * e);
     const double zss = __builtin_sqrt(csqd);
     const double minv = 1. / zm;
     const double dv = (zvolp - a.zvol[z]) * minv;
@@ -471,8 +473,10 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     for (int k = 0; k < NS; ++k) {
         const f2 ss = fmk(-exl[k].y, exl[k].x);                       // rotCCW(ex - zx)
         sfp[k] = fscl(-zpf, ss);
-        const double drho = zrp * (double)mf[k] * (zarea / sa[k]) - zrp;   // srho - r
-        sft[k] = fscl(-(float)(sstmp * drho), ss);
+        // srho - r = zrp (mf zarea - sa) / sa: the difference in FP64, the
+        // division in FP32 (relative to the difference itself)
+        const float drho = (float)(zrp * ((double)mf[k] * zarea - sa[k])) * frcp((float)sa[k]);
+        sft[k] = fscl(-(float)sstmp * drho, ss);
     }
     // QCS corner quantities (corner k: xp0 = x[k], xp1 = ex[k], xp2 = zx,
     // xp3 = ex[k-1]; up2 = zuc is the frame's zero velocity)
@@ -518,9 +522,9 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const float t2 = fabsf(fdot(dxx2, duav) * dx1);
         const float num = (t1 > t2) ? dx1 : dx2;
         const float den = (t1 > t2) ? dx2 : dx1;
-        float evol = __builtin_sqrtf(4.f * cvol * num * frcp(den));
+        float evol = fsqrt(4.f * cvol * num * frcp(den));
         evol = fminr(evol, 2.f * minelen);
-        const float du = __builtin_sqrtf(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
+        const float du = fsqrt(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
         cevol[k] = (div < 0.f) ? evol : 0.f;
         cdu[k] = (div < 0.f) ? du : 0.f;
         if constexpr (CL == CL_FLAG || CL == CL_LEARNED || CL == CL_RECOMP) {
@@ -566,7 +570,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
                     const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
                     const unsigned long long slot = base + (unsigned long long)__popcll(below);
                     if (valid && div < 0.f && slot < (unsigned long long)a.cap) {
-                        const float s2 = __builtin_sqrtf(tl * tl + tq * tq);
+                        const float s2 = fsqrt(tl * tl + tq * tq);
                         float* o = a.cap_buf + slot * PN_MLP_TUPLE;
 #pragma unroll
                         for (int j = 0; j < PN_MLP_NF; ++j) o[j] = f[j];
@@ -582,7 +586,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const float zt2 = gq * cdu[k];
-        const float zkur = zt2 + __builtin_sqrtf(zt2 * zt2 + zt1 * zt1);
+        const float zkur = zt2 + fsqrt(zt2 * zt2 + zt1 * zt1);
         crmu[k] = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
     }
     if constexpr (CL == CL_FLAG) cnt.flag = comp;
@@ -1626,7 +1630,10 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
         for (size_t r = 0; r < pb->zruns.size(); r += 3) {
             const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
             const int sb = h.zsbeg[zb];
-            const dim3 grid(pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
+            // the learned pass holds 2 blocks per CU (registers): 512 blocks
+            // fill the chip without dispatching blocks that find no work
+            const int g = pred_grid(pb, ze - zb, ns, form);
+            const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
             const int run = (int)(r / 3);
             switch (ns) {
                 case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, a, zb, sb, run); break;
