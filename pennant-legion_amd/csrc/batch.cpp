@@ -161,8 +161,7 @@ pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbu
     b->pbc = (unsigned char*)al(P, 1);
     b->cfx = (double*)al(S, 8); b->cfy = (double*)al(S, 8); b->cmass = (double*)al(S, 8);
     b->sfwx = (double*)al(S, 8); b->sfwy = (double*)al(S, 8);
-    b->fcfx = (float*)al(S, 4); b->fcfy = (float*)al(S, 4); b->fcmass = (float*)al(S, 4);
-    b->fsfwx = (float*)al(S, 4); b->fsfwy = (float*)al(S, 4);
+    b->fcrn = (float*)al(3 * S, 4); b->fsfw = (float*)al(2 * S, 4);
     b->vsx = (double*)al(P, 8); b->vsy = (double*)al(P, 8); b->pxpx = (double*)al(P, 8);
     b->ctls = (Ctl*)al((size_t)n, sizeof(Ctl));
     b->probs = (BProb*)al((size_t)n, sizeof(BProb));

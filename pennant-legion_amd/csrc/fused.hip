@@ -69,7 +69,8 @@ struct FusedArgs {
     // per-side intermediates
     double* cfx; double* cfy; double* cmass; double* sfwx; double* sfwy;
     // the mixed-precision form's FP32 counterparts, and smf in FP32
-    float* fcfx; float* fcfy; float* fcmass; float* fsfwx; float* fsfwy;
+    float* fcrn;               // per corner slot {fx, fy, mass} (point-corner CSR order)
+    float* fsfw;               // per side {x, y}
     const float* fsmf;
     // per-point intermediates
     double* vsx; double* vsy; double* pxpx;
@@ -366,6 +367,37 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
+// The work forces of a zone's sides, {x, y} per side.  Quad zones whose
+// first side is even start on a 16-byte boundary: two 16-byte accesses.
+template <int NS>
+__device__ __forceinline__ void store_sides(float* w, int sb, const f2 (&v)[NS]) {
+    if (NS == 4 && (sb & 1) == 0) {
+        float4* q = reinterpret_cast<float4*>(w + 2 * sb);
+        q[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+        q[1] = make_float4(v[2].x, v[2].y, v[3 % NS].x, v[3 % NS].y);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            w[2 * (sb + k)] = v[k].x;
+            w[2 * (sb + k) + 1] = v[k].y;
+        }
+    }
+}
+template <int NS>
+__device__ __forceinline__ void load_sides(const float* w, int sb, f2 (&v)[NS]) {
+    if (NS == 4 && (sb & 1) == 0) {
+        const float4* q = reinterpret_cast<const float4*>(w + 2 * sb);
+        const float4 a = q[0], b = q[1];
+        v[0] = fmk(a.x, a.y);
+        v[1 % NS] = fmk(a.z, a.w);
+        v[2 % NS] = fmk(b.x, b.y);
+        v[3 % NS] = fmk(b.z, b.w);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) v[k] = fmk(w[2 * (sb + k)], w[2 * (sb + k) + 1]);
+    }
+}
+
 // Closure kinds of the mixed K1:
 //   CL_EXACT    the exact QCS coefficient;
 //   CL_FLAG     the exact coefficient, and the zone is reported (ClCount.flag)
@@ -626,6 +658,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         ccos[k] = (csin2 < 1.e-4f) ? 0.f : ccos[k];
     }
     d2 tot[NS];
+    f2 sfwk[NS];
     float dux_max = 0.f;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -634,10 +667,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
                            fscl(cw[kp], fadd(cqe1[kp], fscl(ccos[kp], cqe2[kp]))));
         const f2 sfq = fscl(frcp(el[k]), fq);
         const f2 sfw = fadd(sfp[k], sfq);
-        if (valid) {
-            a.fsfwx[sb + k] = sfw.x;
-            a.fsfwy[sb + k] = sfw.y;
-        }
+        sfwk[k] = sfw;
         tot[k] = mk((double)sfw.x + (double)sft[k].x, (double)sfw.y + (double)sft[k].y);
         // setVelDiff
         const f2 du = fmk(ul[kp].x - ul[k].x, ul[kp].y - ul[k].y);
@@ -645,6 +675,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         const float dux = fabsf(fdot(du, dx)) * frcp(el[k]);
         dux_max = fmaxr(dux_max, (el[k] > 0.f) ? dux : 0.f);
     }
+    if (valid) store_sides<NS>(a.fsfw, sb, sfwk);
     const double zdu = a.q1 * zss + 2. * a.q2 * (double)dux_max;
     const double mzz = zrp * zarea;
     if (valid) {
@@ -653,10 +684,10 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
             const int km = prv<NS>(k);
             const int slot = a.pcpos[sb + k];
             const d2 cf = sub(tot[k], tot[km]);
-            a.fcfx[slot] = (float)cf.x;
-            a.fcfy[slot] = (float)cf.y;
+            a.fcrn[3 * slot] = (float)cf.x;
+            a.fcrn[3 * slot + 1] = (float)cf.y;
             if constexpr (CL != CL_RECOMP)
-                a.fcmass[slot] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+                a.fcrn[3 * slot + 2] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
         }
     }
     const double cdt = smax(zdu, smax(zss, 1.e-99));
@@ -1033,11 +1064,11 @@ template <> struct Corners<double> {
     __device__ static double wy(const FusedArgs& a, int s) { return a.sfwy[s]; }
 };
 template <> struct Corners<float> {
-    __device__ static double mass(const FusedArgs& a, int k) { return (double)a.fcmass[k]; }
-    __device__ static double fx(const FusedArgs& a, int k) { return (double)a.fcfx[k]; }
-    __device__ static double fy(const FusedArgs& a, int k) { return (double)a.fcfy[k]; }
-    __device__ static double wx(const FusedArgs& a, int s) { return (double)a.fsfwx[s]; }
-    __device__ static double wy(const FusedArgs& a, int s) { return (double)a.fsfwy[s]; }
+    __device__ static double mass(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k + 2]; }
+    __device__ static double fx(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k]; }
+    __device__ static double fy(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k + 1]; }
+    __device__ static double wx(const FusedArgs& a, int s) { return (double)a.fsfw[2 * s]; }
+    __device__ static double wy(const FusedArgs& a, int s) { return (double)a.fsfw[2 * s + 1]; }
 };
 
 // Point gathers, fixed BCs, acceleration and position update.
@@ -1132,10 +1163,20 @@ __device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, dou
     }
     // calcWork
     double w = 0., et = a.zetot[z];
+    d2 sfw[NS];
+    if constexpr (sizeof(T) == 4) {
+        f2 v[NS];
+        load_sides<NS>(a.fsfw, sb, v);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sfw[k] = mk((double)v[k].x, (double)v[k].y);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sfw[k] = mk(C::wx(a, sb + k), C::wy(a, sb + k));
+    }
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int p1 = pp[k], p2 = pp[nxt<NS>(k)];
-        const d2 sftot = mk(C::wx(a, sb + k), C::wy(a, sb + k));
+        const d2 sftot = sfw[k];
         const double sd1 = dot(sftot, mk(a.vsx[p1], a.vsy[p1]));
         const double sd2 = dot(neg(sftot), mk(a.vsx[p2], a.vsy[p2]));
         const double dwork = -dth * (sd1 * a.pxpx[p1] + sd2 * a.pxpx[p2]);
@@ -1285,8 +1326,7 @@ FusedArgs make_args(pennant_problem* pb) {
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
     a.cfx = pb->cfx; a.cfy = pb->cfy; a.cmass = pb->cmass; a.sfwx = pb->sfwx; a.sfwy = pb->sfwy;
-    a.fcfx = pb->fcfx; a.fcfy = pb->fcfy; a.fcmass = pb->fcmass; a.fsfwx = pb->fsfwx;
-    a.fsfwy = pb->fsfwy; a.fsmf = pb->fsmf;
+    a.fcrn = pb->fcrn; a.fsfw = pb->fsfw; a.fsmf = pb->fsmf;
     a.vsx = pb->vsx; a.vsy = pb->vsy; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
     a.heavy = pb->heavy; a.nheavy = pb->nheavy;
     a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->npart_pred;
@@ -1490,7 +1530,7 @@ BatchArgs make_batch_args(pennant_batch* b) {
     a.zr = b->zr; a.ze = b->ze; a.zetot = b->zetot; a.zwrate = b->zwrate; a.zvol = b->zvol;
     a.zp = b->zp; a.zm = b->zm;
     a.cfx = b->cfx; a.cfy = b->cfy; a.cmass = b->cmass; a.sfwx = b->sfwx; a.sfwy = b->sfwy;
-    a.fcfx = b->fcfx; a.fcfy = b->fcfy; a.fcmass = b->fcmass; a.fsfwx = b->fsfwx; a.fsfwy = b->fsfwy;
+    a.fcrn = b->fcrn; a.fsfw = b->fsfw;
     a.fsmf = b->fsmf;
     a.vsx = b->vsx; a.vsy = b->vsy; a.pxpx = b->pxpx; a.pbc = b->pbc;
     a.part_dt = b->part_d; a.part_dv = b->part_d + b->nunits;
@@ -1560,11 +1600,11 @@ int fused_setup(pennant_problem* pb) {
     pb->heavy = (int*)al(heavy.size(), 4);
     pb->nheavy = (int)heavy.size();
     // mixed-precision intermediates and the FP32 copy of smf
-    pb->fcfx = (float*)al(S, 4); pb->fcfy = (float*)al(S, 4); pb->fcmass = (float*)al(S, 4);
-    pb->fsfwx = (float*)al(S, 4); pb->fsfwy = (float*)al(S, 4);
+    pb->fcrn = (float*)al(3 * S, 4);
+    pb->fsfw = (float*)al(2 * S, 4);
     float* fsmf = (float*)al(S, 4);
     pb->fsmf = fsmf;
-    if (!pb->fcfx || !pb->fcfy || !pb->fcmass || !pb->fsfwx || !pb->fsfwy || !fsmf)
+    if (!pb->fcrn || !pb->fsfw || !fsmf)
         return (int)hipErrorOutOfMemory;
     if (S > 0)
         hipLaunchKernelGGL(k_to_float, dim3(nblocks((int64_t)S)), dim3(PN_BLOCK), 0, pb->stream,

@@ -87,8 +87,9 @@ struct pennant_problem {
     double* cfx = nullptr; double* cfy = nullptr; double* cmass = nullptr;
     double* sfwx = nullptr; double* sfwy = nullptr;
     double* vsx = nullptr; double* vsy = nullptr; double* pxpx = nullptr;
-    float* fcfx = nullptr; float* fcfy = nullptr; float* fcmass = nullptr;   // mixed form
-    float* fsfwx = nullptr; float* fsfwy = nullptr; const float* fsmf = nullptr;
+    float* fcrn = nullptr;           // mixed form: per corner slot {fx, fy, mass}
+    float* fsfw = nullptr;           // mixed form: per side {x, y}
+    const float* fsmf = nullptr;
     unsigned char* pbc = nullptr;    // per-point BC membership bits (one per list)
     double* part_d = nullptr;        // block partials: dt minima, then dvol maxima
     int* part_i = nullptr;           // block partials: bad-side counts (pred, corr)
@@ -151,7 +152,7 @@ struct pennant_batch {
     // fused intermediates
     double *cfx = nullptr, *cfy = nullptr, *cmass = nullptr, *sfwx = nullptr, *sfwy = nullptr;
     double *vsx = nullptr, *vsy = nullptr, *pxpx = nullptr;
-    float *fcfx = nullptr, *fcfy = nullptr, *fcmass = nullptr, *fsfwx = nullptr, *fsfwy = nullptr;
+    float *fcrn = nullptr, *fsfw = nullptr;
     // work tables
     pn::BUnit* units = nullptr;
     int nunits = 0;
