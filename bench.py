@@ -52,10 +52,19 @@ def workload_deck(name: str, decks):
     raise SystemExit(f"unknown workload {name}")
 
 
-def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False):
+def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False, pipelined: bool = False):
     """Algorithmic (compulsory) HBM bytes per launch of each fused kernel:
     every distinct input element read once, every output written once
     (int32 maps, FP64 values; FP32 side and corner arrays in the mixed form)."""
+    if pipelined:
+        return {
+            # corrector and next predictor of each zone: mapsp1 4, pcpos 4,
+            # smf 4, sfw 8 in + 8 out, corner record 12 out per side; zetot
+            # zvol zm zpn 32 in, zwrate zetot ze zr zvol zp zpn 56 out per
+            # zone; px 16, pu 16, vs 16, pxp.x 8 per point
+            "zone": 40 * S + 88 * Z + 56 * P,
+            "point": 12 * S + 93 * P,
+        }
     if mixed:
         return {
             # mapsp1 4, pcpos 4, smf 4, cf 8, cmass 4, sfw 8 per side
@@ -150,8 +159,10 @@ def physics_check(pa, decks, mlp, n):
 
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
-    "mixed": {"zone_pred": "k_pred_mixed<4>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
-    "learned": {"zone_pred": "k_pred_mixed<4, true>", "point": "k_point<float>", "zone_corr": "k_corr<4, float>"},
+    "mixed": {"zone_pred": "k_pred_mixed<4, false>", "point": "k_point<float>",
+              "zone_corr": "k_corr<4, float>"},
+    "learned": {"zone_pred": "k_pred_mixed<4, true>", "point": "k_point<float>",
+                "zone_corr": "k_corr<4, float>"},
 }
 
 
@@ -195,6 +206,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--phase-steps", type=int, default=20,
+                    help="cycles after the timed ones, with events between the kernels (direct "
+                         "launches), for the per-kernel durations (0 disables)")
+    ap.add_argument("--pipeline", type=int, default=None,
+                    help="mixed forms: 1 pipelined cycle, 0 separate predictor and corrector "
+                         "passes (the default)")
     ap.add_argument("--workload", default="sedov1440")
     ap.add_argument("--mode", default="learned", choices=["learned", "mixed", "fused", "tasks"],
                     help="learned: mixed precision with the learned QCS closure; mixed: mixed "
@@ -250,12 +267,20 @@ def main():
             pb.set_tuning("max_blocks", args.max_blocks)
         if args.xcd is not None:
             pb.set_tuning("xcd", args.xcd)
+        if args.pipeline is not None:
+            pb.set_tuning("pipeline", args.pipeline)
         log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
         if args.warmup > 0:
             pb.run(args.warmup, mode)
-        total_ms, phase_ms = pb.time_cycles(args.steps, mode, phases=(mode != pa.MODE_TASKS))
+        # the timed cycles run as pennant_launch_cycles runs them (16-cycle
+        # graphs); the per-kernel durations come from further cycles with
+        # events between the kernels
+        total_ms, _ = pb.time_cycles(args.steps, mode, phases=False)
+        phase_ms = None
+        if mode != pa.MODE_TASKS and args.phase_steps > 0:
+            _, phase_ms = pb.time_cycles(args.phase_steps, mode, phases=True)
         sc = pb.scalars()
-        assert sc["cycle"] == args.warmup + args.steps, sc
+        assert sc["cycle"] == args.warmup + args.steps + (args.phase_steps if phase_ms else 0), sc
         state = {f: pb.get(f) for f in ("zr", "ze", "zp")}
         if mode == pa.MODE_LEARNED:
             state["closure_stats"] = pb.closure_stats()
@@ -302,20 +327,23 @@ def main():
     roofline = None
     if phase_ms is not None:
         mixed = (mode in (pa.MODE_MIXED, pa.MODE_LEARNED))
-        kb = kernel_bytes(S, Z, P, mixed)
-        names = ("zone_pred", "point", "zone_corr", "finalize")
-        per = {n: phase_ms[i] / args.steps for i, n in enumerate(names)}
+        pipelined = mixed and args.pipeline == 1
+        kb = kernel_bytes(S, Z, P, mixed, pipelined)
+        names = (("zone", "point", "learned_pass", "finalize") if pipelined
+                 else ("zone_pred", "point", "zone_corr", "finalize"))
+        per = {n: phase_ms[i] / args.phase_steps for i, n in enumerate(names)}
         for n in names:
             gbs = kb[n] / (per[n] * 1e-3) / 1e9 if n in kb else 0.0
-            log(f"  {n:10s} {per[n] * 1e3:8.1f} us/launch" +
+            log(f"  {n:12s} {per[n] * 1e3:8.1f} us/launch" +
                 (f"  {kb[n] / 1e6:8.1f} MB  {gbs:8.1f} GB/s" if n in kb else ""))
-        dom = max(("zone_pred", "point", "zone_corr"), key=lambda n: per[n])
+        dom = max(kb, key=lambda n: per[n])
         achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
         # the reference's task decomposition moves 688 S + 536 Z + 512 P bytes
         # per cycle; at the fused cycle's rate that is this effective bandwidth
         b_tasks = 688 * S + 536 * Z + 512 * P
         pform = {pa.MODE_LEARNED: "learned", pa.MODE_MIXED: "mixed"}.get(mode, "fp64")
-        traffic = profiled_traffic(args.workload, dom, pform) if args.k1 in (None, 0) else None
+        traffic = (profiled_traffic(args.workload, dom, pform)
+                   if args.k1 in (None, 0) and not pipelined else None)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,

@@ -280,8 +280,23 @@ pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen)
 void pennant_destroy(pennant_problem* pb);
 
 /* Runs up to ncycles cycles (stopping at cstop or tstop) with the given mode,
- * asynchronously on the problem's stream.  Returns 0 or a hipError_t. */
+ * asynchronously on the problem's stream.  Returns 0 or a hipError_t.
+ *
+ * With the tuning key "pipeline" set, the mixed modes run pipelined: one
+ * zone pass holds a cycle's corrector and the next cycle's predictor, which
+ * assumes the next dt computed without the volume-change limit; the
+ * finalize checks that dt bit for bit and redoes the predictor when the
+ * limit binds (every cycle while a shock crosses zones faster than cflv
+ * allows, as in Sedov; seldom in smooth flow).  Results are bit-identical to
+ * the unpipelined cycle.  The first pipelined launch after a change of state
+ * (pennant_set_field, pennant_set_scalars, pennant_field_device, a closure,
+ * capture or tuning change, cycles of another mode) computes the current
+ * cycle's predictor first. */
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode);
+
+/* Number of look-ahead predictors the pipelined mixed cycle has redone
+ * because the volume-change limit set the next dt.  Synchronous. */
+int pennant_lookahead_redos(pennant_problem* pb, int64_t* count);
 
 /* Waits for the stream; returns 0, PENNANT_EBADSIDES (message in
  * pennant_error) or a hipError_t. */
@@ -311,7 +326,8 @@ int pennant_set_scalars(pennant_problem* pb, const double* in10);
 int pennant_field_info(pennant_problem* pb, const char* name, int64_t* count, int* ncomp);
 int pennant_get_field(pennant_problem* pb, const char* name, double* host_out);
 int pennant_set_field(pennant_problem* pb, const char* name, const double* host_in);
-/* device pointer of one component (0 = x or scalar, 1 = y) */
+/* device pointer of one component (0 = x or scalar, 1 = y); the caller may
+ * write through it, so the pipelined cycle's look-ahead is recomputed */
 double* pennant_field_device(pennant_problem* pb, const char* name, int comp);
 
 /* Maps as int32 host copies: "mapsp1", "mapsp2", "mapsz", "mapss3", "mapss4",
@@ -342,8 +358,11 @@ int pennant_end_cycle(pennant_problem* pb);
  * stream, then synchronises.  *total_ms receives the elapsed time.  In fused
  * mode, phase_ms (4 entries, may be NULL) receives the summed durations of
  * the zone-predictor, point, zone-corrector and finalize kernels, measured
- * by events recorded between their launches in every cycle.  Returns 0,
- * PENNANT_EBADSIDES or a hipError_t. */
+ * by events recorded between their launches in every cycle (direct launches
+ * then, not the captured graph).  For the pipelined mixed cycle the four
+ * entries are the zone pass (corrector and next predictor), the point
+ * kernel, the learned pass, and the finalize with the predictor redo.
+ * Returns 0, PENNANT_EBADSIDES or a hipError_t. */
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms);
 
@@ -361,6 +380,9 @@ int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
  *                0: plain grid-stride order
  *   "graph"      1: pennant_launch_cycles replays runs of 16 fused cycles as one
  *                captured HIP graph (default), 0: direct launches
+ *   "pipeline"   1: the mixed modes run the pipelined cycle, 0: the zone
+ *                predictor and corrector as separate passes (default);
+ *                pennant_autotune tries both for the mixed modes
  * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
 int pennant_get_tuning(pennant_problem* pb, const char* key);

@@ -462,14 +462,49 @@ static void drop_graph(pennant_problem* pb) {
     pb->graph_form = -1;
 }
 
+// The pipelined mixed cycle's look-ahead predictor results no longer match
+// the state (the state, the closure, the capture or the launch shapes
+// changed, or cycles of another kind ran).
+static void drop_lookahead(pennant_problem* pb) { pb->la_form = -1; }
+
+// Whether cycles of `form` run pipelined: the mixed forms, unless switched
+// off or while training tuples are captured.
+static bool pipelined(const pennant_problem* pb, int form) {
+    return form != FORM_FP64 && pb->tune.pipeline && !pb->cap_buf;
+}
+
+// One cycle of `form` (ev as for fused_cycle / fused_cycle_pipe).  A
+// pipelined cycle first computes the look-ahead predictor when it is not
+// current; any other cycle makes it stale.
+static int one_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
+    if (!pipelined(pb, form)) {
+        drop_lookahead(pb);
+        return fused_cycle(pb, ev, form);
+    }
+    return fused_cycle_pipe(pb, ev, form);
+}
+
+static int ensure_lookahead(pennant_problem* pb, int form) {
+    if (!pipelined(pb, form)) {
+        drop_lookahead(pb);
+        return 0;
+    }
+    if (pb->la_form == form) return 0;
+    CK(fused_prologue(pb, form));
+    pb->la_form = form;
+    return 0;
+}
+
 // Captures PN_GRAPH_CYCLES fused cycles of `form` as one HIP graph (kernel
 // nodes only: the fused cycle issues no memset and no host work).
 static int build_graph(pennant_problem* pb, int form) {
     drop_graph(pb);
     hipGraph_t g = nullptr;
+    const bool pipe = pipelined(pb, form);
     CK(hipStreamBeginCapture(pb->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int k = 0; k < PN_GRAPH_CYCLES && !rc; ++k) rc = fused_cycle(pb, nullptr, form);
+    for (int k = 0; k < PN_GRAPH_CYCLES && !rc; ++k)
+        rc = pipe ? fused_cycle_pipe(pb, nullptr, form) : fused_cycle(pb, nullptr, form);
     const int rc2 = (int)hipStreamEndCapture(pb->stream, &g);
     if (rc || rc2) {
         if (g) (void)hipGraphDestroy(g);
@@ -478,7 +513,7 @@ static int build_graph(pennant_problem* pb, int form) {
     rc = (int)hipGraphInstantiate(&pb->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (rc) { pb->gexec = nullptr; return rc; }
-    pb->graph_form = form;
+    pb->graph_form = form + (pipe ? 16 : 0);
     return 0;
 }
 
@@ -498,13 +533,17 @@ int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
         if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
         int left = ncycles;
         if (pb->tune.graph && left >= PN_GRAPH_CYCLES) {
-            if (pb->graph_form != form || !pb->gexec) CK(build_graph(pb, form));
+            const int key = form + (pipelined(pb, form) ? 16 : 0);
+            if (pb->graph_form != key || !pb->gexec) CK(build_graph(pb, form));
+            CK(ensure_lookahead(pb, form));
             for (; left >= PN_GRAPH_CYCLES; left -= PN_GRAPH_CYCLES) CK(hipGraphLaunch(pb->gexec, pb->stream));
         }
-        for (; left > 0; --left) CK(fused_cycle(pb, nullptr, form));
+        if (left > 0) CK(ensure_lookahead(pb, form));
+        for (; left > 0; --left) CK(one_cycle(pb, nullptr, form));
         return 0;
     }
     if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
+    drop_lookahead(pb);
     // Task mode mirrors Driver::run: the host reads the control block each
     // cycle (the reference reads its dt future through host tasks).
     for (int k = 0; k < ncycles; ++k) {
@@ -525,10 +564,12 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     const size_t E = 5;
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
     for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
-    int rc = (int)hipEventRecord(ev[0], pb->stream);
+    int rc = phases ? ensure_lookahead(pb, form) : 0;
+    const bool pipe = phases && pipelined(pb, form);
+    if (!rc) rc = (int)hipEventRecord(ev[0], pb->stream);
     if (!rc) {
         if (phases) {
-            for (int k = 0; k < ncycles && !rc; ++k) rc = fused_cycle(pb, &ev[2 + E * (size_t)k], form);
+            for (int k = 0; k < ncycles && !rc; ++k) rc = one_cycle(pb, &ev[2 + E * (size_t)k], form);
         } else {
             rc = pennant_launch_cycles(pb, ncycles, mode);
         }
@@ -539,11 +580,16 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     if (!rc) rc = (int)hipEventElapsedTime(&ms, ev[0], ev[1]);
     if (!rc) *total_ms = ms;
     if (!rc && phases) {
+        // pipelined: events before K2, the zone pass, K4 and the redo, the
+        // learned pass, and after it; reported as zone pass, K2, learned
+        // pass, K4 and redo
+        static const int kOrder[4] = {0, 1, 2, 3}, kPipe[4] = {1, 0, 3, 2};
+        const int* slot = pipe ? kPipe : kOrder;
         for (int j = 0; j < 4; ++j) phase_ms[j] = 0.;
         for (int k = 0; k < ncycles && !rc; ++k)
             for (int j = 0; j < 4 && !rc; ++j) {
                 rc = (int)hipEventElapsedTime(&ms, ev[2 + E * (size_t)k + j], ev[2 + E * (size_t)k + j + 1]);
-                phase_ms[j] += ms;
+                phase_ms[slot[j]] += ms;
             }
     }
     for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
@@ -553,9 +599,15 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
 
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
     drop_graph(pb);
+    drop_lookahead(pb);
     if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return PENNANT_EINVAL;
         pb->tune.graph = value;
+        return 0;
+    }
+    if (!strcmp(key, "pipeline")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        pb->tune.pipeline = value;
         return 0;
     }
     if (!strcmp(key, "k1_layout")) {
@@ -581,6 +633,7 @@ int pennant_get_tuning(pennant_problem* pb, const char* key) {
     if (!strcmp(key, "max_blocks")) return pb->tune.max_blocks;
     if (!strcmp(key, "xcd")) return pb->tune.xcd;
     if (!strcmp(key, "graph")) return pb->tune.graph;
+    if (!strcmp(key, "pipeline")) return pb->tune.pipeline;
     return PENNANT_EINVAL;
 }
 
@@ -611,6 +664,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
         if (!rc) rc = (int)hipMemcpyAsync(c, a.first, a.second, hipMemcpyDeviceToDevice, pb->stream);
     }
     auto restore = [&]() {
+        drop_lookahead(pb);
         int r = 0;
         for (size_t i = 0; i < copy.size() && !r; ++i)
             r = (int)hipMemcpyAsync(st[i].first, copy[i], st[i].second, hipMemcpyDeviceToDevice, pb->stream);
@@ -622,11 +676,16 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     double bms = 1e300;
     drop_graph(pb);
     const int grids[] = {256, 512, 1024, 2048, 4096};
+    // the mixed forms also try the pipelined cycle (it pays where the
+    // volume-change limit seldom sets dt)
+    const int npipe = (form == FORM_FP64 || pb->cap_buf) ? 1 : 2;
     for (int layout = 0; layout <= (form == FORM_FP64 ? 1 : 0) && !rc; ++layout)
+      for (int pipe = 0; pipe < npipe && !rc; ++pipe)
         for (int xcd = 0; xcd <= 1 && !rc; ++xcd)
             for (int g : grids) {
                 if (rc) break;
                 pb->tune.k1_layout = layout;
+                pb->tune.pipeline = (npipe == 2) ? pipe : keep.pipeline;
                 pb->tune.xcd = xcd;
                 pb->tune.max_blocks = g;
                 pb->tune.graph = 0;               // direct launches while shapes change
@@ -660,6 +719,12 @@ int pennant_run(pennant_problem* pb, int ncycles, int mode) {
     return pb->ctl_host->cycle - c0;
 }
 
+int pennant_lookahead_redos(pennant_problem* pb, int64_t* count) {
+    CK(copy_ctl_to_host(pb));
+    *count = pb->ctl_host->la_redos;
+    return 0;
+}
+
 int pennant_scalars(pennant_problem* pb, double* o) {
     CK(copy_ctl_to_host(pb));
     const Ctl& c = *pb->ctl_host;
@@ -669,6 +734,7 @@ int pennant_scalars(pennant_problem* pb, double* o) {
 }
 
 int pennant_set_scalars(pennant_problem* pb, const double* in) {
+    drop_lookahead(pb);
     CK(copy_ctl_to_host(pb));
     Ctl& c = *pb->ctl_host;
     c.time = in[0]; c.dtlast = in[1]; c.dtrec = in[2]; c.cycle = (int)in[3]; c.dtnew = in[4];
@@ -701,6 +767,7 @@ int pennant_get_field(pennant_problem* pb, const char* name, double* out) {
 }
 
 int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
+    drop_lookahead(pb);
     auto it = pb->fields.find(name);
     if (it == pb->fields.end()) return PENNANT_EINVAL;
     const DevField& d = it->second;
@@ -715,6 +782,8 @@ int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
 }
 
 double* pennant_field_device(pennant_problem* pb, const char* name, int comp) {
+    // the caller may write through the pointer
+    drop_lookahead(pb);
     auto it = pb->fields.find(name);
     if (it == pb->fields.end() || comp < 0 || comp >= it->second.ncomp) return nullptr;
     return it->second.c[comp];
@@ -791,6 +860,7 @@ int pennant_begin_cycle(pennant_problem* pb) {
 }
 
 int pennant_step(pennant_problem* pb, int step) {
+    drop_lookahead(pb);
     CK(task_step(pb, step));
     CK(copy_ctl_to_host(pb));
     if (step == 18 && pb->ctl_host->numsbad_pred > 0) return PENNANT_EBADSIDES;
@@ -799,6 +869,7 @@ int pennant_step(pennant_problem* pb, int step) {
 }
 
 int pennant_end_cycle(pennant_problem* pb) {
+    drop_lookahead(pb);
     CK(launch_end_cycle(pb, 0));
     return pennant_sync(pb);
 }
@@ -897,6 +968,7 @@ const void* pennant_mlp_device_pack(pennant_mlp* m);
 int pennant_set_closure(pennant_problem* pb, pennant_mlp* m) {
     pb->closure = m ? pennant_mlp_device_pack(m) : nullptr;
     drop_graph(pb);
+    drop_lookahead(pb);
     return 0;
 }
 
@@ -915,6 +987,7 @@ int pennant_set_capture(pennant_problem* pb, float* dev_buf, int64_t cap) {
     pb->cap_buf = dev_buf;
     pb->cap = cap;
     drop_graph(pb);
+    drop_lookahead(pb);
     const unsigned long long zero = 0;
     return (int)hipMemcpy(pb->cl_counters, &zero, sizeof zero, hipMemcpyHostToDevice);
 }

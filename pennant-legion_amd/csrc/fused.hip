@@ -66,6 +66,7 @@ struct FusedArgs {
     double* pxx; double* pxy; double* pux; double* puy;
     double* zr; double* ze; double* zetot; double* zwrate; double* zvol; double* zp;
     const double* zm;
+    double* zpn;               // pipelined mixed cycle: the look-ahead predictor's zp
     // per-side intermediates
     double* cfx; double* cfy; double* cmass; double* sfwx; double* sfwy;
     // the mixed-precision form's FP32 counterparts, and smf in FP32
@@ -367,35 +368,30 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
-// The work forces of a zone's sides, {x, y} per side.  Quad zones whose
-// first side is even start on a 16-byte boundary: two 16-byte accesses.
+// The points of a zone: mapsp1 of its sides sb..sb+NS-1.
+template <int NS>
+__device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pp[k] = a.mapsp1[sb + k];
+}
+
+// The work forces of a zone's sides, {x, y} per side, as 2*NS consecutive
+// floats: the compiler merges them into multi-dword accesses (gfx950 global
+// memory takes dword-aligned multi-dword loads and stores).
 template <int NS>
 __device__ __forceinline__ void store_sides(float* w, int sb, const f2 (&v)[NS]) {
-    if (NS == 4 && (sb & 1) == 0) {
-        float4* q = reinterpret_cast<float4*>(w + 2 * sb);
-        q[0] = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
-        q[1] = make_float4(v[2].x, v[2].y, v[3 % NS].x, v[3 % NS].y);
-    } else {
+    float* q = w + 2 * sb;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            w[2 * (sb + k)] = v[k].x;
-            w[2 * (sb + k) + 1] = v[k].y;
-        }
+    for (int k = 0; k < NS; ++k) {
+        q[2 * k] = v[k].x;
+        q[2 * k + 1] = v[k].y;
     }
 }
 template <int NS>
 __device__ __forceinline__ void load_sides(const float* w, int sb, f2 (&v)[NS]) {
-    if (NS == 4 && (sb & 1) == 0) {
-        const float4* q = reinterpret_cast<const float4*>(w + 2 * sb);
-        const float4 a = q[0], b = q[1];
-        v[0] = fmk(a.x, a.y);
-        v[1 % NS] = fmk(a.z, a.w);
-        v[2 % NS] = fmk(b.x, b.y);
-        v[3 % NS] = fmk(b.z, b.w);
-    } else {
+    const float* q = w + 2 * sb;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) v[k] = fmk(w[2 * (sb + k)], w[2 * (sb + k) + 1]);
-    }
+    for (int k = 0; k < NS; ++k) v[k] = fmk(q[2 * k], q[2 * k + 1]);
 }
 
 // Closure kinds of the mixed K1:
@@ -422,25 +418,38 @@ struct ClCount {
     bool flag = false;  // CL_FLAG: this zone has a gated corner
 };
 
-// The mixed-precision predictor half of zone z (sides sb..sb+NS-1).  With
-// CL != CL_EXACT it must be called by all 64 lanes of a wave together (the
-// network runs across the wave); `valid` false marks a lane that only takes
-// part in the wave-wide steps and stores nothing.
+// The zone state a predictor starts from (the previous cycle's results).
+struct ZoneState {
+    double zm, zr, ze, zvol, zwrate;
+};
+
+// The mixed-precision predictor half of zone z (sides sb..sb+NS-1), from its
+// points' positions x0 and velocities u0 and its state st.  With CL !=
+// CL_EXACT it must be called by all 64 lanes of a wave together (the network
+// runs across the wave); `valid` false marks a lane that only takes part in
+// the wave-wide steps and stores nothing.  Writes zp to a.zp.
 template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
-                                                  int& bad, bool valid, ClCount& cnt) {
+__device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z, int sb,
+                                                       const d2 (&x0)[NS], const d2 (&u0)[NS],
+                                                       const ZoneState& st, double dt, int& bad,
+                                                       bool valid, ClCount& cnt) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
-    d2 u0[NS], xp[NS];
+    d2 xp[NS];
     float mf[NS];
+    int slot[NS];
+    const double zm = st.zm;
+    const double r = st.zr;
+    const double ze0 = st.ze;
+    const double zvol0 = st.zvol;
+    const double zwr = st.zwrate;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const int p = a.mapsp1[sb + k];
-        const d2 x0 = mk(a.pxx[p], a.pxy[p]);
-        u0[k] = mk(a.pux[p], a.puy[p]);
-        xp[k] = add(x0, scl(dth, u0[k]));
         mf[k] = a.fsmf[sb + k];
+        slot[k] = a.pcpos[sb + k];
     }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) xp[k] = add(x0[k], scl(dth, u0[k]));
     // zone-mean velocity; velocities from here on live in FP32 relative to it
     d2 zuc = mk(0., 0.);
 #pragma unroll
@@ -478,22 +487,20 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 #pragma unroll
     for (int k = 0; k < NS; ++k) zdl = fminr(zdl, fac * (float)sa[k] * frcp(el[k]));
     // zone EOS in FP64
-    const double zm = a.zm[z];
     const double zrp = zm / zvolp;
     const double gm1 = a.gamma - 1.;
     const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
-    const double r = a.zr[z];
-    const double e = smax(a.ze[z], 0.);
+    const double e = smax(ze0, 0.);
     const double pz = gm1 * r * e;
     const double per = gm1 * r;
     // pre + per pz / r^2 = (gamma - 1) e + (gamma - 1)^2 e = gamma (gamma - 1) e
     const double csqd = smax(ssmin2, a.gamma * gm1 * e);
     const double zss = __builtin_sqrt(csqd);
     const double minv = 1. / zm;
-    const double dv = (zvolp - a.zvol[z]) * minv;
+    const double dv = (zvolp - zvol0) * minv;
     const double bulk = r * csqd;
     const double denom = 1. + 0.5 * per * dv;
-    const double src = a.zwrate[z] * dth * minv;
+    const double src = zwr * dth * minv;
     const double zp = pz + (per * src - r * bulk * dv) / denom;
     if (valid && CL != CL_RECOMP) a.zp[z] = zp;
     // pressure and TTS side forces
@@ -682,35 +689,76 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const int km = prv<NS>(k);
-            const int slot = a.pcpos[sb + k];
             const d2 cf = sub(tot[k], tot[km]);
-            a.fcrn[3 * slot] = (float)cf.x;
-            a.fcrn[3 * slot + 1] = (float)cf.y;
+            a.fcrn[3 * slot[k]] = (float)cf.x;
+            a.fcrn[3 * slot[k] + 1] = (float)cf.y;
             if constexpr (CL != CL_RECOMP)
-                a.fcrn[3 * slot + 2] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
+                a.fcrn[3 * slot[k] + 2] = (float)(mzz * 0.5 * ((double)mf[k] + (double)mf[km]));
         }
     }
     const double cdt = smax(zdu, smax(zss, 1.e-99));
     return (double)zdl * a.cfl / cdt;
 }
 
+// The predictor of zone z whose points are pp (mapsp1 of its sides); every
+// load of the zone is issued before the first use of a loaded value.
+template <int NS, int CL>
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb,
+                                                  const int (&pp)[NS], double dt, int& bad,
+                                                  bool valid, ClCount& cnt) {
+    d2 x0[NS], u0[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        x0[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
+        u0[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
+    }
+    ZoneState st;
+    st.zm = a.zm[z];
+    st.zr = a.zr[z];
+    st.ze = a.ze[z];
+    st.zvol = a.zvol[z];
+    st.zwrate = a.zwrate[z];
+    return zone_pred_mixed_core<NS, CL>(a, z, sb, x0, u0, st, dt, bad, valid, cnt);
+}
+
+template <int NS, int CL>
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
+                                                  int& bad, bool valid, ClCount& cnt) {
+    int pp[NS];
+    load_pp<NS>(a, sb, pp);
+    return zone_pred_mixed<NS, CL>(a, z, sb, pp, dt, bad, valid, cnt);
+}
+
 // K1, mixed precision.  With FLAG, zones with a gated corner are appended to
 // the run's zone list (a.zlist[zbeg ..], count a.zcount[run]) for the
 // learned pass; their order in the list is not fixed, their results are.
-template <int NS, bool FLAG>
+// With REDO it runs only when the pipelined cycle's finalize asked for the
+// predictor to be redone (Ctl::redo).
+template <int NS, bool FLAG, bool REDO = false>
 __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
     if (a.ctl->done) return;
+    if (REDO && !a.ctl->redo) return;
     const double dt = a.ctl->dt;
     double dtmin = 1.e99;
     int bad = 0;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
-    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
-        const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
+    Units w = block_units(nunits, a.xcd);
+    int zn = zbeg + w.u * PN_BLOCK + threadIdx.x;
+    int pn[NS];
+    if (w.u < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
+    for (; w.u < w.end; w.u += w.step) {
+        // this zone's points were loaded by the previous iteration; load the next zone's
+        const int z = zn;
+        int pp[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pp[k] = pn[k];
+        zn = zbeg + (w.u + w.step) * PN_BLOCK + threadIdx.x;
+        if (w.u + w.step < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
         ClCount cnt;
         if (z < zend) {
             const double zdthyd = zone_pred_mixed<NS, FLAG ? CL_FLAG : CL_EXACT>(
-                a, z, sbeg + (z - zbeg) * NS, dt, bad, true, cnt);
+                a, z, sbeg + (z - zbeg) * NS, pp, dt, bad, true, cnt);
             dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
         }
         if constexpr (FLAG) {
@@ -1120,8 +1168,11 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
 }
 
 // K2: blocks [0, nheavy) take the heavy points, the rest one point per thread.
+// Block 0 also clears the learned pass's zone counts: that pass has run
+// before K2, and the next predictor appends to them.
 template <typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+    if (a.zcount && blockIdx.x == 0 && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     if ((int)blockIdx.x < a.nheavy) {
@@ -1134,20 +1185,40 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
 
 // ---------------------------------------------------------------------------
 // K3: corrector geometry, work, work rate, energy, density, dvol of zone z
-// (sides sb..sb+NS-1); folds the bad-side count and the dvol ratio.
-template <int NS, typename T>
-__device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, double dt, int& bad,
-                                          double& dvov) {
+// (sides sb..sb+NS-1, points pp); folds the bad-side count and the dvol
+// ratio.  The zone's own loads are issued before the point gathers, which
+// wait for the point indices.  Returns the points' positions x and the zone's
+// new state st (the next cycle's predictor starts from them).  With PIPE the
+// cycle's predictor pressure comes from a.zpn and is stored to a.zp.
+template <int NS, typename T, bool PIPE>
+__device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb,
+                                               const int (&pp)[NS], double dt, int& bad,
+                                               double& dvov, d2 (&x)[NS], ZoneState& st) {
     using C = Corners<T>;
-    const double dth = 0.5 * dt;
-    const double n = (double)NS;
-    int pp[NS];
-    d2 x[NS];
+    double et = a.zetot[z];
+    const double zvol0 = a.zvol[z];
+    const double zm = a.zm[z];
+    const double zp = PIPE ? a.zpn[z] : a.zp[z];
+    d2 sfw[NS];
+    if constexpr (sizeof(T) == 4) {
+        f2 v[NS];
+        load_sides<NS>(a.fsfw, sb, v);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sfw[k] = mk((double)v[k].x, (double)v[k].y);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sfw[k] = mk(C::wx(a, sb + k), C::wy(a, sb + k));
+    }
+    d2 vs[NS];
+    double xp[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        pp[k] = a.mapsp1[sb + k];
         x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
+        vs[k] = mk(a.vsx[pp[k]], a.vsy[pp[k]]);
+        xp[k] = a.pxpx[pp[k]];
     }
+    const double dth = 0.5 * dt;
+    const double n = (double)NS;
     d2 zx = mk(0., 0.);
 #pragma unroll
     for (int k = 0; k < NS; ++k) zx = add(zx, dvs(x[k], n));
@@ -1162,42 +1233,57 @@ __device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, dou
         if (sv <= 0.) bad += 1;
     }
     // calcWork
-    double w = 0., et = a.zetot[z];
-    d2 sfw[NS];
-    if constexpr (sizeof(T) == 4) {
-        f2 v[NS];
-        load_sides<NS>(a.fsfw, sb, v);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) sfw[k] = mk((double)v[k].x, (double)v[k].y);
-    } else {
-#pragma unroll
-        for (int k = 0; k < NS; ++k) sfw[k] = mk(C::wx(a, sb + k), C::wy(a, sb + k));
-    }
+    double w = 0.;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const int p1 = pp[k], p2 = pp[nxt<NS>(k)];
+        const int k2 = nxt<NS>(k);
         const d2 sftot = sfw[k];
-        const double sd1 = dot(sftot, mk(a.vsx[p1], a.vsy[p1]));
-        const double sd2 = dot(neg(sftot), mk(a.vsx[p2], a.vsy[p2]));
-        const double dwork = -dth * (sd1 * a.pxpx[p1] + sd2 * a.pxpx[p2]);
+        const double sd1 = dot(sftot, vs[k]);
+        const double sd2 = dot(neg(sftot), vs[k2]);
+        const double dwork = -dth * (sd1 * xp[k] + sd2 * xp[k2]);
         et += dwork;
         w += dwork;
     }
-    const double zvol0 = a.zvol[z];
-    const double zm = a.zm[z];
     // calcWorkRate, calcEnergy, calcRho (corr), calcDvol
     const double dtinv = 1. / dt;
     const double dvol = zvol - zvol0;
-    a.zwrate[z] = (w + a.zp[z] * dvol) * dtinv;
+    const double zwr = (w + zp * dvol) * dtinv;
+    a.zwrate[z] = zwr;
     a.zetot[z] = et;
     const double fuzz = 1.e-99;
-    a.ze[z] = et / (zm + fuzz);
-    a.zr[z] = zm / zvol;
+    const double ze = et / (zm + fuzz);
+    a.ze[z] = ze;
+    const double zr = zm / zvol;
+    a.zr[z] = zr;
     a.zvol[z] = zvol;
+    if (PIPE) a.zp[z] = zp;
     const double zdvov = fabs((zvol - zvol0) / zvol0);
     dvov = (zdvov > dvov) ? zdvov : dvov;
+    st.zm = zm;
+    st.zr = zr;
+    st.ze = ze;
+    st.zvol = zvol;
+    st.zwrate = zwr;
 }
 
+template <int NS, typename T>
+__device__ __forceinline__ void corr_zone_pp(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
+                                             double dt, int& bad, double& dvov) {
+    d2 x[NS];
+    ZoneState st;
+    corr_zone_core<NS, T, false>(a, z, sb, pp, dt, bad, dvov, x, st);
+}
+
+template <int NS, typename T>
+__device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, double dt, int& bad,
+                                          double& dvov) {
+    int pp[NS];
+    load_pp<NS>(a, sb, pp);
+    corr_zone_pp<NS, T>(a, z, sb, pp, dt, bad, dvov);
+}
+
+// The point indices of a thread's next zone are loaded while the current
+// zone is processed.
 template <int NS, typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int zend, int sbeg,
                                                    int poff) {
@@ -1206,16 +1292,149 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
     double dvov = 1.e-99;
     int bad = 0;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
-    for (Units wu = block_units(nunits, a.xcd); wu.u < wu.end; wu.u += wu.step) {
-        const int z = zbeg + wu.u * PN_BLOCK + threadIdx.x;
-        if (z >= zend) continue;
-        corr_zone<NS, T>(a, z, sbeg + (z - zbeg) * NS, dt, bad, dvov);
+    Units wu = block_units(nunits, a.xcd);
+    int z = zbeg + wu.u * PN_BLOCK + threadIdx.x;
+    int pp[NS];
+    if (wu.u < wu.end && z < zend) load_pp<NS>(a, sbeg + (z - zbeg) * NS, pp);
+    for (; wu.u < wu.end; wu.u += wu.step) {
+        const int zc = z;
+        int pc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pc[k] = pp[k];
+        z = zbeg + (wu.u + wu.step) * PN_BLOCK + threadIdx.x;
+        if (wu.u + wu.step < wu.end && z < zend) load_pp<NS>(a, sbeg + (z - zbeg) * NS, pp);
+        if (zc < zend) corr_zone_pp<NS, T>(a, zc, sbeg + (zc - zbeg) * NS, pc, dt, bad, dvov);
     }
     const double m = block_max(dvov);
     const int b = block_sum(bad);
     if (threadIdx.x == 0) {
         a.part_dv[poff + blockIdx.x] = m;
         a.part_bc[poff + blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined mixed cycle.  The predictor of cycle n+1 needs dt(n+1), which the
+// finalize of cycle n computes from this cycle's predictor dt limit (known
+// before K2) and its corrector's volume-change limit (known after K3).  The
+// zone pass of cycle n runs each zone's corrector and then, from the state it
+// has just computed, the zone's predictor of cycle n+1 with the dt that the
+// finalize gives when the volume-change limit does not bind.  The finalize
+// checks that dt bit for bit; when it differs, the predictor is redone with
+// the actual dt.  Either way cycle n+1 starts from the same bits as the
+// unpipelined cycle.
+
+// The control-block values the end of a cycle and the look-ahead dt start from.
+struct CtlSnap {
+    double time, dt, la_dtnew;
+    int cycle, la_bp;
+};
+// Read through vector loads (volatile): the caller stores to the block later.
+__device__ __forceinline__ CtlSnap snap_ctl(const Ctl* c) {
+    const volatile Ctl* w = c;
+    CtlSnap v;
+    v.time = w->time;
+    v.dt = w->dt;
+    v.la_dtnew = w->la_dtnew;
+    v.cycle = w->cycle;
+    v.la_bp = w->la_bp;
+    return v;
+}
+
+struct DtParams {
+    double cflv, dtmax, dtfac, tstop;
+    int cstop;
+};
+
+// dt of the cycle after the current one as the finalize computes it
+// (src/Hydro.cc:1647-1667, src/Driver.cc:213-245) when the volume-change
+// limit does not bind; `last` when the current cycle ends the run.
+__device__ __forceinline__ double lookahead_dt(const CtlSnap& c, const DtParams& q, bool& last) {
+    const double t1 = c.time + c.dt;
+    last = (c.cycle + 1 >= q.cstop || t1 >= q.tstop);
+    double r = 1.e99;
+    if (c.la_dtnew < r) r = c.la_dtnew;
+    double dt = q.dtmax;
+    const double dtrecover = q.dtfac * c.dt;
+    if (dtrecover < dt) dt = dtrecover;
+    if ((q.tstop - t1) < dt) dt = q.tstop - t1;
+    if (r < dt) dt = r;
+    return dt;
+}
+
+// Zone pass of the pipelined mixed cycle: the corrector of every zone, then
+// (unless the cycle ends the run) its predictor of the next cycle, which
+// writes zp to a.zpn and, with FLAG, lists zones for the learned pass.
+// Partials: corrector at part_dv/part_bc, predictor at part_dt/part_bp.
+template <int NS, bool FLAG>
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+k_zone_pipe(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run, DtParams q) {
+    const Ctl* c = a.ctl;
+    if (c->done) return;
+    const double dt = c->dt;
+    CtlSnap v;                 // this kernel does not write the control block
+    v.time = c->time;
+    v.dt = dt;
+    v.la_dtnew = c->la_dtnew;
+    v.cycle = c->cycle;
+    v.la_bp = 0;
+    bool last;
+    const double dtn = lookahead_dt(v, q, last);
+    FusedArgs b = a;
+    b.zp = a.zpn;
+    double dvov = 1.e-99, dtmin = 1.e99;
+    int badc = 0, badp = 0;
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    Units w = block_units(nunits, a.xcd);
+    int zn = zbeg + w.u * PN_BLOCK + threadIdx.x;
+    int pn[NS];
+    if (w.u < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
+    for (; w.u < w.end; w.u += w.step) {
+        const int z = zn;
+        int pp[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pp[k] = pn[k];
+        zn = zbeg + (w.u + w.step) * PN_BLOCK + threadIdx.x;
+        if (w.u + w.step < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
+        ClCount cnt;
+        if (z < zend) {
+            const int sb = sbeg + (z - zbeg) * NS;
+            d2 u[NS];
+            if (!last) {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) u[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
+            }
+            d2 x[NS];
+            ZoneState st;
+            corr_zone_core<NS, float, true>(a, z, sb, pp, dt, badc, dvov, x, st);
+            if (!last) {
+                const double zdthyd = zone_pred_mixed_core<NS, FLAG ? CL_FLAG : CL_EXACT>(
+                    b, z, sb, x, u, st, dtn, badp, true, cnt);
+                dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+            }
+        }
+        if constexpr (FLAG) {
+            const uint64_t m = __ballot(cnt.flag);
+            if (m) {
+                unsigned base = 0;
+                if ((threadIdx.x & 63) == 0) base = atomicAdd(a.zcount + run, (unsigned)__popcll(m));
+                base = __shfl(base, 0, 64);
+                if (cnt.flag) {
+                    const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
+                    a.zlist[zbeg + base + (unsigned)__popcll(below)] = z;
+                }
+            }
+        }
+    }
+    const double mv = block_max(dvov);
+    const int bc = block_sum(badc);
+    const double mt = block_min(dtmin);
+    const int bp = block_sum(badp);
+    if (threadIdx.x == 0) {
+        a.part_dv[poff + blockIdx.x] = mv;
+        a.part_bc[poff + blockIdx.x] = bc;
+        a.part_dt[poff + blockIdx.x] = mt;
+        a.part_bp[poff + blockIdx.x] = bp;
     }
 }
 
@@ -1258,6 +1477,10 @@ __device__ __forceinline__ int blk_sum_fin(int v) {
 
 // The end of the cycle for one problem, by one block of PN_FIN_BLOCK threads,
 // from npred predictor and ncorr corrector partials.
+__device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
+                                         int cstop, double dtmax, double dtfac, double tstop,
+                                         double* next_dt = nullptr);
+
 __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, const int* part_bp,
                                                int npred, const double* part_dv, const int* part_bc,
                                                int ncorr, double cflv, int cstop, double dtmax,
@@ -1280,33 +1503,51 @@ __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, co
     bp = blk_sum_fin(bp);
     bc = blk_sum_fin(bc);
     if (threadIdx.x != 0) return;
+    end_cycle(c, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop);
+}
+
+// The end of a cycle from its predictor's dt limit and bad-side count (mn,
+// bp) and its corrector's dvol ratio and bad-side count (mx, bc).  The
+// control block is read once through vector loads before any store to it (a
+// scalar-cache load is not ordered with a later vector store to the same
+// address).
+// Returns the new done flag; *next_dt (when given) receives the next dt.
+__device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
+                                         int cstop, double dtmax, double dtfac, double tstop,
+                                         double* next_dt) {
+    const CtlSnap v = snap_ctl(c);
+    double r = 1.e99;
+    if (mn < r) r = mn;
+    const double d2v = v.dt * cflv / mx;
+    if (d2v < r) r = d2v;
     c->dtnew = mn;
     c->dvovmax = mx;
     c->numsbad_pred = bp;
     c->numsbad_corr = bc;
-    double r = 1.e99;
-    if (mn < r) r = mn;
-    const double d2v = c->dt * cflv / mx;
-    if (d2v < r) r = d2v;
     c->dtrec = r;
     if (bp > 0 || bc > 0) {
-        c->err_cycle = c->cycle;
+        c->err_cycle = v.cycle;
         c->err_pred = bp > 0 ? 1 : 0;
         c->done = 1;
-        return;
+        return 1;
     }
-    c->time = c->time + c->dt;
-    c->cycle = c->cycle + 1;
-    c->dtlast = c->dt;
-    c->done = (c->cycle >= cstop || c->time >= tstop) ? 1 : 0;
+    const double time = v.time + v.dt;
+    const int cycle = v.cycle + 1;
+    const double dtlast = v.dt;
     double dt = dtmax;
-    const double dtrecover = dtfac * c->dtlast;
+    const double dtrecover = dtfac * dtlast;
     if (dtrecover < dt) dt = dtrecover;
-    if ((tstop - c->time) < dt) dt = tstop - c->time;
-    if (c->dtrec < dt) dt = c->dtrec;
+    if ((tstop - time) < dt) dt = tstop - time;
+    if (r < dt) dt = r;
+    const int done = (cycle >= cstop || time >= tstop) ? 1 : 0;
+    c->time = time;
+    c->cycle = cycle;
+    c->dtlast = dtlast;
+    c->done = done;
     c->dt = dt;
+    if (next_dt) *next_dt = dt;
+    return done;
 }
-
 
 __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
                                                            double cflv, int cstop, double dtmax,
@@ -1315,6 +1556,80 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npre
     if (a.zcount && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
     finalize_block(a.ctl, a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, cflv, cstop,
                    dtmax, dtfac, tstop);
+}
+
+// K4 of the pipelined mixed cycle: ends cycle n from its predictor's results
+// (carried in Ctl::la_*) and its corrector's partials, then accepts the
+// look-ahead predictor of cycle n+1 when the dt it assumed is the dt just
+// computed, or asks for it to be redone (clearing the learned pass's zone
+// counts, which the redo fills again).  The learned pass's zone counts of
+// cycle n move from the pending to the reported counters.
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int npred, int ncorr,
+                                                                DtParams q) {
+    Ctl* c = a.ctl;
+    if (c->done) return;
+    double mn = 1.e99, mx = 1.e-99;
+    int bp = 0, bc = 0;
+    for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
+        const double v = a.part_dt[i];
+        mn = (v < mn) ? v : mn;
+        bp += a.part_bp[i];
+    }
+    for (int i = threadIdx.x; i < ncorr; i += PN_FIN_BLOCK) {
+        const double v = a.part_dv[i];
+        mx = (v > mx) ? v : mx;
+        bc += a.part_bc[i];
+    }
+    mn = blk_min_fin(mn);
+    mx = blk_max_fin(mx);
+    bp = blk_sum_fin(bp);
+    bc = blk_sum_fin(bc);
+    if (threadIdx.x != 0) return;
+    if (a.cl_stats) {
+        const unsigned long long nl = atomicExch(a.cl_stats + 2, 0ull);
+        const unsigned long long nf = atomicExch(a.cl_stats + 3, 0ull);
+        atomicAdd(a.cl_stats, nl);
+        atomicAdd(a.cl_stats + 1, nf);
+    }
+    const CtlSnap v = snap_ctl(c);
+    bool last;
+    const double dtn = lookahead_dt(v, q, last);
+    double dt1 = 0.;
+    if (end_cycle(c, v.la_dtnew, mx, v.la_bp, bc, q.cflv, q.cstop, q.dtmax, q.dtfac, q.tstop, &dt1))
+        return;
+    if (!last && dt1 == dtn) {
+        c->la_dtnew = mn;
+        c->la_bp = bp;
+        c->redo = 0;
+    } else {
+        c->redo = 1;
+        c->la_redos += 1;
+        if (a.zcount)
+            for (int r = 0; r < a.nzruns; ++r) a.zcount[r] = 0u;
+    }
+}
+
+// After a predictor of the pipelined cycle (the prologue's, or with REDO a
+// redone one): its dt limit and bad-side count become the Ctl::la_* values
+// the next finalize ends the cycle with.
+template <bool REDO>
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_la(FusedArgs a, int npred) {
+    Ctl* c = a.ctl;
+    if (c->done) return;
+    if (REDO && !c->redo) return;
+    double mn = 1.e99;
+    int bp = 0;
+    for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
+        const double v = a.part_dt[i];
+        mn = (v < mn) ? v : mn;
+        bp += a.part_bp[i];
+    }
+    mn = blk_min_fin(mn);
+    bp = blk_sum_fin(bp);
+    if (threadIdx.x != 0) return;
+    c->la_dtnew = mn;
+    c->la_bp = bp;
+    c->redo = 0;
 }
 
 FusedArgs make_args(pennant_problem* pb) {
@@ -1326,7 +1641,7 @@ FusedArgs make_args(pennant_problem* pb) {
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
     a.cfx = pb->cfx; a.cfy = pb->cfy; a.cmass = pb->cmass; a.sfwx = pb->sfwx; a.sfwy = pb->sfwy;
-    a.fcrn = pb->fcrn; a.fsfw = pb->fsfw; a.fsmf = pb->fsmf;
+    a.fcrn = pb->fcrn; a.fsfw = pb->fsfw; a.fsmf = pb->fsmf; a.zpn = pb->zpn;
     a.vsx = pb->vsx; a.vsy = pb->vsy; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
     a.heavy = pb->heavy; a.nheavy = pb->nheavy;
     a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->npart_pred;
@@ -1351,6 +1666,15 @@ FusedArgs make_args(pennant_problem* pb) {
 __global__ void __launch_bounds__(PN_BLOCK) k_to_float(int n, const double* src, float* dst) {
     const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
     if (i < n) dst[i] = (float)src[i];
+}
+
+// Small fills (kernels rather than memsets, so captured graphs hold kernel
+// nodes only); one block of 64 threads, n at most 64.
+__global__ void __launch_bounds__(64) k_zero_u32(unsigned* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+__global__ void __launch_bounds__(64) k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 int grid_for(int needed, int cap) { return needed < 1 ? 1 : (needed < cap ? needed : cap); }
@@ -1622,10 +1946,11 @@ int fused_setup(pennant_problem* pb) {
     const int nruns = (int)pb->zruns.size() / 3;
     pb->npart_pred = nruns * PN_MAX_GRID;
     pb->npart_corr = nruns * PN_MAX_GRID;
-    pb->cl_counters = (unsigned long long*)al(3, 8);
+    pb->cl_counters = (unsigned long long*)al(5, 8);
+    pb->zpn = (double*)al((size_t)h.numz, 8);
     pb->zlist = (int*)al((size_t)h.numz, 4);
     pb->zcount = (unsigned*)al(pb->zruns.size() / 3 + 1, 4);
-    if (!pb->cl_counters || !pb->zlist || !pb->zcount) return (int)hipErrorOutOfMemory;
+    if (!pb->cl_counters || !pb->zlist || !pb->zcount || !pb->zpn) return (int)hipErrorOutOfMemory;
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
     if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
@@ -1712,6 +2037,138 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
     if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred, ncorr, q.cflv,
                        q.cstop, q.dtmax, q.dtfac, q.tstop);
+    if (ev) (void)hipEventRecord(ev[4], s);
+    return (int)hipGetLastError();
+}
+
+namespace {
+
+DtParams dt_params(const pennant_problem* pb) {
+    const Params& q = pb->prm;
+    DtParams d;
+    d.cflv = q.cflv; d.dtmax = q.dtmax; d.dtfac = q.dtfac; d.tstop = q.tstop; d.cstop = q.cstop;
+    return d;
+}
+
+// Predictor launches of the pipelined cycle (the prologue's, or with REDO
+// the redo), writing zp to zpn; returns the number of partials.
+template <bool REDO>
+int launch_lookahead_pred(const pennant_problem* pb, const FusedArgs& a, int form, hipStream_t s) {
+    FusedArgs b = a;
+    b.zp = pb->zpn;
+    const HostMesh& h = pb->hm;
+    int poff = 0;
+    for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+        const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+        const int sb = h.zsbeg[zb];
+        const dim3 grid(pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
+        const int run = (int)(r / 3);
+        const bool fl = (form == FORM_LEARNED);
+#define PN_LA(NS)                                                                                  \
+    if (fl) hipLaunchKernelGGL((k_pred_mixed<NS, true, REDO>), grid, blk, 0, s, b, zb, ze, sb, poff, run); \
+    else hipLaunchKernelGGL((k_pred_mixed<NS, false, REDO>), grid, blk, 0, s, b, zb, ze, sb, poff, run);
+        switch (ns) {
+            case 3: PN_LA(3) break;
+            case 4: PN_LA(4) break;
+            case 5: PN_LA(5) break;
+            case 6: PN_LA(6) break;
+            case 7: PN_LA(7) break;
+            case 8: PN_LA(8) break;
+            default: return -1;
+        }
+#undef PN_LA
+        poff += grid.x;
+    }
+    return poff;
+}
+
+// The learned pass over the zones a flagging predictor listed; with
+// `pending` its counts go to the pending counters.
+int launch_learned_pass(const pennant_problem* pb, const FusedArgs& a, bool pending, hipStream_t s) {
+    FusedArgs b = a;
+    if (pending) b.cl_stats = pb->cl_counters + 3;
+    const HostMesh& h = pb->hm;
+    for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+        const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+        const int sb = h.zsbeg[zb];
+        // the learned pass holds 2 blocks per CU (registers): 512 blocks
+        // fill the chip without dispatching blocks that find no work
+        const int g = pred_grid(pb, ze - zb, ns, FORM_LEARNED);
+        const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
+        const int run = (int)(r / 3);
+        switch (ns) {
+            case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, b, zb, sb, run); break;
+            default: return PENNANT_EINVAL;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+int fused_prologue(pennant_problem* pb, int form) {
+    if (form == FORM_FP64 || (form == FORM_LEARNED && !pb->closure)) return PENNANT_EINVAL;
+    const FusedArgs a = make_args(pb);
+    hipStream_t s = pb->stream;
+    const int nz = (int)pb->zruns.size() / 3;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pb->zcount, nz);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, pb->cl_counters + 3, 2);
+    const int npred = launch_lookahead_pred<false>(pb, a, form, s);
+    if (npred < 0) return PENNANT_EINVAL;
+    hipLaunchKernelGGL(k_finalize_la<false>, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred);
+    if (form == FORM_LEARNED) {
+        const int rc = launch_learned_pass(pb, a, true, s);
+        if (rc) return rc;
+    }
+    return (int)hipGetLastError();
+}
+
+int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
+    if (form == FORM_FP64 || (form == FORM_LEARNED && !pb->closure)) return PENNANT_EINVAL;
+    const FusedArgs a = make_args(pb);
+    const DtParams q = dt_params(pb);
+    hipStream_t s = pb->stream;
+    const HostMesh& h = pb->hm;
+    const bool fl = (form == FORM_LEARNED);
+    if (ev) (void)hipEventRecord(ev[0], s);
+    hipLaunchKernelGGL(k_point<float>, dim3(pb->nheavy + nblocks(h.nump)), dim3(PN_BLOCK), 0, s, a);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    int poff = 0;
+    for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+        const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+        const int sb = h.zsbeg[zb];
+        const dim3 grid(pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
+        const int run = (int)(r / 3);
+#define PN_ZP(NS)                                                                                  \
+    if (fl) hipLaunchKernelGGL((k_zone_pipe<NS, true>), grid, blk, 0, s, a, zb, ze, sb, poff, run, q); \
+    else hipLaunchKernelGGL((k_zone_pipe<NS, false>), grid, blk, 0, s, a, zb, ze, sb, poff, run, q);
+        switch (ns) {
+            case 3: PN_ZP(3) break;
+            case 4: PN_ZP(4) break;
+            case 5: PN_ZP(5) break;
+            case 6: PN_ZP(6) break;
+            case 7: PN_ZP(7) break;
+            case 8: PN_ZP(8) break;
+            default: return PENNANT_EINVAL;
+        }
+#undef PN_ZP
+        poff += grid.x;
+    }
+    if (ev) (void)hipEventRecord(ev[2], s);
+    hipLaunchKernelGGL(k_finalize_pipe, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, poff, poff, q);
+    const int npred = launch_lookahead_pred<true>(pb, a, form, s);
+    if (npred != poff) return PENNANT_EINVAL;
+    hipLaunchKernelGGL(k_finalize_la<true>, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    if (fl) {
+        const int rc = launch_learned_pass(pb, a, true, s);
+        if (rc) return rc;
+    }
     if (ev) (void)hipEventRecord(ev[4], s);
     return (int)hipGetLastError();
 }

@@ -28,6 +28,12 @@ struct Ctl {
     int numsbad_corr;
     int err_cycle;    // cycle whose bad-side check failed, or -1
     int err_pred;     // 1 when the failed check was the predictor's
+    // look-ahead of the pipelined mixed cycle: the predictor of the next
+    // cycle (its dt limit and bad-side count), and whether it must be redone
+    double la_dtnew;
+    int la_bp;
+    int redo;
+    int la_redos;     // redone look-ahead predictors since the problem was created
 };
 
 #define PN_MAX_GRID 4096   // largest grid of a fused K1/K3 launch
@@ -39,6 +45,7 @@ struct Tuning {
     int max_blocks = 2048;   // grid cap of the grid-stride K1/K3 launches
     int xcd = 1;             // XCD-aware unit mapping of the grid-stride launches
     int graph = 1;           // batched cycles through a captured HIP graph
+    int pipeline = 0;        // mixed forms: next cycle's predictor in the corrector pass
 };
 
 struct Params {
@@ -101,12 +108,19 @@ struct pennant_problem {
     const void* closure = nullptr;
     float* cap_buf = nullptr;
     int64_t cap = 0;
-    unsigned long long* cl_counters = nullptr;   // [0] tuples, [1] learned zones, [2] fallback zones
+    // [0] tuples, [1] learned zones, [2] fallback zones; [3], [4] the same
+    // counts of a look-ahead predictor, added to [1], [2] when its cycle ends
+    unsigned long long* cl_counters = nullptr;
     int* zlist = nullptr;            // zones for the learned pass (per run, at the run's zone range)
     unsigned* zcount = nullptr;      // per run
     pn::Tuning tune;
     hipGraphExec_t gexec = nullptr;  // captured fused cycles of form graph_form
-    int graph_form = -1;
+    int graph_form = -1;             // form, + 16 for the pipelined cycle
+    // pipelined mixed cycle: the form whose look-ahead predictor results
+    // (corner records, side work forces, zpn, Ctl la_*) hold for the current
+    // cycle, or -1 when they must be computed first
+    int la_form = -1;
+    double* zpn = nullptr;           // the look-ahead predictor's zp
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
     double bcvx[8], bcvy[8];
@@ -188,6 +202,14 @@ int fused_setup(pennant_problem* pb);
 enum { FORM_FP64 = 0, FORM_MIXED = 1, FORM_LEARNED = 2 };
 // ev: NULL, or 5 events recorded before K1, K2, K3, K4 and after K4
 int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form);
+// Pipelined mixed cycle (FORM_MIXED, FORM_LEARNED): K2, then one zone pass
+// with this cycle's corrector and the next cycle's predictor, K4, and a redo
+// of the predictor when the next dt differs from the one it assumed.
+// fused_prologue computes the current cycle's predictor first.
+// ev: NULL, or 5 events recorded before K2, the zone pass, K4 and the redo,
+// the learned pass, and after it
+int fused_prologue(pennant_problem* pb, int form);
+int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);
 int launch_end_cycle(pennant_problem* pb, int compute_dtrec);

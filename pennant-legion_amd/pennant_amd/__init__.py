@@ -164,6 +164,7 @@ DRIVER_SIGNATURES = {
     "pennant_bench_copy": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                            ctypes.c_int),
     "pennant_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "pennant_lookahead_redos": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
 }
 
 
@@ -275,7 +276,10 @@ class Problem:
     def time_cycles(self, ncycles: int, mode: int = MODE_FUSED, phases: bool = True):
         """(total_ms, phase_ms or None) for ncycles cycles timed with HIP
         events on the problem's stream; phase_ms holds the summed durations of
-        the zone-predictor, point, zone-corrector and finalize kernels."""
+        the zone-predictor, point, zone-corrector and finalize kernels (for
+        the pipelined mixed cycle: zone pass, point kernel, learned pass,
+        finalize with the predictor redo).  Without phases the cycles run as
+        pennant_launch_cycles runs them (captured graphs)."""
         tot = ctypes.c_double()
         ph = (ctypes.c_double * 4)() if phases else None
         self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
@@ -293,6 +297,12 @@ class Problem:
 
     def get_tuning(self, key: str) -> int:
         return lib().pennant_get_tuning(self._h, key.encode())
+
+    def lookahead_redos(self) -> int:
+        """Look-ahead predictors the pipelined mixed cycle has redone."""
+        n = ctypes.c_int64()
+        self._check(lib().pennant_lookahead_redos(self._h, ctypes.byref(n)))
+        return int(n.value)
 
     def scalars(self) -> dict:
         out = (ctypes.c_double * 10)()
