@@ -359,11 +359,28 @@ __device__ __forceinline__ f2 fmk(float x, float y) { f2 v; v.x = x; v.y = y; re
 __device__ __forceinline__ f2 tof(d2 v) { return fmk((float)v.x, (float)v.y); }
 __device__ __forceinline__ f2 fadd(f2 a, f2 b) { return fmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ f2 fscl(float r, f2 v) { return fmk(v.x * r, v.y * r); }
-__device__ __forceinline__ float fdot(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
-__device__ __forceinline__ float fcross(f2 a, f2 b) { return a.x * b.y - a.y * b.x; }
+// 1/x in FP64 from the hardware reciprocal and two Newton steps (within an
+// ulp or two of the quotient; the mixed form's FP64 divisions use it)
+__device__ __forceinline__ double drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.);
+    return __builtin_fma(r, e, r);
+}
+
+// The mixed form's FP32 products fold one multiply into a fused multiply-add
+// (explicitly: the FP64 form is built without contraction, and explicit
+// fmaf keeps every kernel that runs this code bit-identical).
+__device__ __forceinline__ float fdot(f2 a, f2 b) { return __builtin_fmaf(a.x, b.x, a.y * b.y); }
+__device__ __forceinline__ float fcross(f2 a, f2 b) { return __builtin_fmaf(a.x, b.y, -(a.y * b.x)); }
+// a + r b
+__device__ __forceinline__ f2 faxpy(float r, f2 b, f2 a) {
+    return fmk(__builtin_fmaf(r, b.x, a.x), __builtin_fmaf(r, b.y, a.y));
+}
 // the hardware square root (1 ulp), without the correctly rounded expansion
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float flen(f2 v) { return fsqrt(v.x * v.x + v.y * v.y); }
+__device__ __forceinline__ float flen(f2 v) { return fsqrt(fdot(v, v)); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
@@ -449,7 +466,8 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         slot[k] = a.pcpos[sb + k];
     }
 #pragma unroll
-    for (int k = 0; k < NS; ++k) xp[k] = add(x0[k], scl(dth, u0[k]));
+    for (int k = 0; k < NS; ++k)
+        xp[k] = mk(__builtin_fma(dth, u0[k].x, x0[k].x), __builtin_fma(dth, u0[k].y, x0[k].y));
     // zone-mean velocity; velocities from here on live in FP32 relative to it
     d2 zuc = mk(0., 0.);
 #pragma unroll
@@ -487,7 +505,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
 #pragma unroll
     for (int k = 0; k < NS; ++k) zdl = fminr(zdl, fac * (float)sa[k] * frcp(el[k]));
     // zone EOS in FP64
-    const double zrp = zm / zvolp;
+    const double zrp = zm * drcp(zvolp);
     const double gm1 = a.gamma - 1.;
     const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
     const double e = smax(ze0, 0.);
@@ -496,12 +514,12 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     // pre + per pz / r^2 = (gamma - 1) e + (gamma - 1)^2 e = gamma (gamma - 1) e
     const double csqd = smax(ssmin2, a.gamma * gm1 * e);
     const double zss = __builtin_sqrt(csqd);
-    const double minv = 1. / zm;
+    const double minv = drcp(zm);
     const double dv = (zvolp - zvol0) * minv;
     const double bulk = r * csqd;
     const double denom = 1. + 0.5 * per * dv;
     const double src = zwr * dth * minv;
-    const double zp = pz + (per * src - r * bulk * dv) / denom;
+    const double zp = pz + (per * src - r * bulk * dv) * drcp(denom);
     if (valid && CL != CL_RECOMP) a.zp[z] = zp;
     // pressure and TTS side forces
     const float zpf = (float)zp;
@@ -548,7 +566,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         const f2 du31 = fmk(u3.x - u1.x, u3.y - u1.y);                  // up3 - up1
         const f2 dva = fmk(u1.x - ul[k].x - u3.x, u1.y - ul[k].y - u3.y);   // up1+up2-up0-up3
         const f2 dvb = fmk(u3.x - ul[k].x - u1.x, u3.y - ul[k].y - u1.y);   // up2+up3-up0-up1
-        const f2 duav = fadd(zucf, fscl(0.25f, fadd(fadd(ul[k], u1), u3)));
+        const f2 duav = faxpy(0.25f, fadd(fadd(ul[k], u1), u3), zucf);
         const float cvol = 0.5f * fcross(A, B);
         carea[k] = cvol;
         const float de1 = el[km], de2 = el[k];
@@ -660,7 +678,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     float cw[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const float csin2 = 1.f - ccos[k] * ccos[k];
+        const float csin2 = __builtin_fmaf(-ccos[k], ccos[k], 1.f);
         cw[k] = (csin2 < 1.e-4f) ? 0.f : carea[k] * frcp(csin2);
         ccos[k] = (csin2 < 1.e-4f) ? 0.f : ccos[k];
     }
@@ -670,10 +688,9 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int kp = nxt<NS>(k);
-        const f2 fq = fadd(fscl(cw[k], fadd(cqe2[k], fscl(ccos[k], cqe1[k]))),
-                           fscl(cw[kp], fadd(cqe1[kp], fscl(ccos[kp], cqe2[kp]))));
-        const f2 sfq = fscl(frcp(el[k]), fq);
-        const f2 sfw = fadd(sfp[k], sfq);
+        const f2 fq = faxpy(cw[k], faxpy(ccos[k], cqe1[k], cqe2[k]),
+                            fscl(cw[kp], faxpy(ccos[kp], cqe2[kp], cqe1[kp])));
+        const f2 sfw = faxpy(frcp(el[k]), fq, sfp[k]);
         sfwk[k] = sfw;
         tot[k] = mk((double)sfw.x + (double)sft[k].x, (double)sfw.y + (double)sft[k].y);
         // setVelDiff
@@ -697,7 +714,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         }
     }
     const double cdt = smax(zdu, smax(zss, 1.e-99));
-    return (double)zdl * a.cfl / cdt;
+    return (double)zdl * a.cfl * drcp(cdt);
 }
 
 // The predictor of zone z whose points are pp (mapsp1 of its sides); every
