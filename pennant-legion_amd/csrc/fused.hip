@@ -1088,6 +1088,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_corner(FusedArgs a, int zbeg,
 // K2
 
 // point update shared by the regular and the heavy-point paths
+// T = float (the mixed form): calcWork needs (pu0 + pu) only times pxp.x, so
+// that product is stored in vsx/vsy and pxpx is not written.
+template <typename T>
 __device__ __forceinline__ void point_update(const FusedArgs& a, int p, double mass, d2 f,
                                              double dt) {
     const double dth = 0.5 * dt;
@@ -1108,10 +1111,16 @@ __device__ __forceinline__ void point_update(const FusedArgs& a, int p, double m
     const d2 acc = dvs(f, smax(mass, fuzz));
     const d2 u = add(u0, scl(dt, acc));
     const d2 x = add(x0, scl(dt * 0.5, add(u0, u)));
-    a.pxpx[p] = x0.x + dth * uin.x;          // pxp.x for calcWork
+    const double xpx = x0.x + dth * uin.x;   // pxp.x for calcWork
     const d2 vs = add(u0, u);                // pu0 + pu for calcWork
-    a.vsx[p] = vs.x;
-    a.vsy[p] = vs.y;
+    if constexpr (sizeof(T) == 4) {
+        a.vsx[p] = vs.x * xpx;
+        a.vsy[p] = vs.y * xpx;
+    } else {
+        a.pxpx[p] = xpx;
+        a.vsx[p] = vs.x;
+        a.vsy[p] = vs.y;
+    }
     a.pxx[p] = x.x;
     a.pxy[p] = x.y;
     a.pux[p] = u.x;
@@ -1165,7 +1174,7 @@ __device__ __forceinline__ void point_heavy(const FusedArgs& a, int p, double dt
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) point_update(a, p, mass, f, dt);
+    if (threadIdx.x == 0) point_update<T>(a, p, mass, f, dt);
 }
 
 // An ordinary point, one thread; points with more than PN_HEAVY corners are
@@ -1181,7 +1190,7 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
         mass += C::mass(a, k);
         f = add(f, mk(C::fx(a, k), C::fy(a, k)));
     }
-    point_update(a, p, mass, f, dt);
+    point_update<T>(a, p, mass, f, dt);
 }
 
 // K2: blocks [0, nheavy) take the heavy points, the rest one point per thread.
@@ -1226,13 +1235,13 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
 #pragma unroll
         for (int k = 0; k < NS; ++k) sfw[k] = mk(C::wx(a, sb + k), C::wy(a, sb + k));
     }
-    d2 vs[NS];
+    d2 vs[NS];                 // mixed form: (pu0 + pu) pxp.x
     double xp[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
         vs[k] = mk(a.vsx[pp[k]], a.vsy[pp[k]]);
-        xp[k] = a.pxpx[pp[k]];
+        if constexpr (sizeof(T) == 8) xp[k] = a.pxpx[pp[k]];
     }
     const double dth = 0.5 * dt;
     const double n = (double)NS;
@@ -1257,7 +1266,7 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
         const d2 sftot = sfw[k];
         const double sd1 = dot(sftot, vs[k]);
         const double sd2 = dot(neg(sftot), vs[k2]);
-        const double dwork = -dth * (sd1 * xp[k] + sd2 * xp[k2]);
+        const double dwork = (sizeof(T) == 8) ? -dth * (sd1 * xp[k] + sd2 * xp[k2]) : -dth * (sd1 + sd2);
         et += dwork;
         w += dwork;
     }
