@@ -60,21 +60,22 @@ def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False, pipelined: bool = 
         return {
             # corrector and next predictor of each zone: mapsp1 4, pcpos 4,
             # smf 4, sfw 8 in + 8 out, corner record 12 out per side; zetot
-            # zvol zm zpn 32 in, zwrate zetot ze zr zvol zp zpn 56 out per
-            # zone; px 16, pu 16, (pu0 + pu) pxp.x 16 per point
-            "zone": 40 * S + 88 * Z + 48 * P,
+            # zvol zm zpn 32 in, zwrate zetot zvol zp zpn 40 out per zone;
+            # px 16, pu 16, (pu0 + pu) pxp.x 16 per point
+            "zone": 40 * S + 72 * Z + 48 * P,
             "point": 12 * S + 85 * P,
         }
     if mixed:
         return {
-            # mapsp1 4, pcpos 4, smf 4, cf 8, cmass 4, sfw 8 per side
-            "zone_pred": 32 * S + 48 * Z + 32 * P,
+            # mapsp1 4, pcpos 4, smf 4, cf 8, cmass 4, sfw 8 per side; zm zvol
+            # zetot zwrate 32 in (zr and ze derived), zp 8 out per zone
+            "zone_pred": 32 * S + 40 * Z + 32 * P,
             # cf 8, cmass 4 per corner; pcbeg 4, px/pu 32 in + 32 out, bc bits
             # 1, (pu0 + pu) pxp.x 16 out per point
             "point": 12 * S + 85 * P,
-            # mapsp1 4, sfw 8 per side; zone 32 in, 40 out; px 16, (pu0 + pu)
-            # pxp.x 16 per point
-            "zone_corr": 12 * S + 72 * Z + 32 * P,
+            # mapsp1 4, sfw 8 per side; zetot zvol zp 24 in, zwrate zetot zvol
+            # 24 out per zone; px 16, (pu0 + pu) pxp.x 16 per point
+            "zone_corr": 12 * S + 48 * Z + 32 * P,
         }
     return {
         # mapsp1 4, pcpos 4, smf 8, cf.x/cf.y/cmass/sfw.x/sfw.y 40 per side;

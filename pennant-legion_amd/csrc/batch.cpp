@@ -275,6 +275,9 @@ int pennant_batch_launch_cycles(pennant_batch* b, int ncycles, int mode) {
     const int form = batch_form(mode);
     if (form < 0 || ncycles < 0) return PENNANT_EINVAL;
     if (form == FORM_LEARNED && !b->closure) return PENNANT_EINVAL;
+    // the FP64 form reads zr and ze; mixed cycles leave them derived
+    if (form == FORM_FP64) BCK(batch_materialize_zre(b));
+    else b->zre_stale = true;
     int left = ncycles;
     if (b->tune.graph && left >= PN_GRAPH_CYCLES) {
         if (b->graph_form != form || !b->gexec) BCK(build_graph(b, form));
@@ -341,6 +344,7 @@ int pennant_batch_scalars(pennant_batch* b, int i, double* o) {
 
 int pennant_batch_get_field(pennant_batch* b, int i, const char* name, double* out) {
     if (i < 0 || i >= b->nprob) return PENNANT_EINVAL;
+    BCK(batch_materialize_zre(b));
     const int64_t zo = b->zoff[i], nz = b->zoff[i + 1] - zo;
     const int64_t po = b->poff[i], np = b->poff[i + 1] - po;
     const std::string f(name);
@@ -444,6 +448,7 @@ int pennant_batch_autotune(pennant_batch* b, int mode, int cycles, double* best_
             b->tune.graph = 0;
             drop_graph(b);
             rc = snap_restore(b, sn);
+            b->zre_stale = true;
             // one untimed cycle, then the timed ones
             if (!rc) rc = pennant_batch_launch_cycles(b, 1, mode);
             double ms = 0.;
@@ -456,6 +461,7 @@ int pennant_batch_autotune(pennant_batch* b, int mode, int cycles, double* best_
     b->tune = best;
     drop_graph(b);
     const int rc2 = snap_restore(b, sn);
+    b->zre_stale = true;
     snap_free(sn);
     if (best_ms) *best_ms = bms;
     return rc ? rc : rc2;

@@ -390,6 +390,7 @@ pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen)
         return fail("hipHostMalloc failed");
     std::memset(pb->ctl_host, 0, sizeof(Ctl));
     pb->ctl_host->err_cycle = -1;
+    pb->ctl_host->zre_valid = 1;
     if (hipMemcpy(pb->ctl, pb->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(pb->f("px", 0), h.px.data(), h.px.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(pb->f("px", 1), h.py.data(), h.py.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
@@ -529,6 +530,9 @@ static int fused_form(int mode) {
 
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
     const int form = fused_form(mode);
+    // the FP64 and task forms read zr and ze; mixed cycles leave them derived
+    if (form == FORM_FP64 || mode == PENNANT_MODE_TASKS) CK(materialize_zre(pb));
+    else if (form > 0) pb->zre_stale = true;
     if (form >= 0) {
         if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
         int left = ncycles;
@@ -565,6 +569,8 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
     for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
     int rc = phases ? ensure_lookahead(pb, form) : 0;
+    if (!rc && phases && form == FORM_FP64) rc = materialize_zre(pb);
+    if (phases && form > 0) pb->zre_stale = true;
     const bool pipe = phases && pipelined(pb, form);
     if (!rc) rc = (int)hipEventRecord(ev[0], pb->stream);
     if (!rc) {
@@ -665,6 +671,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     }
     auto restore = [&]() {
         drop_lookahead(pb);
+        pb->zre_stale = true;        // the restored flag decides; materialising checks it
         int r = 0;
         for (size_t i = 0; i < copy.size() && !r; ++i)
             r = (int)hipMemcpyAsync(st[i].first, copy[i], st[i].second, hipMemcpyDeviceToDevice, pb->stream);
@@ -753,6 +760,7 @@ int pennant_field_info(pennant_problem* pb, const char* name, int64_t* count, in
 }
 
 int pennant_get_field(pennant_problem* pb, const char* name, double* out) {
+    CK(materialize_zre(pb));
     auto it = pb->fields.find(name);
     if (it == pb->fields.end()) return PENNANT_EINVAL;
     const DevField& d = it->second;
@@ -768,6 +776,7 @@ int pennant_get_field(pennant_problem* pb, const char* name, double* out) {
 
 int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
     drop_lookahead(pb);
+    CK(materialize_zre(pb));
     auto it = pb->fields.find(name);
     if (it == pb->fields.end()) return PENNANT_EINVAL;
     const DevField& d = it->second;
@@ -784,6 +793,7 @@ int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
 double* pennant_field_device(pennant_problem* pb, const char* name, int comp) {
     // the caller may write through the pointer
     drop_lookahead(pb);
+    if (materialize_zre(pb)) return nullptr;
     auto it = pb->fields.find(name);
     if (it == pb->fields.end() || comp < 0 || comp >= it->second.ncomp) return nullptr;
     return it->second.c[comp];
@@ -861,6 +871,7 @@ int pennant_begin_cycle(pennant_problem* pb) {
 
 int pennant_step(pennant_problem* pb, int step) {
     drop_lookahead(pb);
+    CK(materialize_zre(pb));
     CK(task_step(pb, step));
     CK(copy_ctl_to_host(pb));
     if (step == 18 && pb->ctl_host->numsbad_pred > 0) return PENNANT_EBADSIDES;
@@ -870,6 +881,7 @@ int pennant_step(pennant_problem* pb, int step) {
 
 int pennant_end_cycle(pennant_problem* pb) {
     drop_lookahead(pb);
+    CK(materialize_zre(pb));
     CK(launch_end_cycle(pb, 0));
     return pennant_sync(pb);
 }

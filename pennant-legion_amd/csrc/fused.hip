@@ -731,10 +731,16 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     }
     ZoneState st;
     st.zm = a.zm[z];
-    st.zr = a.zr[z];
-    st.ze = a.ze[z];
     st.zvol = a.zvol[z];
     st.zwrate = a.zwrate[z];
+    if (a.ctl->zre_valid) {
+        st.zr = a.zr[z];
+        st.ze = a.ze[z];
+    } else {
+        // the mixed corrector stored neither; these are its quotients
+        st.zr = st.zm / st.zvol;
+        st.ze = a.zetot[z] / (st.zm + 1.e-99);
+    }
     return zone_pred_mixed_core<NS, CL>(a, z, sb, x0, u0, st, dt, bad, valid, cnt);
 }
 
@@ -1278,9 +1284,12 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
     a.zetot[z] = et;
     const double fuzz = 1.e-99;
     const double ze = et / (zm + fuzz);
-    a.ze[z] = ze;
     const double zr = zm / zvol;
-    a.zr[z] = zr;
+    if constexpr (sizeof(T) == 8) {
+        // the mixed form's predictors derive zr and ze from zm, zvol, zetot
+        a.ze[z] = ze;
+        a.zr[z] = zr;
+    }
     a.zvol[z] = zvol;
     if (PIPE) a.zp[z] = zp;
     const double zdvov = fabs((zvol - zvol0) / zvol0);
@@ -1505,12 +1514,12 @@ __device__ __forceinline__ int blk_sum_fin(int v) {
 // from npred predictor and ncorr corrector partials.
 __device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
                                          int cstop, double dtmax, double dtfac, double tstop,
-                                         double* next_dt = nullptr);
+                                         int zre, double* next_dt = nullptr);
 
 __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, const int* part_bp,
                                                int npred, const double* part_dv, const int* part_bc,
                                                int ncorr, double cflv, int cstop, double dtmax,
-                                               double dtfac, double tstop) {
+                                               double dtfac, double tstop, int zre) {
     if (c->done) return;
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
@@ -1529,7 +1538,7 @@ __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, co
     bp = blk_sum_fin(bp);
     bc = blk_sum_fin(bc);
     if (threadIdx.x != 0) return;
-    end_cycle(c, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop);
+    end_cycle(c, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop, zre);
 }
 
 // The end of a cycle from its predictor's dt limit and bad-side count (mn,
@@ -1538,10 +1547,12 @@ __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, co
 // scalar-cache load is not ordered with a later vector store to the same
 // address).
 // Returns the new done flag; *next_dt (when given) receives the next dt.
+// zre: whether this cycle's corrector stored zr and ze (Ctl::zre_valid).
 __device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
                                          int cstop, double dtmax, double dtfac, double tstop,
-                                         double* next_dt) {
+                                         int zre, double* next_dt) {
     const CtlSnap v = snap_ctl(c);
+    c->zre_valid = zre;
     double r = 1.e99;
     if (mn < r) r = mn;
     const double d2v = v.dt * cflv / mx;
@@ -1577,11 +1588,11 @@ __device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, i
 
 __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npred, int ncorr,
                                                            double cflv, int cstop, double dtmax,
-                                                           double dtfac, double tstop) {
+                                                           double dtfac, double tstop, int zre) {
     // the learned pass of this cycle has consumed its zone lists
     if (a.zcount && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
     finalize_block(a.ctl, a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, cflv, cstop,
-                   dtmax, dtfac, tstop);
+                   dtmax, dtfac, tstop, zre);
 }
 
 // K4 of the pipelined mixed cycle: ends cycle n from its predictor's results
@@ -1621,7 +1632,7 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
     bool last;
     const double dtn = lookahead_dt(v, q, last);
     double dt1 = 0.;
-    if (end_cycle(c, v.la_dtnew, mx, v.la_bp, bc, q.cflv, q.cstop, q.dtmax, q.dtfac, q.tstop, &dt1))
+    if (end_cycle(c, v.la_dtnew, mx, v.la_bp, bc, q.cflv, q.cstop, q.dtmax, q.dtfac, q.tstop, 0, &dt1))
         return;
     if (!last && dt1 == dtn) {
         c->la_dtnew = mn;
@@ -1692,6 +1703,21 @@ FusedArgs make_args(pennant_problem* pb) {
 __global__ void __launch_bounds__(PN_BLOCK) k_to_float(int n, const double* src, float* dst) {
     const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
     if (i < n) dst[i] = (float)src[i];
+}
+
+// zr and ze as the FP64 corrector stores them (zr = zm / zvol, ze = zetot /
+// (zm + 1e-99)), where mixed cycles left them to be derived; then the flag.
+__global__ void __launch_bounds__(PN_BLOCK) k_zre(FusedArgs a, int nz) {
+    if (a.ctl->zre_valid) return;
+    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (z < nz) {
+        const double zm = a.zm[z];
+        a.zr[z] = zm / a.zvol[z];
+        a.ze[z] = a.zetot[z] / (zm + 1.e-99);
+    }
+}
+__global__ void __launch_bounds__(64) k_zre_done(Ctl* c, int n) {
+    for (int i = threadIdx.x; i < n; i += 64) c[i].zre_valid = 1;
 }
 
 // Small fills (kernels rather than memsets, so captured graphs hold kernel
@@ -1792,10 +1818,11 @@ __attribute__((amdgpu_waves_per_eu((CL == CL_LEARNED) ? 2 : ((FORM == FORM_MIXED
 k_pred_batch(BatchArgs B) {
     for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
         const BUnit U = B.units[w.u];
-        const Ctl* c = B.ctls + U.pid;
+        Ctl* c = B.ctls + U.pid;
         if (c->done) continue;
         FusedArgs a = B.a;
         apply_prob(a, B.probs[U.pid]);
+        a.ctl = c;                 // the problem's zr/ze flag for the predictor
         const double dt = c->dt;
         double dtmin = 1.e99;
         int bad = 0;
@@ -1862,12 +1889,26 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr_batch(BatchArgs B) {
     }
 }
 
-__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_batch(BatchArgs B) {
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_batch(BatchArgs B, int zre) {
     const BProb& q = B.probs[blockIdx.x];
     const int n = q.uend - q.ubeg;
     finalize_block(B.ctls + blockIdx.x, B.a.part_dt + q.ubeg, B.a.part_bp + q.ubeg, n,
                    B.a.part_dv + q.ubeg, B.a.part_bc + q.ubeg, n, q.cflv, q.cstop, q.dtmax,
-                   q.dtfac, q.tstop);
+                   q.dtfac, q.tstop, zre);
+}
+
+// k_zre over the units of every problem whose flag is clear.
+__global__ void __launch_bounds__(PN_BLOCK) k_zre_batch(BatchArgs B) {
+    for (Units w = block_units(B.nunits, 0); w.u < w.end; w.u += w.step) {
+        const BUnit U = B.units[w.u];
+        if (B.ctls[U.pid].zre_valid) continue;
+        const int z = U.zb + (int)threadIdx.x;
+        if (z < U.ze) {
+            const double zm = B.a.zm[z];
+            B.a.zr[z] = zm / B.a.zvol[z];
+            B.a.ze[z] = B.a.zetot[z] / (zm + 1.e-99);
+        }
+    }
 }
 
 BatchArgs make_batch_args(pennant_batch* b) {
@@ -2062,7 +2103,7 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
     const int ncorr = poff;
     if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred, ncorr, q.cflv,
-                       q.cstop, q.dtmax, q.dtfac, q.tstop);
+                       q.cstop, q.dtmax, q.dtfac, q.tstop, mixed ? 0 : 1);
     if (ev) (void)hipEventRecord(ev[4], s);
     return (int)hipGetLastError();
 }
@@ -2199,6 +2240,26 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     return (int)hipGetLastError();
 }
 
+int materialize_zre(pennant_problem* pb) {
+    if (!pb->zre_stale) return 0;
+    const FusedArgs a = make_args(pb);
+    const int nz = pb->hm.numz;
+    hipLaunchKernelGGL(k_zre, dim3(nblocks(nz)), dim3(PN_BLOCK), 0, pb->stream, a, nz);
+    hipLaunchKernelGGL(k_zre_done, dim3(1), dim3(64), 0, pb->stream, pb->ctl, 1);
+    pb->zre_stale = false;
+    return (int)hipGetLastError();
+}
+
+int batch_materialize_zre(pennant_batch* b) {
+    if (!b->zre_stale) return 0;
+    const BatchArgs B = make_batch_args(b);
+    hipLaunchKernelGGL(k_zre_batch, dim3(grid_for(b->nunits, PN_MAX_GRID)), dim3(PN_BLOCK), 0,
+                       b->stream, B);
+    hipLaunchKernelGGL(k_zre_done, dim3(1), dim3(64), 0, b->stream, b->ctls, b->nprob);
+    b->zre_stale = false;
+    return (int)hipGetLastError();
+}
+
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s) {
     if (n > 0)
         hipLaunchKernelGGL(k_to_float, dim3(nblocks(n)), dim3(PN_BLOCK), 0, s, (int)n, src, dst);
@@ -2226,7 +2287,8 @@ int batch_cycle(pennant_batch* b, int form) {
         hipLaunchKernelGGL(k_point_batch<float>, pg, blk, 0, s, B);
         hipLaunchKernelGGL(k_corr_batch<float>, zg, blk, 0, s, B);
     }
-    hipLaunchKernelGGL(k_finalize_batch, dim3(b->nprob), dim3(PN_FIN_BLOCK), 0, s, B);
+    hipLaunchKernelGGL(k_finalize_batch, dim3(b->nprob), dim3(PN_FIN_BLOCK), 0, s, B,
+                       form == FORM_FP64 ? 1 : 0);
     return (int)hipGetLastError();
 }
 

@@ -34,6 +34,11 @@ struct Ctl {
     int la_bp;
     int redo;
     int la_redos;     // redone look-ahead predictors since the problem was created
+    // 1: zr and ze in memory are current (after initialisation, FP64 cycles
+    // and field writes) and the next predictor reads them; 0: the mixed
+    // corrector stored neither, and predictors derive them from zm, zvol and
+    // zetot with the same quotients until they are materialised
+    int zre_valid;
 };
 
 #define PN_MAX_GRID 4096   // largest grid of a fused K1/K3 launch
@@ -121,6 +126,7 @@ struct pennant_problem {
     // cycle, or -1 when they must be computed first
     int la_form = -1;
     double* zpn = nullptr;           // the look-ahead predictor's zp
+    bool zre_stale = false;          // mixed cycles ran since zr, ze were last materialised
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
     int nbcs = 0;
     double bcvx[8], bcvy[8];
@@ -189,6 +195,7 @@ struct pennant_batch {
     // captured cycles: graph_cycles cycles of form graph_form
     hipGraphExec_t gexec = nullptr;
     int graph_form = -1, graph_cycles = 0;
+    bool zre_stale = false;          // mixed cycles ran since zr, ze were last materialised
     std::string err;
 };
 
@@ -210,6 +217,10 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form);
 // the learned pass, and after it
 int fused_prologue(pennant_problem* pb, int form);
 int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form);
+// Stores zr = zm / zvol and ze = zetot / (zm + 1e-99) where mixed cycles left
+// them to be derived (the mixed corrector does not store them).
+int materialize_zre(pennant_problem* pb);
+int batch_materialize_zre(pennant_batch* b);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);
 int launch_end_cycle(pennant_problem* pb, int compute_dtrec);

@@ -158,3 +158,48 @@ def test_mixed_and_plain_interleave_on_one_problem():
         b.run(20, pa.MODE_TASKS)
         for f in ZF + ("px", "pu"):
             assert np.array_equal(a.get(f).view(np.uint64), b.get(f).view(np.uint64)), f
+
+
+@pytest.mark.parametrize("mode", [pa.MODE_MIXED, pa.MODE_LEARNED])
+def test_mixed_zr_ze_are_the_corrector_quotients(mode):
+    """The mixed corrector does not store zr and ze; its predictors derive
+    them from zm, zvol and zetot, and reading the fields materialises them
+    with the FP64 corrector's quotients, bit for bit."""
+    from pennant_amd.closure import Mlp
+    from pennant_amd.train_closure import DEFAULT_OUT
+    m = Mlp.load(DEFAULT_OUT) if mode == pa.MODE_LEARNED else None
+    for text in (wd.sedov_rect(40, cstop=60), wd.noh_pie(30, 20, cstop=50, dtinit=1e-4)):
+        with pa.Problem(text) as pb:
+            if m is not None:
+                pb.set_closure(m)
+            pb.run(37, mode)
+            zm, zvol, zetot = pb.get("zm"), pb.get("zvol"), pb.get("zetot")
+            assert np.array_equal(pb.get("zr").view(np.uint64), (zm / zvol).view(np.uint64))
+            assert np.array_equal(pb.get("ze").view(np.uint64),
+                                  (zetot / (zm + 1e-99)).view(np.uint64))
+    if m is not None:
+        m.close()
+
+
+def test_written_zr_drives_the_next_mixed_predictor():
+    """A density written between mixed cycles is what the next predictor's
+    EOS reads (calcStateHalf reads ZR, src/PolyGas.cc:63-108): writing back
+    the materialised values changes nothing, a changed density changes the
+    next cycle's pressure in those zones."""
+    text = wd.sedov_rect(32, cstop=200)
+    with pa.Problem(text) as a, pa.Problem(text) as b, pa.Problem(text) as c:
+        for pb in (a, b, c):
+            pb.run(25, pa.MODE_MIXED)
+        zr = a.get("zr")
+        b.set("zr", zr)
+        zr2 = zr.copy()
+        zr2[::5] *= 1.0 + 1e-3
+        c.set("zr", zr2)
+        for pb in (a, b, c):
+            pb.run(1, pa.MODE_MIXED)
+        for f in ZF + ("px", "pu", "zetot", "zvol"):
+            assert np.array_equal(a.get(f).view(np.uint64), b.get(f).view(np.uint64)), f
+        zp_a, zp_c = a.get("zp")[::5], c.get("zp")[::5]
+        hot = zp_a != 0.0          # zero internal energy gives zero pressure at any density
+        assert hot.sum() > 10
+        assert np.all(zp_a[hot] != zp_c[hot])
