@@ -1094,16 +1094,29 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_corner(FusedArgs a, int zbeg,
 // K2
 
 // point update shared by the regular and the heavy-point paths
+// A point's state as K2 reads it.
+struct PointIn {
+    d2 x0, u0;
+    unsigned bits;
+};
+__device__ __forceinline__ PointIn load_point(const FusedArgs& a, int p) {
+    PointIn q;
+    q.x0 = mk(a.pxx[p], a.pxy[p]);
+    q.u0 = mk(a.pux[p], a.puy[p]);
+    q.bits = a.pbc[p];
+    return q;
+}
+
 // T = float (the mixed form): calcWork needs (pu0 + pu) only times pxp.x, so
 // that product is stored in vsx/vsy and pxpx is not written.
 template <typename T>
-__device__ __forceinline__ void point_update(const FusedArgs& a, int p, double mass, d2 f,
-                                             double dt) {
+__device__ __forceinline__ void point_update(const FusedArgs& a, int p, const PointIn& q,
+                                             double mass, d2 f, double dt) {
     const double dth = 0.5 * dt;
-    const d2 x0 = mk(a.pxx[p], a.pxy[p]);
-    const d2 uin = mk(a.pux[p], a.puy[p]);
+    const d2 x0 = q.x0;
+    const d2 uin = q.u0;
     d2 u0 = uin;
-    const unsigned bits = a.pbc[p];
+    const unsigned bits = q.bits;
     if (bits) {
         for (int b = 0; b < a.nbcs; ++b) {
             if (bits & (1u << b)) {
@@ -1180,23 +1193,44 @@ __device__ __forceinline__ void point_heavy(const FusedArgs& a, int p, double dt
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) point_update<T>(a, p, mass, f, dt);
+    if (threadIdx.x == 0) point_update<T>(a, p, load_point(a, p), mass, f, dt);
 }
 
 // An ordinary point, one thread; points with more than PN_HEAVY corners are
 // left to point_heavy.
+//
+// The point's state and its first four corners (most points have at most
+// four) are loaded together, then summed in ascending order.
 template <typename T>
 __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt) {
     using C = Corners<T>;
     const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
     if (ke - kb > PN_HEAVY) return;
+    const PointIn q = load_point(a, p);
+    double m4[4];
+    d2 f4[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // every point has a corner: clamped to the last one, the load is in
+        // range and unconditional (no branch between the loads)
+        const int k = (kb + j < ke) ? kb + j : ke - 1;
+        m4[j] = C::mass(a, k);
+        f4[j] = mk(C::fx(a, k), C::fy(a, k));
+    }
     double mass = 0.;
     d2 f = mk(0., 0.);
-    for (int k = kb; k < ke; ++k) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (kb + j < ke) {
+            mass += m4[j];
+            f = add(f, f4[j]);
+        }
+    }
+    for (int k = kb + 4; k < ke; ++k) {
         mass += C::mass(a, k);
         f = add(f, mk(C::fx(a, k), C::fy(a, k)));
     }
-    point_update<T>(a, p, mass, f, dt);
+    point_update<T>(a, p, q, mass, f, dt);
 }
 
 // K2: blocks [0, nheavy) take the heavy points, the rest one point per thread.
