@@ -164,9 +164,9 @@ def physics_check(pa, decks, mlp, n):
 
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
-    "mixed": {"zone_pred": "k_pred_mixed<4, false>", "point": "k_point<float>",
+    "mixed": {"zone_pred": "k_pred_mixed<4, false, false>", "point": "k_point<float>",
               "zone_corr": "k_corr<4, float>"},
-    "learned": {"zone_pred": "k_pred_mixed<4, true>", "point": "k_point<float>",
+    "learned": {"zone_pred": "k_pred_mixed<4, true, false>", "point": "k_point<float>",
                 "zone_corr": "k_corr<4, float>"},
 }
 

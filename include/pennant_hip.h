@@ -322,12 +322,18 @@ int pennant_set_scalars(pennant_problem* pb, const double* in10);
 
 /* Field access by reference field name ("px", "pu0", "zr", "sfq", ...;
  * lower-case FID names, plus "exp" for EXP).  ncomp is 1 or 2; data is
- * interleaved (x0,y0,x1,y1,...) for vector fields.  Synchronous. */
+ * interleaved (x0,y0,x1,y1,...) for vector fields.  Synchronous.
+ * The mixed modes do not store zr and ze each cycle (their predictors derive
+ * them from zm, zvol and zetot); these calls, the writers, pennant_energy
+ * and FP64 or task-mode cycles first store them, with the corrector's
+ * quotients.  A written zr or ze is what the next predictor reads. */
 int pennant_field_info(pennant_problem* pb, const char* name, int64_t* count, int* ncomp);
 int pennant_get_field(pennant_problem* pb, const char* name, double* host_out);
 int pennant_set_field(pennant_problem* pb, const char* name, const double* host_in);
-/* device pointer of one component (0 = x or scalar, 1 = y); the caller may
- * write through it, so the pipelined cycle's look-ahead is recomputed */
+/* device pointer of one component (0 = x or scalar, 1 = y), with zr and ze
+ * stored (on the problem's stream); the caller may write through it, so the
+ * pipelined cycle's look-ahead is recomputed.  NULL for an unknown name or a
+ * launch error. */
 double* pennant_field_device(pennant_problem* pb, const char* name, int comp);
 
 /* Maps as int32 host copies: "mapsp1", "mapsp2", "mapsz", "mapss3", "mapss4",

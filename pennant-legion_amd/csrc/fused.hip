@@ -718,11 +718,12 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
 }
 
 // The predictor of zone z whose points are pp (mapsp1 of its sides); every
-// load of the zone is issued before the first use of a loaded value.
+// load of the zone is issued before the first use of a loaded value.  zre:
+// the control block's zre_valid, read once by the caller.
 template <int NS, int CL>
 __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb,
                                                   const int (&pp)[NS], double dt, int& bad,
-                                                  bool valid, ClCount& cnt) {
+                                                  bool valid, ClCount& cnt, bool zre) {
     d2 x0[NS], u0[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -733,7 +734,7 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
     st.zm = a.zm[z];
     st.zvol = a.zvol[z];
     st.zwrate = a.zwrate[z];
-    if (a.ctl->zre_valid) {
+    if (zre) {
         st.zr = a.zr[z];
         st.ze = a.ze[z];
     } else {
@@ -746,10 +747,10 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 
 template <int NS, int CL>
 __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
-                                                  int& bad, bool valid, ClCount& cnt) {
+                                                  int& bad, bool valid, ClCount& cnt, bool zre) {
     int pp[NS];
     load_pp<NS>(a, sb, pp);
-    return zone_pred_mixed<NS, CL>(a, z, sb, pp, dt, bad, valid, cnt);
+    return zone_pred_mixed<NS, CL>(a, z, sb, pp, dt, bad, valid, cnt, zre);
 }
 
 // K1, mixed precision.  With FLAG, zones with a gated corner are appended to
@@ -763,6 +764,7 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
     if (a.ctl->done) return;
     if (REDO && !a.ctl->redo) return;
     const double dt = a.ctl->dt;
+    const bool zre = a.ctl->zre_valid != 0;   // read once: the loop stores
     double dtmin = 1.e99;
     int bad = 0;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
@@ -781,7 +783,7 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
         ClCount cnt;
         if (z < zend) {
             const double zdthyd = zone_pred_mixed<NS, FLAG ? CL_FLAG : CL_EXACT>(
-                a, z, sbeg + (z - zbeg) * NS, pp, dt, bad, true, cnt);
+                a, z, sbeg + (z - zbeg) * NS, pp, dt, bad, true, cnt, zre);
             dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
         }
         if constexpr (FLAG) {
@@ -829,13 +831,14 @@ k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
     FusedArgs b = a;
     b.pack = &wp;
     const double dt = a.ctl->dt;
+    const bool zre = a.ctl->zre_valid != 0;   // read once: the loop stores
     int bad = 0;
     ClCount cnt;
     for (; w.u < w.end; w.u += w.step) {
         const int i = w.u * PN_BLOCK + threadIdx.x;
         const bool valid = i < count;
         const int z = a.zlist[zbeg + (valid ? i : 0)];
-        (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, dt, bad, valid, cnt);
+        (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, dt, bad, valid, cnt, zre);
     }
     const int nl = block_sum(cnt.learned);
     const int nf = block_sum(cnt.fallback);
@@ -854,6 +857,7 @@ __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2
 k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg, int poff) {
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
+    const bool zre = a.ctl->zre_valid != 0;   // read once: the loop stores
     double dtmin = 1.e99;
     int bad = 0;
     ClCount cnt;
@@ -862,7 +866,7 @@ k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg, int poff) {
         const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
         const bool valid = z < zend;
         const int zc = valid ? z : zend - 1;
-        const double zdthyd = zone_pred_mixed<NS, CL>(a, zc, sbeg + (zc - zbeg) * NS, dt, bad, valid, cnt);
+        const double zdthyd = zone_pred_mixed<NS, CL>(a, zc, sbeg + (zc - zbeg) * NS, dt, bad, valid, cnt, zre);
         if (valid) dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
     const double m = block_min(dtmin);
@@ -1836,12 +1840,14 @@ __device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, do
         }
     } else if constexpr (CL == CL_EXACT) {
         if (valid) {
-            const double d = zone_pred_mixed<NS, CL_EXACT>(a, z, U.sb + (z - U.zb) * NS, dt, bad, true, cnt);
+            const double d = zone_pred_mixed<NS, CL_EXACT>(a, z, U.sb + (z - U.zb) * NS, dt, bad, true, cnt,
+                                                           a.ctl->zre_valid != 0);
             dtmin = (d < dtmin) ? d : dtmin;
         }
     } else {
         const int zc = valid ? z : U.ze - 1;
-        const double d = zone_pred_mixed<NS, CL>(a, zc, U.sb + (zc - U.zb) * NS, dt, bad, valid, cnt);
+        const double d = zone_pred_mixed<NS, CL>(a, zc, U.sb + (zc - U.zb) * NS, dt, bad, valid, cnt,
+                                                 a.ctl->zre_valid != 0);
         if (valid) dtmin = (d < dtmin) ? d : dtmin;
     }
 }
