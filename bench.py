@@ -214,6 +214,10 @@ def main():
     ap.add_argument("--phase-steps", type=int, default=20,
                     help="cycles after the timed ones, with events between the kernels (direct "
                          "launches), for the per-kernel durations (0 disables)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: pennant_autotune picks each form's launch shape (grid cap, XCD "
+                         "mapping, K1 layout, pipelined cycle) after the warm-up; 0: defaults or "
+                         "the explicit options")
     ap.add_argument("--pipeline", type=int, default=None,
                     help="mixed forms: 1 pipelined cycle, 0 separate predictor and corrector "
                          "passes (the default)")
@@ -277,6 +281,14 @@ def main():
         log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
         if args.warmup > 0:
             pb.run(args.warmup, mode)
+        explicit = any(v is not None for v in (args.k1, args.max_blocks, args.xcd, args.pipeline))
+        if args.autotune and not explicit and mode != pa.MODE_TASKS:
+            # times every launch shape from a snapshot of the state and
+            # restores it, so the problem does not advance
+            ms = pb.autotune(mode, cycles=4)
+            log(f"autotune: {ms / 4 * 1e3:.1f} us/cycle at the best shape")
+        tune = {k: pb.get_tuning(k) for k in ("k1_layout", "max_blocks", "xcd", "pipeline", "graph")}
+        log(f"tuning {tune}")
         # the timed cycles run as pennant_launch_cycles runs them (16-cycle
         # graphs); the per-kernel durations come from further cycles with
         # events between the kernels
@@ -291,10 +303,10 @@ def main():
             state["closure_stats"] = pb.closure_stats()
         dims = (pb.numz, pb.nums, pb.nump)
         pb.close()
-        return total_ms, phase_ms, sc, state, dims
+        return total_ms, phase_ms, sc, state, dims, tune
 
     mode = modes[args.mode]
-    total_ms, phase_ms, sc, state, (Z, S, P) = measure(mode)
+    total_ms, phase_ms, sc, state, (Z, S, P), tune = measure(mode)
     ms_per_step = total_ms / args.steps
     value = Z * args.steps / (total_ms * 1e-3)
     log(f"{args.mode}: {args.steps} cycles in {total_ms:.3f} ms -> {ms_per_step * 1e3:.1f} us/cycle, "
@@ -304,7 +316,7 @@ def main():
     if mode in (pa.MODE_MIXED, pa.MODE_LEARNED):
         # the plain FP64 form on the same workload, same cycle count: its rate
         # is the denominator of the speedup and its state the accuracy reference
-        p_ms, _, p_sc, p_state, _ = measure(pa.MODE_FUSED)
+        p_ms, _, p_sc, p_state, _, p_tune = measure(pa.MODE_FUSED)
         plain = Z * args.steps / (p_ms * 1e-3)
         err = field_errors(state, p_state)
         log(f"plain fp64 fused: {p_ms / args.steps * 1e3:.1f} us/cycle, {plain:.4e} zone-cycles/s; "
@@ -316,9 +328,10 @@ def main():
                     "rel_l1": {k: float(f"{v[0]:.3e}") for k, v in err.items()},
                     "rel_linf": {k: float(f"{v[1]:.3e}") for k, v in err.items()},
                     "plain_fp64_value": round(plain, 1),
+                    "plain_fp64_tuning": p_tune,
                     "speedup_vs_plain_fp64": round(value / plain, 4)}
         if mode == pa.MODE_LEARNED:
-            m_ms, _, _, m_state, _ = measure(pa.MODE_MIXED)
+            m_ms, _, _, m_state, _, _ = measure(pa.MODE_MIXED)
             mixed_v = Z * args.steps / (m_ms * 1e-3)
             nl, nf = state["closure_stats"]
             log(f"mixed exact QCS: {m_ms / args.steps * 1e3:.1f} us/cycle, {mixed_v:.4e} zone-cycles/s; "
@@ -332,7 +345,7 @@ def main():
     roofline = None
     if phase_ms is not None:
         mixed = (mode in (pa.MODE_MIXED, pa.MODE_LEARNED))
-        pipelined = mixed and args.pipeline == 1
+        pipelined = mixed and tune["pipeline"] == 1
         kb = kernel_bytes(S, Z, P, mixed, pipelined)
         names = (("zone", "point", "learned_pass", "finalize") if pipelined
                  else ("zone_pred", "point", "zone_corr", "finalize"))
@@ -348,7 +361,7 @@ def main():
         b_tasks = 688 * S + 536 * Z + 512 * P
         pform = {pa.MODE_LEARNED: "learned", pa.MODE_MIXED: "mixed"}.get(mode, "fp64")
         traffic = (profiled_traffic(args.workload, dom, pform)
-                   if args.k1 in (None, 0) and not pipelined else None)
+                   if tune["k1_layout"] == 0 and not pipelined else None)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
@@ -385,7 +398,8 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": {pa.MODE_LEARNED: "bf16+f32+f64", pa.MODE_MIXED: "f32+f64"}.get(mode, "f64"),
         "data": "synthetic",
-        "config": {"workload": wname, "form": form[args.mode], "zones": Z, "sides": S, "points": P},
+        "config": {"workload": wname, "form": form[args.mode], "zones": Z, "sides": S, "points": P,
+                   "tuning": tune},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if accuracy is not None:

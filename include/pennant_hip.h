@@ -381,7 +381,7 @@ int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
 
 /* Launch-shape choices of the fused cycle; results do not depend on them.
  *   "k1_layout"  0: one zone per thread (default), 1: one corner per lane
- *   "max_blocks" grid cap of the grid-stride zone kernels, 1..4096 (2048)
+ *   "max_blocks" grid cap of the grid-stride zone kernels, 1..4096 (4096)
  *   "xcd"        1: each of the 8 XCDs takes a contiguous band of zones (default),
  *                0: plain grid-stride order
  *   "graph"      1: pennant_launch_cycles replays runs of 16 fused cycles as one

@@ -77,7 +77,9 @@ std::vector<std::pair<void*, size_t>> state_arrays(pennant_batch* b) {
     const size_t Z = (size_t)b->numz * 8, P = (size_t)b->nump * 8;
     return {{b->pxx, P}, {b->pxy, P}, {b->pux, P}, {b->puy, P}, {b->zr, Z}, {b->ze, Z},
             {b->zetot, Z}, {b->zwrate, Z}, {b->zvol, Z}, {b->zp, Z},
-            {b->ctls, sizeof(Ctl) * (size_t)b->nprob}};
+            {b->ctls, sizeof(Ctl) * (size_t)b->nprob},
+            // the closure and capture counters, so the timing cycles do not count
+            {b->cl_counters, 3 * sizeof(unsigned long long)}};
 }
 
 int snap_take(pennant_batch* b, Snap& sn) {

@@ -47,7 +47,7 @@ struct Ctl {
 // Launch-shape choices of the fused cycle (pennant_set_tuning).
 struct Tuning {
     int k1_layout = 0;       // 0 zone per thread, 1 corner per lane
-    int max_blocks = 2048;   // grid cap of the grid-stride K1/K3 launches
+    int max_blocks = 4096;   // grid cap of the grid-stride K1/K3 launches
     int xcd = 1;             // XCD-aware unit mapping of the grid-stride launches
     int graph = 1;           // batched cycles through a captured HIP graph
     int pipeline = 0;        // mixed forms: next cycle's predictor in the corrector pass

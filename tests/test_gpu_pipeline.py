@@ -156,3 +156,27 @@ def test_stops_at_tstop():
     say(f"[tstop] cycle {a[1]['cycle']:.0f} time {a[1]['time']:.6e}")
     assert a[1]["done"] == 1
     same(a, b, "tstop")
+
+
+def test_autotune_learned_keeps_state_and_counts():
+    """The autotuner (which also tries the pipelined cycle for the mixed
+    forms) restores the state and the closure counters: a tuned problem goes
+    on exactly like an untuned one."""
+    m = closure()
+    text = wd.sedov_rect(64, cstop=500)
+    with pa.Problem(text) as a, pa.Problem(text) as b:
+        for pb in (a, b):
+            pb.set_closure(m)
+            pb.run(20, pa.MODE_LEARNED)
+        before = b.closure_stats()
+        b.autotune(pa.MODE_LEARNED, cycles=3)
+        say(f"[autotune learned] max_blocks {b.get_tuning('max_blocks')} xcd {b.get_tuning('xcd')} "
+            f"pipeline {b.get_tuning('pipeline')}")
+        assert b.closure_stats() == before
+        assert b.scalars()["cycle"] == 20
+        a.run(37, pa.MODE_LEARNED)
+        b.run(37, pa.MODE_LEARNED)
+        for f in STATE:
+            assert np.array_equal(bits(a.get(f)), bits(b.get(f))), f
+        assert a.closure_stats() == b.closure_stats()
+    m.close()
