@@ -466,7 +466,11 @@ static void drop_graph(pennant_problem* pb) {
 // The pipelined mixed cycle's look-ahead predictor results no longer match
 // the state (the state, the closure, the capture or the launch shapes
 // changed, or cycles of another kind ran).
-static void drop_lookahead(pennant_problem* pb) { pb->la_form = -1; }
+static void drop_lookahead(pennant_problem* pb) {
+    // a pipelined launch leaves the look-ahead predictor's zone list behind
+    if (pb->la_form >= 0) (void)reset_zone_lists(pb);
+    pb->la_form = -1;
+}
 
 // Whether cycles of `form` run pipelined: the mixed forms, unless switched
 // off or while training tuples are captured.
@@ -674,7 +678,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     auto restore = [&]() {
         drop_lookahead(pb);
         pb->zre_stale = true;        // the restored flag decides; materialising checks it
-        int r = 0;
+        int r = reset_zone_lists(pb);
         for (size_t i = 0; i < copy.size() && !r; ++i)
             r = (int)hipMemcpyAsync(st[i].first, copy[i], st[i].second, hipMemcpyDeviceToDevice, pb->stream);
         if (!r) r = (int)hipStreamSynchronize(pb->stream);

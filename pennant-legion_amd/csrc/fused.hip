@@ -126,24 +126,42 @@ __device__ __forceinline__ int nxt(int k) { return (k + 1 == NS) ? 0 : k + 1; }
 template <int NS>
 __device__ __forceinline__ int prv(int k) { return (k == 0) ? NS - 1 : k - 1; }
 
+// The points of a zone: mapsp1 of its sides sb..sb+NS-1.
+template <int NS>
+__device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pp[k] = a.mapsp1[sb + k];
+}
+
 // ---------------------------------------------------------------------------
 // K1, zone-per-thread: the whole predictor half of doCycle for zone z, whose
 // sides are sb..sb+NS-1.  Returns the zone's calcDtNew candidate.
 template <int NS>
-__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, double dt,
-                                            int& bad) {
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
+                                            double dt, int& bad) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
+    // every load of the zone first (points pp = mapsp1 of its sides)
     d2 u0[NS], xp[NS];
     double mf[NS];
+    int slot[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const int p = a.mapsp1[sb + k];
-        const d2 x0 = mk(a.pxx[p], a.pxy[p]);
-        u0[k] = mk(a.pux[p], a.puy[p]);
-        xp[k] = add(x0, scl(dth, u0[k]));                 // advPosHalf
-        mf[k] = a.smf[sb + k];
+        xp[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
+        u0[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
     }
+    const double zm = a.zm[z];
+    const double zr0 = a.zr[z];
+    const double ze0 = a.ze[z];
+    const double zvol0 = a.zvol[z];
+    const double zwr = a.zwrate[z];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        mf[k] = a.smf[sb + k];
+        slot[k] = a.pcpos[sb + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) xp[k] = add(xp[k], scl(dth, u0[k]));   // advPosHalf
     // calcCtrs
     d2 ex[NS];
     d2 zx = mk(0., 0.);
@@ -171,23 +189,22 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, d
 #pragma unroll
     for (int k = 0; k < NS; ++k) zdl = smin(zdl, fac * sa[k] / el[k]);
     // calcRho (pred)
-    const double zm = a.zm[z];
     const double zrp = zm / zvolp;
     // calcStateHalf (reads ZR, ZE, ZWRATE of the previous cycle, ZVOL0 = ZVOL)
     const double gm1 = a.gamma - 1.;
     const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
-    const double r = a.zr[z];
-    const double e = smax(a.ze[z], 0.);
+    const double r = zr0;
+    const double e = smax(ze0, 0.);
     const double p = gm1 * r * e;
     const double pre = gm1 * e;
     const double per = gm1 * r;
     const double csqd = smax(ssmin2, pre + per * p / (r * r));
     const double zss = __builtin_sqrt(csqd);
     const double minv = 1. / zm;
-    const double dv = (zvolp - a.zvol[z]) * minv;
+    const double dv = (zvolp - zvol0) * minv;
     const double bulk = r * csqd;
     const double denom = 1. + 0.5 * per * dv;
-    const double src = a.zwrate[z] * dth * minv;
+    const double src = zwr * dth * minv;
     const double zp = p + (per * src - r * bulk * dv) / denom;
     a.zp[z] = zp;
     // calcSurfVecs, calcForcePGas, calcForceTTS
@@ -301,10 +318,10 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, d
     for (int k = 0; k < NS; ++k) {
         const int km = prv<NS>(k);
         const d2 cf = sub(tot[k], tot[km]);
-        const int slot = a.pcpos[sb + k];
-        a.cfx[slot] = cf.x;
-        a.cfy[slot] = cf.y;
-        a.cmass[slot] = mzz * 0.5 * (mf[k] + mf[km]);
+        const int sl = slot[k];
+        a.cfx[sl] = cf.x;
+        a.cfy[sl] = cf.y;
+        a.cmass[sl] = mzz * 0.5 * (mf[k] + mf[km]);
         a.sfwx[sb + k] = sfw[k].x;
         a.sfwy[sb + k] = sfw[k].y;
     }
@@ -315,6 +332,13 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, d
 }
 
 template <int NS>
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, double dt, int& bad) {
+    int pp[NS];
+    load_pp<NS>(a, sb, pp);
+    return zone_pred<NS>(a, z, sb, pp, dt, bad);
+}
+
+template <int NS>
 __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, int zend, int sbeg,
                                                         int poff) {
     if (a.ctl->done) return;
@@ -322,10 +346,20 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
     double dtmin = 1.e99;
     int bad = 0;
     const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
-    for (Units w = block_units(nunits, a.xcd); w.u < w.end; w.u += w.step) {
-        const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
+    Units w = block_units(nunits, a.xcd);
+    int zn = zbeg + w.u * PN_BLOCK + threadIdx.x;
+    int pn[NS];
+    if (w.u < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
+    for (; w.u < w.end; w.u += w.step) {
+        // this zone's points were loaded by the previous iteration; load the next zone's
+        const int z = zn;
+        int pp[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) pp[k] = pn[k];
+        zn = zbeg + (w.u + w.step) * PN_BLOCK + threadIdx.x;
+        if (w.u + w.step < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
         if (z >= zend) continue;
-        const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS, dt, bad);
+        const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS, pp, dt, bad);
         // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
@@ -384,13 +418,6 @@ __device__ __forceinline__ float flen(f2 v) { return fsqrt(fdot(v, v)); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
-
-// The points of a zone: mapsp1 of its sides sb..sb+NS-1.
-template <int NS>
-__device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) pp[k] = a.mapsp1[sb + k];
-}
 
 // The work forces of a zone's sides, {x, y} per side, as 2*NS consecutive
 // floats: the compiler merges them into multi-dword accesses (gfx950 global
@@ -1759,9 +1786,9 @@ __global__ void __launch_bounds__(64) k_zre_done(Ctl* c, int n) {
 }
 
 // Small fills (kernels rather than memsets, so captured graphs hold kernel
-// nodes only); one block of 64 threads, n at most 64.
+// nodes only); one block of 64 threads.
 __global__ void __launch_bounds__(64) k_zero_u32(unsigned* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < n; i += 64) p[i] = 0u;
 }
 __global__ void __launch_bounds__(64) k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
@@ -2277,6 +2304,13 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
         if (rc) return rc;
     }
     if (ev) (void)hipEventRecord(ev[4], s);
+    return (int)hipGetLastError();
+}
+
+int reset_zone_lists(pennant_problem* pb) {
+    const int nz = (int)pb->zruns.size() / 3;
+    if (pb->zcount && nz > 0)
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, pb->stream, pb->zcount, nz);
     return (int)hipGetLastError();
 }
 

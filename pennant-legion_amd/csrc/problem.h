@@ -220,6 +220,9 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form);
 // Stores zr = zm / zvol and ze = zetot / (zm + 1e-99) where mixed cycles left
 // them to be derived (the mixed corrector does not store them).
 int materialize_zre(pennant_problem* pb);
+// Empties the learned pass's zone lists (a pipelined launch ends with the
+// look-ahead predictor's list in them).
+int reset_zone_lists(pennant_problem* pb);
 int batch_materialize_zre(pennant_batch* b);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);
