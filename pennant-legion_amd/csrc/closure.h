@@ -34,6 +34,9 @@ namespace pn {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 #define PN_MLP_NF 10          // features
 #define PN_MLP_NH 64          // hidden units per layer
@@ -66,13 +69,28 @@ struct ClosurePack {
     float pad[10];
 };
 
-// sign(x) log(1 + |x|)
-__device__ __forceinline__ float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
+// sign(x) log(1 + |x|), from the hardware base-2 logarithm of 1 + |x| (the
+// rounding of 1 + |x| bounds the absolute error by about 1e-7, far below the
+// BF16 resolution of the network inputs)
+__device__ __forceinline__ float slog(float x) {
+    return copysignf(__builtin_amdgcn_logf(1.f + fabsf(x)) * 0.693147180559945309f, x);
+}
 
+// a and b rounded to BF16 (to nearest even), a in the low half: one
+// conversion of the pair
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
-    const __bf16 x = (__bf16)a, y = (__bf16)b;
-    return (unsigned)__builtin_bit_cast(unsigned short, x) |
-           ((unsigned)__builtin_bit_cast(unsigned short, y) << 16);
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+
+// relu(a), relu(b) as a BF16 pair: the pair is rounded first, then every
+// negative half (sign bit set, so negative as a 16-bit integer, -0 included)
+// becomes +0 by an integer max.  Rounding keeps the sign, so this equals
+// rounding relu(a), relu(b).
+__device__ __forceinline__ unsigned relu_bf16(float a, float b) {
+    const i16x2 h = __builtin_bit_cast(i16x2, pack_bf16(a, b));
+    const i16x2 z = {0, 0};
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(h, z));
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -82,7 +100,7 @@ __device__ __forceinline__ bf16x8 relu_chunk(const f32x16& a, int base) {
     u32x4 v;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
-        v[t] = pack_bf16(fmaxf(a[base + 2 * t], 0.f), fmaxf(a[base + 2 * t + 1], 0.f));
+        v[t] = relu_bf16(a[base + 2 * t], a[base + 2 * t + 1]);
     return as_bf16x8(v);
 }
 

@@ -619,20 +619,22 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
             }
         }
         if constexpr (CL == CL_LEARNED || CL == CL_RECOMP || CL == CL_CAPTURE) {
-            // network features of this corner (closure.h)
-            const float inv = 1.f / zssf;
+            // network features of this corner (closure.h); the quotients are
+            // reciprocal products (within an ulp or two: the network reads
+            // them in BF16)
+            const float inv = frcp(zssf);
             float f[PN_MLP_NF];
             f[0] = slog((ul[k].x - ul[km].x) * inv);
             f[1] = slog((ul[k].y - ul[km].y) * inv);
             f[2] = slog((ul[kp].x - ul[k].x) * inv);
             f[3] = slog((ul[kp].y - ul[k].y) * inv);
             f[4] = slog(div * evol * inv);
-            f[5] = evol / minelen;
-            f[6] = zssf / (zssf + flen(duav));
-            f[7] = zrpf / (float)r;
+            f[5] = evol * frcp(minelen);
+            f[6] = zssf * frcp(zssf + flen(duav));
+            f[7] = zrpf * frcp((float)r);
             f[8] = slog(du * inv);
             const float tl = q1f * zssf, tq = gq * du;
-            f[9] = (tl + tq > 0.f) ? tq / (tl + tq) : 0.f;
+            f[9] = (tl + tq > 0.f) ? tq * frcp(tl + tq) : 0.f;
             if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
                 if (gated & (1u << k)) {
 #pragma unroll
