@@ -155,15 +155,17 @@ pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbu
         return p;
     };
     const size_t Z = b->numz, S = b->nums, P = b->nump;
+    if (!corner_slots_fit((int64_t)P, (int64_t)S)) return fail(b, "batch too large for 32-bit corner slots");
+    const size_t NC = (size_t)corner_slots((int64_t)P, (int64_t)S);
     b->pxx = (double*)al(P, 8); b->pxy = (double*)al(P, 8); b->pux = (double*)al(P, 8); b->puy = (double*)al(P, 8);
     b->zr = (double*)al(Z, 8); b->ze = (double*)al(Z, 8); b->zetot = (double*)al(Z, 8);
     b->zwrate = (double*)al(Z, 8); b->zvol = (double*)al(Z, 8); b->zp = (double*)al(Z, 8);
     b->zm = (double*)al(Z, 8); b->smf = (double*)al(S, 8); b->fsmf = (float*)al(S, 4);
     b->mapsp1 = (int*)al(S, 4); b->pcbeg = (int*)al(P + 1, 4); b->pcpos = (int*)al(S, 4);
     b->pbc = (unsigned char*)al(P, 1);
-    b->cfx = (double*)al(S, 8); b->cfy = (double*)al(S, 8); b->cmass = (double*)al(S, 8);
+    b->cfx = (double*)al(NC, 8); b->cfy = (double*)al(NC, 8); b->cmass = (double*)al(NC, 8);
     b->sfwx = (double*)al(S, 8); b->sfwy = (double*)al(S, 8);
-    b->fcrn = (float*)al(3 * S, 4); b->fsfw = (float*)al(2 * S, 4);
+    b->fcrn = (float*)al(3 * NC, 4); b->fsfw = (float*)al(2 * S, 4);
     b->vsx = (double*)al(P, 8); b->vsy = (double*)al(P, 8); b->pxpx = (double*)al(P, 8);
     b->ctls = (Ctl*)al((size_t)n, sizeof(Ctl));
     b->probs = (BProb*)al((size_t)n, sizeof(BProb));
@@ -198,7 +200,11 @@ pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbu
         if (rc) { pennant_destroy(pb); return fail(b, "arena copy failed"); }
         for (size_t s = 0; s < ns; ++s) mapsp1[so + s] = (int)(po + h.mapsp1[s]);
         for (size_t p = 0; p < np; ++p) pcbeg[po + p] = (int)(so + h.pcbeg[p]);
-        for (size_t k = 0; k < ns; ++k) pcpos[so + h.pcside[k]] = (int)(so + k);
+        // corner slots of the arena (its points numbered po + p)
+        for (size_t p = 0; p < np; ++p)
+            for (int k = h.pcbeg[p]; k < h.pcbeg[p + 1]; ++k)
+                pcpos[so + h.pcside[k]] =
+                    (int)corner_slot((int64_t)P, po + (int64_t)p, k - h.pcbeg[p], so + k);
         for (size_t p = 0; p < np; ++p)
             if (h.pcbeg[p + 1] - h.pcbeg[p] > PN_HEAVY) heavy.push_back({(int)(po + p), i});
         BProb& q = probs[i];

@@ -59,7 +59,7 @@ struct FusedArgs {
     // mesh
     const int32_t* mapsp1;
     const int32_t* pcbeg;
-    const int32_t* pcpos;      // side -> slot of its corner in the point-corner CSR
+    const int32_t* pcpos;      // side -> slot of its corner record (corner_slot)
     const double* smf;
     int nump;
     // state (in place)
@@ -70,7 +70,7 @@ struct FusedArgs {
     // per-side intermediates
     double* cfx; double* cfy; double* cmass; double* sfwx; double* sfwy;
     // the mixed-precision form's FP32 counterparts, and smf in FP32
-    float* fcrn;               // per corner slot {fx, fy, mass} (point-corner CSR order)
+    float* fcrn;               // per corner slot {fx, fy, mass} (corner_slot order)
     float* fsfw;               // per side {x, y}
     const float* fsmf;
     // per-point intermediates
@@ -1179,8 +1179,8 @@ __device__ __forceinline__ void point_update(const FusedArgs& a, int p, const Po
     a.puy[p] = u.y;
 }
 
-// Corner values in point-corner CSR order: FP64 arrays for the plain form,
-// FP32 arrays for the mixed-precision form (summed in FP64 either way).
+// Corner values by corner slot: FP64 arrays for the plain form, FP32 records
+// for the mixed-precision form (summed in FP64 either way).
 template <typename T> struct Corners;
 template <> struct Corners<double> {
     __device__ static double mass(const FusedArgs& a, int k) { return a.cmass[k]; }
@@ -1202,6 +1202,12 @@ template <> struct Corners<float> {
 // A high-valence point (the pie origin has one corner per centre triangle)
 // takes a whole block: the block stages the corner values in LDS and one
 // thread adds them in ascending order.  Called by every thread of the block.
+// The slot of position c (CSR index k) of point p's corner list (the host's
+// corner_slot, with a.nump the mesh's or the batch arena's point count).
+__device__ __forceinline__ int corner_slot(const FusedArgs& a, int p, int c, int k) {
+    return (c < 4) ? c * a.nump + p : 4 * a.nump + k;
+}
+
 template <typename T>
 __device__ __forceinline__ void point_heavy(const FusedArgs& a, int p, double dt) {
     using C = Corners<T>;
@@ -1212,9 +1218,10 @@ __device__ __forceinline__ void point_heavy(const FusedArgs& a, int p, double dt
     for (int c0 = kb; c0 < ke; c0 += PN_BLOCK) {
         const int k = c0 + threadIdx.x;
         if (k < ke) {
-            sm[threadIdx.x] = C::mass(a, k);
-            sx[threadIdx.x] = C::fx(a, k);
-            sy[threadIdx.x] = C::fy(a, k);
+            const int sl = corner_slot(a, p, k - kb, k);
+            sm[threadIdx.x] = C::mass(a, sl);
+            sx[threadIdx.x] = C::fx(a, sl);
+            sy[threadIdx.x] = C::fy(a, sl);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1238,7 +1245,8 @@ template <typename T>
 __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt) {
     using C = Corners<T>;
     const int kb = a.pcbeg[p], ke = a.pcbeg[p + 1];
-    if (ke - kb > PN_HEAVY) return;
+    const int n = ke - kb;
+    if (n > PN_HEAVY) return;
     const PointIn q = load_point(a, p);
     double m4[4];
     d2 f4[4];
@@ -1246,9 +1254,9 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
     for (int j = 0; j < 4; ++j) {
         // every point has a corner: clamped to the last one, the load is in
         // range and unconditional (no branch between the loads)
-        const int k = (kb + j < ke) ? kb + j : ke - 1;
-        m4[j] = C::mass(a, k);
-        f4[j] = mk(C::fx(a, k), C::fy(a, k));
+        const int sl = ((j < n) ? j : n - 1) * a.nump + p;
+        m4[j] = C::mass(a, sl);
+        f4[j] = mk(C::fx(a, sl), C::fy(a, sl));
     }
     double mass = 0.;
     d2 f = mk(0., 0.);
@@ -1260,8 +1268,9 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
         }
     }
     for (int k = kb + 4; k < ke; ++k) {
-        mass += C::mass(a, k);
-        f = add(f, mk(C::fx(a, k), C::fy(a, k)));
+        const int sl = 4 * a.nump + k;
+        mass += C::mass(a, sl);
+        f = add(f, mk(C::fx(a, sl), C::fy(a, sl)));
     }
     point_update<T>(a, p, q, mass, f, dt);
 }
@@ -2053,14 +2062,16 @@ int fused_setup(pennant_problem* pb) {
         return p;
     };
     const size_t S = h.nums, P = h.nump;
-    pb->cfx = (double*)al(S, 8); pb->cfy = (double*)al(S, 8); pb->cmass = (double*)al(S, 8);
+    if (!corner_slots_fit((int64_t)P, (int64_t)S)) return PENNANT_EINVAL;
+    const size_t NC = (size_t)corner_slots((int64_t)P, (int64_t)S);
+    pb->cfx = (double*)al(NC, 8); pb->cfy = (double*)al(NC, 8); pb->cmass = (double*)al(NC, 8);
     pb->sfwx = (double*)al(S, 8); pb->sfwy = (double*)al(S, 8);
     pb->vsx = (double*)al(P, 8); pb->vsy = (double*)al(P, 8); pb->pxpx = (double*)al(P, 8);
     pb->pbc = (unsigned char*)al(P, 1);
     pb->heavy = (int*)al(heavy.size(), 4);
     pb->nheavy = (int)heavy.size();
     // mixed-precision intermediates and the FP32 copy of smf
-    pb->fcrn = (float*)al(3 * S, 4);
+    pb->fcrn = (float*)al(3 * NC, 4);
     pb->fsfw = (float*)al(2 * S, 4);
     float* fsmf = (float*)al(S, 4);
     pb->fsmf = fsmf;
@@ -2069,10 +2080,12 @@ int fused_setup(pennant_problem* pb) {
     if (S > 0)
         hipLaunchKernelGGL(k_to_float, dim3(nblocks((int64_t)S)), dim3(PN_BLOCK), 0, pb->stream,
                            (int)S, (const double*)pb->f("smf"), fsmf);
-    // K1 writes each corner's force and mass straight into its point's CSR
-    // slot, so that K2 reads every point's corners as one contiguous run
+    // K1 writes each corner's force and mass straight into its corner slot,
+    // where K2 reads it
     std::vector<int> pcpos((size_t)h.nums);
-    for (int k = 0; k < h.nums; ++k) pcpos[h.pcside[k]] = k;
+    for (int p = 0; p < h.nump; ++p)
+        for (int k = h.pcbeg[p]; k < h.pcbeg[p + 1]; ++k)
+            pcpos[h.pcside[k]] = (int)corner_slot((int64_t)P, p, k - h.pcbeg[p], k);
     pb->pcpos = (int*)al(S, 4);
     if (pb->pcpos &&
         hipMemcpy(pb->pcpos, pcpos.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess)

@@ -42,6 +42,22 @@ struct Ctl {
 };
 
 #define PN_MAX_GRID 4096   // largest grid of a fused K1/K3 launch
+
+// Slot of a corner record (K1 writes it, K2 reads it): position c of point
+// p's corner list (ascending side order; k = pcbeg[p] + c is its point-corner
+// CSR index) in a mesh of nump points.  The first four positions are
+// corner-major, c * nump + p: neighbouring zones write the records of
+// neighbouring points at one position, and neighbouring points read them,
+// so both sides touch whole cache lines.  Later positions (points with more
+// than four corners) take 4 * nump + k.  A mesh needs corner_slots() slots.
+inline int64_t corner_slot(int64_t nump, int64_t p, int64_t c, int64_t k) {
+    return c < 4 ? c * nump + p : 4 * nump + k;
+}
+inline int64_t corner_slots(int64_t nump, int64_t nums) { return 4 * nump + nums; }
+// The kernels index FP32 corner records {fx, fy, mass} as 3 * slot + j in int.
+inline bool corner_slots_fit(int64_t nump, int64_t nums) {
+    return 3 * corner_slots(nump, nums) < (int64_t)0x7fffffff;
+}
 #define PN_HEAVY 64        // points with more corners get a whole block in K2
 
 // Launch-shape choices of the fused cycle (pennant_set_tuning).
