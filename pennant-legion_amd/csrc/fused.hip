@@ -134,11 +134,40 @@ __device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS
 }
 
 // ---------------------------------------------------------------------------
-// K1, zone-per-thread: the whole predictor half of doCycle for zone z, whose
-// sides are sb..sb+NS-1.  Returns the zone's calcDtNew candidate.
+// Side work forces (K1 writes them, K3 reads them) are stored pair-major
+// within each unit of n <= PN_BLOCK zones of a run (sides u0 .. u0 + NS n - 1):
+// sides 2j and 2j+1 of the unit's zone i sit together at u0 + 2 j n + 2 i
+// (a lone last side of an odd NS at u0 + 2 j n + i), so that a wave's lanes
+// (consecutive zones) cover one contiguous run per side pair and each lane
+// moves a pair in one access.  SideW locates a zone in its unit.
+struct SideW {
+    int u0, i, n;
+};
 template <int NS>
-__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
-                                            double dt, int& bad) {
+__device__ __forceinline__ int sw_slot(const SideW& w, int k) {
+    const int j = k >> 1;
+    return w.u0 + 2 * j * w.n + (((k | 1) < NS) ? 2 * w.i + (k & 1) : w.i);
+}
+// A side pair's two values as one access (the types carry only the element
+// alignment: gfx950 global memory takes dword-aligned multi-dword accesses).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+
+__device__ __forceinline__ SideW side_w(int z, int zbeg, int zend, int sbeg, int ns) {
+    const int zu = zbeg + ((z - zbeg) / PN_BLOCK) * PN_BLOCK;
+    SideW w;
+    w.u0 = sbeg + (zu - zbeg) * ns;
+    w.i = z - zu;
+    w.n = (zend - zu < PN_BLOCK) ? zend - zu : PN_BLOCK;
+    return w;
+}
+
+// K1, zone-per-thread: the whole predictor half of doCycle for zone z, whose
+// sides are sb..sb+NS-1 (side work force slots sw).  Returns the zone's
+// calcDtNew candidate.
+template <int NS>
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, SideW sw,
+                                            const int (&pp)[NS], double dt, int& bad) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
     // every load of the zone first (points pp = mapsp1 of its sides)
@@ -322,8 +351,18 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, c
         a.cfx[sl] = cf.x;
         a.cfy[sl] = cf.y;
         a.cmass[sl] = mzz * 0.5 * (mf[k] + mf[km]);
-        a.sfwx[sb + k] = sfw[k].x;
-        a.sfwy[sb + k] = sfw[k].y;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; k += 2) {
+        const int sl = sw_slot<NS>(sw, k);
+        if (k + 1 < NS) {
+            const f64x2u vx = {sfw[k].x, sfw[k + 1].x}, vy = {sfw[k].y, sfw[k + 1].y};
+            *reinterpret_cast<f64x2u*>(a.sfwx + sl) = vx;
+            *reinterpret_cast<f64x2u*>(a.sfwy + sl) = vy;
+        } else {
+            a.sfwx[sl] = sfw[k].x;
+            a.sfwy[sl] = sfw[k].y;
+        }
     }
     // calcDtNew for this zone
     const double fuzz = 1.e-99;
@@ -332,10 +371,11 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, c
 }
 
 template <int NS>
-__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, double dt, int& bad) {
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, SideW sw, double dt,
+                                            int& bad) {
     int pp[NS];
     load_pp<NS>(a, sb, pp);
-    return zone_pred<NS>(a, z, sb, pp, dt, bad);
+    return zone_pred<NS>(a, z, sb, sw, pp, dt, bad);
 }
 
 template <int NS>
@@ -359,7 +399,8 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
         zn = zbeg + (w.u + w.step) * PN_BLOCK + threadIdx.x;
         if (w.u + w.step < w.end && zn < zend) load_pp<NS>(a, sbeg + (zn - zbeg) * NS, pn);
         if (z >= zend) continue;
-        const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS, pp, dt, bad);
+        const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS,
+                                            side_w(z, zbeg, zend, sbeg, NS), pp, dt, bad);
         // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
@@ -419,23 +460,34 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
 
-// The work forces of a zone's sides, {x, y} per side, as 2*NS consecutive
-// floats: the compiler merges them into multi-dword accesses (gfx950 global
-// memory takes dword-aligned multi-dword loads and stores).
+// The work forces of a zone's sides, {x, y} per side slot (sw_slot).
 template <int NS>
-__device__ __forceinline__ void store_sides(float* w, int sb, const f2 (&v)[NS]) {
-    float* q = w + 2 * sb;
+__device__ __forceinline__ void store_sides(float* w, const SideW& sw, const f2 (&v)[NS]) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        q[2 * k] = v[k].x;
-        q[2 * k + 1] = v[k].y;
+    for (int k = 0; k < NS; k += 2) {
+        float* q = w + 2 * sw_slot<NS>(sw, k);
+        if (k + 1 < NS) {
+            const f32x4u t = {v[k].x, v[k].y, v[k + 1].x, v[k + 1].y};
+            *reinterpret_cast<f32x4u*>(q) = t;
+        } else {
+            q[0] = v[k].x;
+            q[1] = v[k].y;
+        }
     }
 }
 template <int NS>
-__device__ __forceinline__ void load_sides(const float* w, int sb, f2 (&v)[NS]) {
-    const float* q = w + 2 * sb;
+__device__ __forceinline__ void load_sides(const float* w, const SideW& sw, f2 (&v)[NS]) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) v[k] = fmk(q[2 * k], q[2 * k + 1]);
+    for (int k = 0; k < NS; k += 2) {
+        const float* q = w + 2 * sw_slot<NS>(sw, k);
+        if (k + 1 < NS) {
+            const f32x4u t = *reinterpret_cast<const f32x4u*>(q);
+            v[k] = fmk(t[0], t[1]);
+            v[k + 1] = fmk(t[2], t[3]);
+        } else {
+            v[k] = fmk(q[0], q[1]);
+        }
+    }
 }
 
 // Closure kinds of the mixed K1:
@@ -473,7 +525,7 @@ struct ZoneState {
 // runs across the wave); `valid` false marks a lane that only takes part in
 // the wave-wide steps and stores nothing.  Writes zp to a.zp.
 template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z, int sb,
+__device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z, int sb, SideW sw,
                                                        const d2 (&x0)[NS], const d2 (&u0)[NS],
                                                        const ZoneState& st, double dt, int& bad,
                                                        bool valid, ClCount& cnt) {
@@ -728,7 +780,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         const float dux = fabsf(fdot(du, dx)) * frcp(el[k]);
         dux_max = fmaxr(dux_max, (el[k] > 0.f) ? dux : 0.f);
     }
-    if (valid) store_sides<NS>(a.fsfw, sb, sfwk);
+    if (valid) store_sides<NS>(a.fsfw, sw, sfwk);
     const double zdu = a.q1 * zss + 2. * a.q2 * (double)dux_max;
     const double mzz = zrp * zarea;
     if (valid) {
@@ -750,7 +802,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
 // load of the zone is issued before the first use of a loaded value.  zre:
 // the control block's zre_valid, read once by the caller.
 template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb,
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, SideW sw,
                                                   const int (&pp)[NS], double dt, int& bad,
                                                   bool valid, ClCount& cnt, bool zre) {
     d2 x0[NS], u0[NS];
@@ -771,15 +823,16 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         st.zr = st.zm / st.zvol;
         st.ze = a.zetot[z] / (st.zm + 1.e-99);
     }
-    return zone_pred_mixed_core<NS, CL>(a, z, sb, x0, u0, st, dt, bad, valid, cnt);
+    return zone_pred_mixed_core<NS, CL>(a, z, sb, sw, x0, u0, st, dt, bad, valid, cnt);
 }
 
 template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, double dt,
-                                                  int& bad, bool valid, ClCount& cnt, bool zre) {
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, SideW sw,
+                                                  double dt, int& bad, bool valid, ClCount& cnt,
+                                                  bool zre) {
     int pp[NS];
     load_pp<NS>(a, sb, pp);
-    return zone_pred_mixed<NS, CL>(a, z, sb, pp, dt, bad, valid, cnt, zre);
+    return zone_pred_mixed<NS, CL>(a, z, sb, sw, pp, dt, bad, valid, cnt, zre);
 }
 
 // K1, mixed precision.  With FLAG, zones with a gated corner are appended to
@@ -812,7 +865,8 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
         ClCount cnt;
         if (z < zend) {
             const double zdthyd = zone_pred_mixed<NS, FLAG ? CL_FLAG : CL_EXACT>(
-                a, z, sbeg + (z - zbeg) * NS, pp, dt, bad, true, cnt, zre);
+                a, z, sbeg + (z - zbeg) * NS, side_w(z, zbeg, zend, sbeg, NS), pp, dt, bad, true, cnt,
+                zre);
             dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
         }
         if constexpr (FLAG) {
@@ -844,7 +898,7 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
 // store nothing.
 template <int NS>
 __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
-k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
+k_pred_list(FusedArgs a, int zbeg, int zend, int sbeg, int run) {
     if (a.ctl->done) return;
     const int count = (int)a.zcount[run];
     const int nunits = (count + PN_BLOCK - 1) / PN_BLOCK;
@@ -867,7 +921,8 @@ k_pred_list(FusedArgs a, int zbeg, int sbeg, int run) {
         const int i = w.u * PN_BLOCK + threadIdx.x;
         const bool valid = i < count;
         const int z = a.zlist[zbeg + (valid ? i : 0)];
-        (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, dt, bad, valid, cnt, zre);
+        (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, side_w(z, zbeg, zend, sbeg, NS),
+                                             dt, bad, valid, cnt, zre);
     }
     const int nl = block_sum(cnt.learned);
     const int nf = block_sum(cnt.fallback);
@@ -895,7 +950,9 @@ k_pred_cl(FusedArgs a, int zbeg, int zend, int sbeg, int poff) {
         const int z = zbeg + w.u * PN_BLOCK + threadIdx.x;
         const bool valid = z < zend;
         const int zc = valid ? z : zend - 1;
-        const double zdthyd = zone_pred_mixed<NS, CL>(a, zc, sbeg + (zc - zbeg) * NS, dt, bad, valid, cnt, zre);
+        const double zdthyd = zone_pred_mixed<NS, CL>(a, zc, sbeg + (zc - zbeg) * NS,
+                                                      side_w(zc, zbeg, zend, sbeg, NS), dt, bad, valid,
+                                                      cnt, zre);
         if (valid) dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
     const double m = block_min(dtmin);
@@ -951,7 +1008,7 @@ struct Grp {
 // calcDtNew candidate on the group's corner 0 and 1e99 elsewhere.
 template <int NS>
 __device__ __forceinline__ double corner_pred(const FusedArgs& a, const Grp<NS>& g, int z, int s,
-                                              bool live, double dt, int& bad) {
+                                              int sws, bool live, double dt, int& bad) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
     // advPosHalf at this corner's point
@@ -1082,8 +1139,8 @@ __device__ __forceinline__ double corner_pred(const FusedArgs& a, const Grp<NS>&
         a.cfy[slot] = cf.y;
         a.cmass[slot] = cmass;
         const d2 sfw = add(sfp, sfq);
-        a.sfwx[s] = sfw.x;
-        a.sfwy[s] = sfw.y;
+        a.sfwx[sws] = sfw.x;
+        a.sfwy[sws] = sfw.y;
         if (g.k == 0) a.zp[z] = zp;
     }
     const double fuzz = 1.e-99;
@@ -1112,7 +1169,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_corner(FusedArgs a, int zbeg,
         const int zw = zbeg + w.u * ZPB + wave * ZPW + gi;
         const bool live = (gi < ZPW) && (zw < zend);
         const int z = live ? zw : zbeg;               // idle lanes read a valid zone
-        const double zdthyd = corner_pred<NS>(a, g, z, sbeg + (z - zbeg) * NS + g.k, live, dt, bad);
+        const SideW sw = side_w(z, zbeg, zend, sbeg, NS);
+        const double zdthyd = corner_pred<NS>(a, g, z, sbeg + (z - zbeg) * NS + g.k,
+                                              sw_slot<NS>(sw, g.k), live, dt, bad);
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
     const double m = block_min(dtmin);
@@ -1299,7 +1358,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
 // new state st (the next cycle's predictor starts from them).  With PIPE the
 // cycle's predictor pressure comes from a.zpn and is stored to a.zp.
 template <int NS, typename T, bool PIPE>
-__device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb,
+__device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb, SideW sw,
                                                const int (&pp)[NS], double dt, int& bad,
                                                double& dvov, d2 (&x)[NS], ZoneState& st) {
     using C = Corners<T>;
@@ -1310,12 +1369,22 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
     d2 sfw[NS];
     if constexpr (sizeof(T) == 4) {
         f2 v[NS];
-        load_sides<NS>(a.fsfw, sb, v);
+        load_sides<NS>(a.fsfw, sw, v);
 #pragma unroll
         for (int k = 0; k < NS; ++k) sfw[k] = mk((double)v[k].x, (double)v[k].y);
     } else {
 #pragma unroll
-        for (int k = 0; k < NS; ++k) sfw[k] = mk(C::wx(a, sb + k), C::wy(a, sb + k));
+        for (int k = 0; k < NS; k += 2) {
+            const int sl = sw_slot<NS>(sw, k);
+            if (k + 1 < NS) {
+                const f64x2u vx = *reinterpret_cast<const f64x2u*>(a.sfwx + sl);
+                const f64x2u vy = *reinterpret_cast<const f64x2u*>(a.sfwy + sl);
+                sfw[k] = mk(vx[0], vy[0]);
+                sfw[k + 1] = mk(vx[1], vy[1]);
+            } else {
+                sfw[k] = mk(a.sfwx[sl], a.sfwy[sl]);
+            }
+        }
     }
     d2 vs[NS];                 // mixed form: (pu0 + pu) pxp.x
     double xp[NS];
@@ -1378,19 +1447,19 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
 }
 
 template <int NS, typename T>
-__device__ __forceinline__ void corr_zone_pp(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
-                                             double dt, int& bad, double& dvov) {
+__device__ __forceinline__ void corr_zone_pp(const FusedArgs& a, int z, int sb, SideW sw,
+                                             const int (&pp)[NS], double dt, int& bad, double& dvov) {
     d2 x[NS];
     ZoneState st;
-    corr_zone_core<NS, T, false>(a, z, sb, pp, dt, bad, dvov, x, st);
+    corr_zone_core<NS, T, false>(a, z, sb, sw, pp, dt, bad, dvov, x, st);
 }
 
 template <int NS, typename T>
-__device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, double dt, int& bad,
-                                          double& dvov) {
+__device__ __forceinline__ void corr_zone(const FusedArgs& a, int z, int sb, SideW sw, double dt,
+                                          int& bad, double& dvov) {
     int pp[NS];
     load_pp<NS>(a, sb, pp);
-    corr_zone_pp<NS, T>(a, z, sb, pp, dt, bad, dvov);
+    corr_zone_pp<NS, T>(a, z, sb, sw, pp, dt, bad, dvov);
 }
 
 // The point indices of a thread's next zone are loaded while the current
@@ -1414,7 +1483,9 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
         for (int k = 0; k < NS; ++k) pc[k] = pp[k];
         z = zbeg + (wu.u + wu.step) * PN_BLOCK + threadIdx.x;
         if (wu.u + wu.step < wu.end && z < zend) load_pp<NS>(a, sbeg + (z - zbeg) * NS, pp);
-        if (zc < zend) corr_zone_pp<NS, T>(a, zc, sbeg + (zc - zbeg) * NS, pc, dt, bad, dvov);
+        if (zc < zend)
+            corr_zone_pp<NS, T>(a, zc, sbeg + (zc - zbeg) * NS, side_w(zc, zbeg, zend, sbeg, NS), pc, dt,
+                                bad, dvov);
     }
     const double m = block_max(dvov);
     const int b = block_sum(bad);
@@ -1510,6 +1581,7 @@ k_zone_pipe(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run, DtPara
         ClCount cnt;
         if (z < zend) {
             const int sb = sbeg + (z - zbeg) * NS;
+            const SideW sw = side_w(z, zbeg, zend, sbeg, NS);
             d2 u[NS];
             if (!last) {
 #pragma unroll
@@ -1517,10 +1589,10 @@ k_zone_pipe(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run, DtPara
             }
             d2 x[NS];
             ZoneState st;
-            corr_zone_core<NS, float, true>(a, z, sb, pp, dt, badc, dvov, x, st);
+            corr_zone_core<NS, float, true>(a, z, sb, sw, pp, dt, badc, dvov, x, st);
             if (!last) {
                 const double zdthyd = zone_pred_mixed_core<NS, FLAG ? CL_FLAG : CL_EXACT>(
-                    b, z, sb, x, u, st, dtn, badp, true, cnt);
+                    b, z, sb, sw, x, u, st, dtn, badp, true, cnt);
                 dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
             }
         }
@@ -1866,6 +1938,16 @@ __device__ __forceinline__ void apply_prob(FusedArgs& a, const BProb& q) {
     a.bcv = q.bcvx;            // q.bcvy follows q.bcvx in memory
 }
 
+// A batch unit's side work force slots: the unit's zones are a PN_BLOCK
+// chunk of one side-count run, as side_w takes them.
+__device__ __forceinline__ SideW unit_w(const BUnit& U, int z) {
+    SideW w;
+    w.u0 = U.sb;
+    w.i = z - U.zb;
+    w.n = U.ze - U.zb;
+    return w;
+}
+
 template <int NS, int FORM, int CL>
 __device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, double dt,
                                           double& dtmin, int& bad, ClCount& cnt) {
@@ -1873,19 +1955,19 @@ __device__ __forceinline__ void pred_unit(const FusedArgs& a, const BUnit& U, do
     const bool valid = z < U.ze;
     if constexpr (FORM == FORM_FP64) {
         if (valid) {
-            const double d = zone_pred<NS>(a, z, U.sb + (z - U.zb) * NS, dt, bad);
+            const double d = zone_pred<NS>(a, z, U.sb + (z - U.zb) * NS, unit_w(U, z), dt, bad);
             dtmin = (d < dtmin) ? d : dtmin;
         }
     } else if constexpr (CL == CL_EXACT) {
         if (valid) {
-            const double d = zone_pred_mixed<NS, CL_EXACT>(a, z, U.sb + (z - U.zb) * NS, dt, bad, true, cnt,
-                                                           a.ctl->zre_valid != 0);
+            const double d = zone_pred_mixed<NS, CL_EXACT>(a, z, U.sb + (z - U.zb) * NS, unit_w(U, z), dt,
+                                                           bad, true, cnt, a.ctl->zre_valid != 0);
             dtmin = (d < dtmin) ? d : dtmin;
         }
     } else {
         const int zc = valid ? z : U.ze - 1;
-        const double d = zone_pred_mixed<NS, CL>(a, zc, U.sb + (zc - U.zb) * NS, dt, bad, valid, cnt,
-                                                 a.ctl->zre_valid != 0);
+        const double d = zone_pred_mixed<NS, CL>(a, zc, U.sb + (zc - U.zb) * NS, unit_w(U, zc), dt, bad,
+                                                 valid, cnt, a.ctl->zre_valid != 0);
         if (valid) dtmin = (d < dtmin) ? d : dtmin;
     }
 }
@@ -1955,8 +2037,8 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr_batch(BatchArgs B) {
         int bad = 0;
         const int z = U.zb + (int)threadIdx.x;
         if (z < U.ze) {
-            if (U.ns == 4) corr_zone<4, T>(B.a, z, U.sb + (z - U.zb) * 4, dt, bad, dvov);
-            else corr_zone<3, T>(B.a, z, U.sb + (z - U.zb) * 3, dt, bad, dvov);
+            if (U.ns == 4) corr_zone<4, T>(B.a, z, U.sb + (z - U.zb) * 4, unit_w(U, z), dt, bad, dvov);
+            else corr_zone<3, T>(B.a, z, U.sb + (z - U.zb) * 3, unit_w(U, z), dt, bad, dvov);
         }
         const double m = block_max(dvov);
         const int nb = block_sum(bad);
@@ -2150,12 +2232,12 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
             const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
             const int run = (int)(r / 3);
             switch (ns) {
-                case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, a, zb, sb, run); break;
-                case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, a, zb, sb, run); break;
-                case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, a, zb, sb, run); break;
-                case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, a, zb, sb, run); break;
-                case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, a, zb, sb, run); break;
-                case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, a, zb, sb, run); break;
+                case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, a, zb, ze, sb, run); break;
+                case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, a, zb, ze, sb, run); break;
+                case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, a, zb, ze, sb, run); break;
+                case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, a, zb, ze, sb, run); break;
+                case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, a, zb, ze, sb, run); break;
+                case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, a, zb, ze, sb, run); break;
                 default: return PENNANT_EINVAL;
             }
         }
@@ -2246,12 +2328,12 @@ int launch_learned_pass(const pennant_problem* pb, const FusedArgs& a, bool pend
         const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
         const int run = (int)(r / 3);
         switch (ns) {
-            case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, b, zb, sb, run); break;
-            case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, b, zb, sb, run); break;
-            case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, b, zb, sb, run); break;
-            case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, b, zb, sb, run); break;
-            case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, b, zb, sb, run); break;
-            case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, b, zb, sb, run); break;
+            case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, b, zb, ze, sb, run); break;
             default: return PENNANT_EINVAL;
         }
     }
