@@ -166,7 +166,7 @@ pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbu
     b->cfx = (double*)al(NC, 8); b->cfy = (double*)al(NC, 8); b->cmass = (double*)al(NC, 8);
     b->sfwx = (double*)al(S, 8); b->sfwy = (double*)al(S, 8);
     b->fcrn = (float*)al(3 * NC, 4); b->fsfw = (float*)al(2 * S, 4);
-    b->vsx = (double*)al(P, 8); b->vsy = (double*)al(P, 8); b->pxpx = (double*)al(P, 8);
+    b->vs2 = (double*)al(2 * P, 8); b->pxpx = (double*)al(P, 8);
     b->ctls = (Ctl*)al((size_t)n, sizeof(Ctl));
     b->probs = (BProb*)al((size_t)n, sizeof(BProb));
     b->cl_counters = (unsigned long long*)al(3, 8);

@@ -74,7 +74,7 @@ struct FusedArgs {
     float* fsfw;               // per side {x, y}
     const float* fsmf;
     // per-point intermediates
-    double* vsx; double* vsy; double* pxpx;
+    double* vs2; double* pxpx;   // vs2: per point {x, y} (see point_update)
     const unsigned char* pbc;
     const int* heavy;
     int nheavy;
@@ -1200,7 +1200,8 @@ __device__ __forceinline__ PointIn load_point(const FusedArgs& a, int p) {
 }
 
 // T = float (the mixed form): calcWork needs (pu0 + pu) only times pxp.x, so
-// that product is stored in vsx/vsy and pxpx is not written.
+// that product is stored in vs2 and pxpx is not written.  vs2 holds a point's
+// pair {x, y} together (K3 gathers it as one 16-byte access).
 template <typename T>
 __device__ __forceinline__ void point_update(const FusedArgs& a, int p, const PointIn& q,
                                              double mass, d2 f, double dt) {
@@ -1225,12 +1226,12 @@ __device__ __forceinline__ void point_update(const FusedArgs& a, int p, const Po
     const double xpx = x0.x + dth * uin.x;   // pxp.x for calcWork
     const d2 vs = add(u0, u);                // pu0 + pu for calcWork
     if constexpr (sizeof(T) == 4) {
-        a.vsx[p] = vs.x * xpx;
-        a.vsy[p] = vs.y * xpx;
+        const f64x2u v = {vs.x * xpx, vs.y * xpx};
+        *reinterpret_cast<f64x2u*>(a.vs2 + 2 * (int64_t)p) = v;
     } else {
         a.pxpx[p] = xpx;
-        a.vsx[p] = vs.x;
-        a.vsy[p] = vs.y;
+        const f64x2u v = {vs.x, vs.y};
+        *reinterpret_cast<f64x2u*>(a.vs2 + 2 * (int64_t)p) = v;
     }
     a.pxx[p] = x.x;
     a.pxy[p] = x.y;
@@ -1391,7 +1392,8 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
-        vs[k] = mk(a.vsx[pp[k]], a.vsy[pp[k]]);
+        const f64x2u v = *reinterpret_cast<const f64x2u*>(a.vs2 + 2 * (int64_t)pp[k]);
+        vs[k] = mk(v[0], v[1]);
         if constexpr (sizeof(T) == 8) xp[k] = a.pxpx[pp[k]];
     }
     const double dth = 0.5 * dt;
@@ -1827,7 +1829,7 @@ FusedArgs make_args(pennant_problem* pb) {
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
     a.cfx = pb->cfx; a.cfy = pb->cfy; a.cmass = pb->cmass; a.sfwx = pb->sfwx; a.sfwy = pb->sfwy;
     a.fcrn = pb->fcrn; a.fsfw = pb->fsfw; a.fsmf = pb->fsmf; a.zpn = pb->zpn;
-    a.vsx = pb->vsx; a.vsy = pb->vsy; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
+    a.vs2 = pb->vs2; a.pxpx = pb->pxpx; a.pbc = pb->pbc;
     a.heavy = pb->heavy; a.nheavy = pb->nheavy;
     a.part_dt = pb->part_d; a.part_dv = pb->part_d + pb->npart_pred;
     a.part_bp = pb->part_i; a.part_bc = pb->part_i + pb->npart_pred;
@@ -2083,7 +2085,7 @@ BatchArgs make_batch_args(pennant_batch* b) {
     a.cfx = b->cfx; a.cfy = b->cfy; a.cmass = b->cmass; a.sfwx = b->sfwx; a.sfwy = b->sfwy;
     a.fcrn = b->fcrn; a.fsfw = b->fsfw;
     a.fsmf = b->fsmf;
-    a.vsx = b->vsx; a.vsy = b->vsy; a.pxpx = b->pxpx; a.pbc = b->pbc;
+    a.vs2 = b->vs2; a.pxpx = b->pxpx; a.pbc = b->pbc;
     a.part_dt = b->part_d; a.part_dv = b->part_d + b->nunits;
     a.part_bp = b->part_i; a.part_bc = b->part_i + b->nunits;
     a.xcd = b->tune.xcd;
@@ -2148,7 +2150,7 @@ int fused_setup(pennant_problem* pb) {
     const size_t NC = (size_t)corner_slots((int64_t)P, (int64_t)S);
     pb->cfx = (double*)al(NC, 8); pb->cfy = (double*)al(NC, 8); pb->cmass = (double*)al(NC, 8);
     pb->sfwx = (double*)al(S, 8); pb->sfwy = (double*)al(S, 8);
-    pb->vsx = (double*)al(P, 8); pb->vsy = (double*)al(P, 8); pb->pxpx = (double*)al(P, 8);
+    pb->vs2 = (double*)al(2 * P, 8); pb->pxpx = (double*)al(P, 8);
     pb->pbc = (unsigned char*)al(P, 1);
     pb->heavy = (int*)al(heavy.size(), 4);
     pb->nheavy = (int)heavy.size();
@@ -2184,7 +2186,7 @@ int fused_setup(pennant_problem* pb) {
     if (!pb->cl_counters || !pb->zlist || !pb->zcount || !pb->zpn) return (int)hipErrorOutOfMemory;
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
-    if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vsx || !pb->vsy ||
+    if (!pb->cfx || !pb->cfy || !pb->cmass || !pb->sfwx || !pb->sfwy || !pb->vs2 ||
         !pb->pxpx || !pb->pbc || !pb->heavy || !pb->pcpos || !pb->part_d || !pb->part_i)
         return (int)hipErrorOutOfMemory;
     if (!heavy.empty() &&

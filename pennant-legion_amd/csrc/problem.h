@@ -114,7 +114,7 @@ struct pennant_problem {
     // fused-path buffers
     double* cfx = nullptr; double* cfy = nullptr; double* cmass = nullptr;
     double* sfwx = nullptr; double* sfwy = nullptr;
-    double* vsx = nullptr; double* vsy = nullptr; double* pxpx = nullptr;
+    double* vs2 = nullptr; double* pxpx = nullptr;   // vs2: {x, y} per point
     float* fcrn = nullptr;           // mixed form: per corner slot {fx, fy, mass}
     float* fsfw = nullptr;           // mixed form: per side {x, y}
     const float* fsmf = nullptr;
@@ -187,7 +187,7 @@ struct pennant_batch {
     unsigned char* pbc = nullptr;
     // fused intermediates
     double *cfx = nullptr, *cfy = nullptr, *cmass = nullptr, *sfwx = nullptr, *sfwy = nullptr;
-    double *vsx = nullptr, *vsy = nullptr, *pxpx = nullptr;
+    double *vs2 = nullptr, *pxpx = nullptr;
     float *fcrn = nullptr, *fsfw = nullptr;
     // work tables
     pn::BUnit* units = nullptr;
