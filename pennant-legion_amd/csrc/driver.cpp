@@ -534,6 +534,7 @@ static int fused_form(int mode) {
 
 int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
     const int form = fused_form(mode);
+    if (pb->smf_stale) CK(refresh_side_fracs(pb));
     // the FP64 and task forms read zr and ze; mixed cycles leave them derived
     if (form == FORM_FP64 || mode == PENNANT_MODE_TASKS) CK(materialize_zre(pb));
     else if (form > 0) pb->zre_stale = true;
@@ -572,7 +573,8 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     const size_t E = 5;
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
     for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
-    int rc = phases ? ensure_lookahead(pb, form) : 0;
+    int rc = pb->smf_stale ? refresh_side_fracs(pb) : 0;
+    if (!rc && phases) rc = ensure_lookahead(pb, form);
     if (!rc && phases && form == FORM_FP64) rc = materialize_zre(pb);
     if (phases && form > 0) pb->zre_stale = true;
     const bool pipe = phases && pipelined(pb, form);
@@ -793,6 +795,8 @@ int pennant_set_field(pennant_problem* pb, const char* name, const double* in) {
         for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = in[i * d.ncomp + c];
         CK(hipMemcpy(d.c[c], tmp.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     }
+    // the mixed forms read an FP32 copy of smf
+    if (!strcmp(name, "smf")) CK(refresh_side_fracs(pb));
     return 0;
 }
 
@@ -802,6 +806,7 @@ double* pennant_field_device(pennant_problem* pb, const char* name, int comp) {
     if (materialize_zre(pb)) return nullptr;
     auto it = pb->fields.find(name);
     if (it == pb->fields.end() || comp < 0 || comp >= it->second.ncomp) return nullptr;
+    if (!strcmp(name, "smf")) pb->smf_stale = true;   // rebuilt before the next fused cycle
     return it->second.c[comp];
 }
 

@@ -2108,6 +2108,15 @@ namespace pn {
 // Builds the zone runs, the per-point BC bits, the heavy-point list and the
 // fused-path buffers (partials sized for the largest grids the tuning
 // allows).
+int refresh_side_fracs(pennant_problem* pb) {
+    const int64_t S = pb->hm.nums;
+    if (S > 0)
+        hipLaunchKernelGGL(k_to_float, dim3(nblocks(S)), dim3(PN_BLOCK), 0, pb->stream, (int)S,
+                           (const double*)pb->f("smf"), const_cast<float*>(pb->fsmf));
+    pb->smf_stale = false;
+    return (int)hipGetLastError();
+}
+
 int fused_setup(pennant_problem* pb) {
     const HostMesh& h = pb->hm;
     pb->zruns.clear();
@@ -2161,9 +2170,7 @@ int fused_setup(pennant_problem* pb) {
     pb->fsmf = fsmf;
     if (!pb->fcrn || !pb->fsfw || !fsmf)
         return (int)hipErrorOutOfMemory;
-    if (S > 0)
-        hipLaunchKernelGGL(k_to_float, dim3(nblocks((int64_t)S)), dim3(PN_BLOCK), 0, pb->stream,
-                           (int)S, (const double*)pb->f("smf"), fsmf);
+    if (refresh_side_fracs(pb)) return (int)hipErrorUnknown;
     // K1 writes each corner's force and mass straight into its corner slot,
     // where K2 reads it
     std::vector<int> pcpos((size_t)h.nums);

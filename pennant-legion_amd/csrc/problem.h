@@ -122,7 +122,8 @@ struct pennant_problem {
     double* part_d = nullptr;        // block partials: dt minima, then dvol maxima
     int* part_i = nullptr;           // block partials: bad-side counts (pred, corr)
     int npart_pred = 0, npart_corr = 0;
-    int* pcpos = nullptr;            // side -> slot of its corner in the point-corner CSR
+    int* pcpos = nullptr;            // side -> slot of its corner record (corner_slot)
+    bool smf_stale = false;          // smf may have been written through its device pointer
     int* heavy = nullptr;            // points with more corners than a K2 thread takes
     int nheavy = 0;
     // learned closure (a ClosurePack on the device) and training-tuple capture
@@ -239,6 +240,9 @@ int materialize_zre(pennant_problem* pb);
 // Empties the learned pass's zone lists (a pipelined launch ends with the
 // look-ahead predictor's list in them).
 int reset_zone_lists(pennant_problem* pb);
+// Rebuilds the FP32 copy of smf that the mixed forms read (at set-up and
+// whenever smf may have changed).
+int refresh_side_fracs(pennant_problem* pb);
 int batch_materialize_zre(pennant_batch* b);
 // device init and end-of-cycle kernels (driver_kernels.hip)
 int launch_init_state(pennant_problem* pb);

@@ -203,3 +203,32 @@ def test_written_zr_drives_the_next_mixed_predictor():
         hot = zp_a != 0.0          # zero internal energy gives zero pressure at any density
         assert hot.sum() > 10
         assert np.all(zp_a[hot] != zp_c[hot])
+
+
+def test_written_smf_drives_the_fused_cycles():
+    """The fused cycles read FP32 copies of smf and per-zone codes that mark
+    uniform side fractions; a write to smf (set_field, or a store through its
+    device pointer) rebuilds both.  Perturbed fractions on a few zones: the
+    mixed form then follows the FP64 form on the perturbed mesh (to its usual
+    accuracy) and departs from the unperturbed run."""
+    text = wd.sedov_rect(48, cstop=40)
+
+    def run(mode, perturb):
+        with pa.Problem(text) as pb:
+            if perturb:
+                smf = pb.get("smf")
+                z = np.arange(0, smf.size // 4, 7)          # every 7th zone
+                s = 4 * z
+                smf[s] += 0.01                               # side 0 of those zones
+                smf[s + 2] -= 0.01                           # side 2
+                pb.set("smf", smf)
+            pb.run(40, mode)
+            return {f: pb.get(f) for f in ZF}
+
+    mix_p, fp_p, mix_0 = run(pa.MODE_MIXED, True), run(pa.MODE_FUSED, True), run(pa.MODE_MIXED, False)
+    for f in ZF:
+        e = rel_l1(mix_p[f], fp_p[f])
+        d = rel_l1(mix_p[f], mix_0[f])
+        say(f"[smf write] {f}: mixed vs FP64 (perturbed) {e:.2e}, perturbed vs not {d:.2e}")
+        assert e < 1e-3, f
+        assert d > 10 * e, f
