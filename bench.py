@@ -347,10 +347,12 @@ def main():
         mixed = (mode in (pa.MODE_MIXED, pa.MODE_LEARNED))
         pipelined = mixed and tune["pipeline"] == 1
         kb = kernel_bytes(S, Z, P, mixed, pipelined)
-        names = (("zone", "point", "learned_pass", "finalize") if pipelined
-                 else ("zone_pred", "point", "zone_corr", "finalize"))
+        names = (("zone", "point", "unused", "finalize", "learned_pass") if pipelined
+                 else ("zone_pred", "point", "zone_corr", "finalize", "learned_pass"))
         per = {n: phase_ms[i] / args.phase_steps for i, n in enumerate(names)}
         for n in names:
+            if n == "unused" or (n == "learned_pass" and mode != pa.MODE_LEARNED):
+                continue
             gbs = kb[n] / (per[n] * 1e-3) / 1e9 if n in kb else 0.0
             log(f"  {n:12s} {per[n] * 1e3:8.1f} us/launch" +
                 (f"  {kb[n] / 1e6:8.1f} MB  {gbs:8.1f} GB/s" if n in kb else ""))
@@ -366,6 +368,8 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic,
                     "alg_bytes_per_launch": kb[dom], "avg_launch_us": round(per[dom] * 1e3, 2),
+                    "phase_us": {n: round(per[n] * 1e3, 2) for n in names
+                                 if n != "unused" and (n != "learned_pass" or mode == pa.MODE_LEARNED)},
                     "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1),
                     "task_decomposition_effective_GBs": round(b_tasks / (ms_per_step * 1e-3) / 1e9, 1)}
         if args.copy_mb > 0:

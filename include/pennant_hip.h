@@ -362,12 +362,13 @@ int pennant_end_cycle(pennant_problem* pb);
 
 /* Runs ncycles cycles in `mode` between HIP events recorded on the problem's
  * stream, then synchronises.  *total_ms receives the elapsed time.  In fused
- * mode, phase_ms (4 entries, may be NULL) receives the summed durations of
- * the zone-predictor, point, zone-corrector and finalize kernels, measured
- * by events recorded between their launches in every cycle (direct launches
- * then, not the captured graph).  For the pipelined mixed cycle the four
- * entries are the zone pass (corrector and next predictor), the point
- * kernel, the learned pass, and the finalize with the predictor redo.
+ * mode, phase_ms (5 entries, may be NULL) receives the summed durations of
+ * the zone-predictor, point, zone-corrector and finalize kernels and of the
+ * learned pass (0 outside the learned form), measured by events recorded
+ * between their launches in every cycle (direct launches then, not the
+ * captured graph).  For the pipelined mixed cycle the first entry is the
+ * zone pass (corrector and next predictor), the third is 0 and the fourth
+ * includes the predictor redo.
  * Returns 0, PENNANT_EBADSIDES or a hipError_t. */
 int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* total_ms,
                         double* phase_ms);

@@ -569,8 +569,9 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     if (ncycles < 1) return PENNANT_EINVAL;
     const int form = fused_form(mode);
     const bool phases = (form >= 0 && phase_ms);
-    // per cycle: events before K1, K2, K3, K4 and after K4
-    const size_t E = 5;
+    // per cycle: events before K1, the learned pass, K2, K3, K4 and after K4
+    // (the pipelined cycle records the first five)
+    const size_t E = 6;
     std::vector<hipEvent_t> ev(phases ? E * (size_t)ncycles + 2 : 2);
     for (hipEvent_t& e : ev) CK(hipEventCreate(&e));
     int rc = pb->smf_stale ? refresh_side_fracs(pb) : 0;
@@ -592,14 +593,16 @@ int pennant_time_cycles(pennant_problem* pb, int ncycles, int mode, double* tota
     if (!rc) rc = (int)hipEventElapsedTime(&ms, ev[0], ev[1]);
     if (!rc) *total_ms = ms;
     if (!rc && phases) {
-        // pipelined: events before K2, the zone pass, K4 and the redo, the
-        // learned pass, and after it; reported as zone pass, K2, learned
-        // pass, K4 and redo
-        static const int kOrder[4] = {0, 1, 2, 3}, kPipe[4] = {1, 0, 3, 2};
+        // reported as K1 (or the pipelined zone pass), K2, K3, K4 (with the
+        // pipelined redo) and the learned pass.  Unpipelined intervals: K1,
+        // learned pass, K2, K3, K4; pipelined: K2, zone pass, K4 and redo,
+        // learned pass
+        static const int kOrder[5] = {0, 4, 1, 2, 3}, kPipe[4] = {1, 0, 3, 4};
         const int* slot = pipe ? kPipe : kOrder;
-        for (int j = 0; j < 4; ++j) phase_ms[j] = 0.;
+        const int nint = pipe ? 4 : 5;
+        for (int j = 0; j < 5; ++j) phase_ms[j] = 0.;
         for (int k = 0; k < ncycles && !rc; ++k)
-            for (int j = 0; j < 4 && !rc; ++j) {
+            for (int j = 0; j < nint && !rc; ++j) {
                 rc = (int)hipEventElapsedTime(&ms, ev[2 + E * (size_t)k + j], ev[2 + E * (size_t)k + j + 1]);
                 phase_ms[slot[j]] += ms;
             }

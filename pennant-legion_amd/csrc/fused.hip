@@ -2231,6 +2231,7 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
         poff += pred_grid(pb, ze - zb, ns, form);
     }
     const int npred = poff;
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (form == FORM_LEARNED) {
         for (size_t r = 0; r < pb->zruns.size(); r += 3) {
             const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
@@ -2251,13 +2252,13 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
             }
         }
     }
-    if (ev) (void)hipEventRecord(ev[1], s);
+    if (ev) (void)hipEventRecord(ev[2], s);
     const dim3 pgrid(pb->nheavy + nblocks(h.nump));
     if (mixed)
         hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, s, a);
     else
         hipLaunchKernelGGL(k_point<double>, pgrid, dim3(PN_BLOCK), 0, s, a);
-    if (ev) (void)hipEventRecord(ev[2], s);
+    if (ev) (void)hipEventRecord(ev[3], s);
     poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
@@ -2274,10 +2275,10 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
         poff += corr_grid(pb, ze - zb);
     }
     const int ncorr = poff;
-    if (ev) (void)hipEventRecord(ev[3], s);
+    if (ev) (void)hipEventRecord(ev[4], s);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred, ncorr, q.cflv,
                        q.cstop, q.dtmax, q.dtfac, q.tstop, mixed ? 0 : 1);
-    if (ev) (void)hipEventRecord(ev[4], s);
+    if (ev) (void)hipEventRecord(ev[5], s);
     return (int)hipGetLastError();
 }
 

@@ -224,7 +224,8 @@ int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 int fused_setup(pennant_problem* pb);
 // forms of the fused cycle
 enum { FORM_FP64 = 0, FORM_MIXED = 1, FORM_LEARNED = 2 };
-// ev: NULL, or 5 events recorded before K1, K2, K3, K4 and after K4
+// ev: NULL, or 6 events recorded before K1, the learned pass, K2, K3, K4 and
+// after K4
 int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form);
 // Pipelined mixed cycle (FORM_MIXED, FORM_LEARNED): K2, then one zone pass
 // with this cycle's corrector and the next cycle's predictor, K4, and a redo

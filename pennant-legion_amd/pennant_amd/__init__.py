@@ -276,12 +276,12 @@ class Problem:
     def time_cycles(self, ncycles: int, mode: int = MODE_FUSED, phases: bool = True):
         """(total_ms, phase_ms or None) for ncycles cycles timed with HIP
         events on the problem's stream; phase_ms holds the summed durations of
-        the zone-predictor, point, zone-corrector and finalize kernels (for
-        the pipelined mixed cycle: zone pass, point kernel, learned pass,
-        finalize with the predictor redo).  Without phases the cycles run as
+        the zone-predictor, point, zone-corrector and finalize kernels and of
+        the learned pass (for the pipelined mixed cycle: zone pass, point
+        kernel, 0, finalize with the predictor redo, learned pass).  Without phases the cycles run as
         pennant_launch_cycles runs them (captured graphs)."""
         tot = ctypes.c_double()
-        ph = (ctypes.c_double * 4)() if phases else None
+        ph = (ctypes.c_double * 5)() if phases else None
         self._check(lib().pennant_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(tot), ph))
         return tot.value, (list(ph) if phases and mode in (MODE_FUSED, MODE_MIXED, MODE_LEARNED) else None)
 
