@@ -285,8 +285,9 @@ def main():
         if args.autotune and not explicit and mode != pa.MODE_TASKS:
             # times every launch shape from a snapshot of the state and
             # restores it, so the problem does not advance
-            ms = pb.autotune(mode, cycles=4)
-            log(f"autotune: {ms / 4 * 1e3:.1f} us/cycle at the best shape")
+            tune_cycles = 16
+            ms = pb.autotune(mode, cycles=tune_cycles)
+            log(f"autotune: {ms / tune_cycles * 1e3:.1f} us/cycle at the best shape")
         tune = {k: pb.get_tuning(k) for k in ("k1_layout", "max_blocks", "xcd", "pipeline", "graph")}
         log(f"tuning {tune}")
         # the timed cycles run as pennant_launch_cycles runs them (16-cycle
