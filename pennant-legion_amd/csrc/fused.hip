@@ -1246,15 +1246,11 @@ template <> struct Corners<double> {
     __device__ static double mass(const FusedArgs& a, int k) { return a.cmass[k]; }
     __device__ static double fx(const FusedArgs& a, int k) { return a.cfx[k]; }
     __device__ static double fy(const FusedArgs& a, int k) { return a.cfy[k]; }
-    __device__ static double wx(const FusedArgs& a, int s) { return a.sfwx[s]; }
-    __device__ static double wy(const FusedArgs& a, int s) { return a.sfwy[s]; }
 };
 template <> struct Corners<float> {
     __device__ static double mass(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k + 2]; }
     __device__ static double fx(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k]; }
     __device__ static double fy(const FusedArgs& a, int k) { return (double)a.fcrn[3 * k + 1]; }
-    __device__ static double wx(const FusedArgs& a, int s) { return (double)a.fsfw[2 * s]; }
-    __device__ static double wy(const FusedArgs& a, int s) { return (double)a.fsfw[2 * s + 1]; }
 };
 
 // Point gathers, fixed BCs, acceleration and position update.
