@@ -93,6 +93,9 @@ CYCLE_DECKS = {
     "sedov90": (lambda: wd.sedov_90(cstop=40), 40),
     "noh_pie30x100": (lambda: ref_deck("noh"), 10),
     "noh_pie64": (lambda: wd.noh_pie(64, 48, cstop=25, dtinit=1e-4), 25),
+    # 96 sectors: the origin has 96 corners, more than a K2 thread takes, so
+    # its block-wide sum reads corner slots past the first four positions
+    "noh_pie24x96": (lambda: wd.noh_pie(24, 96, cstop=12, dtinit=1e-4), 12),
     "leblanc10x90": (lambda: ref_deck("leblanc"), 10),
     "leblanc256x16": (lambda: wd.leblanc_rect(256, 16, cstop=30), 30),
 }
