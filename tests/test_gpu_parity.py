@@ -9,8 +9,9 @@ tests compare raw bits, not values within a tolerance.
     entry points on the device and through the oracle on the host, and every
     field (49 arrays) plus the reduction scalars must match after every step;
   * cycle parity: whole cycles in task mode and in fused mode against the
-    oracle (rect with and without far BCs, pie with triangles, two-state
-    LeBlanc);
+    oracle (rect with and without far BCs, pie with triangles, a pie origin
+    past the per-thread corner sum, two-state LeBlanc, and decks setting the
+    parameters no reference deck sets);
   * full size: Sedov 1440^2 (2.07M zones) -- a few cycles against the oracle,
     and fused mode against task mode;
   * the C ABI called directly on caller-owned device buffers.
@@ -57,7 +58,19 @@ def load_oracle_state(pb, o, dt_current):
                     0, 0, dt_current, 0])
 
 
+def override(text, **kv):
+    """The deck with the keys kv set (replaced where present, else added)."""
+    lines = [ln for ln in text.splitlines() if ln.split() and ln.split()[0] not in kv]
+    return "\n".join(lines + [f"{k} {v!r}" for k, v in kv.items()]) + "\n"
+
+
+# Parameters no reference deck sets (gamma, qgamma, q2 and alfa away from
+# their defaults, ssmin binding, a tight cflv, dtmax and dtfac)
+PARAMS = dict(gamma=1.4, qgamma=1.4, q1=0.3, q2=1.5, alfa=0.3, ssmin=0.05, cflv=0.05, dtmax=5e-4,
+              dtfac=1.05)
+
 STEP_DECKS = {
+    "sedov32_params": lambda: override(wd.sedov_rect(32, cstop=10 ** 6), **PARAMS),
     "sedov45": lambda: ref_deck("sedov"),
     "noh_pie30x100": lambda: ref_deck("noh"),
     "nohsmall_pie2x20": lambda: ref_deck("nohsmall"),
@@ -96,6 +109,9 @@ CYCLE_DECKS = {
     # 96 sectors: the origin has 96 corners, more than a K2 thread takes, so
     # its block-wide sum reads corner slots past the first four positions
     "noh_pie24x96": (lambda: wd.noh_pie(24, 96, cstop=12, dtinit=1e-4), 12),
+    "sedov40_params": (lambda: override(wd.sedov_rect(40, cstop=40), **PARAMS), 40),
+    "noh_pie20x16_params": (lambda: override(wd.noh_pie(20, 16, cstop=25, dtinit=1e-4),
+                                             **dict(PARAMS, q1=0.5, q2=1.2, qgamma=1.2)), 25),
     "leblanc10x90": (lambda: ref_deck("leblanc"), 10),
     "leblanc256x16": (lambda: wd.leblanc_rect(256, 16, cstop=30), 30),
 }
