@@ -15,7 +15,7 @@ import pennant_amd as pa
 from pennant_amd import decks as wd
 from pennant_amd.accuracy import rel_l1
 from pennant_amd.device import DeviceBuffer
-from golden_decks import ref_deck
+from golden_decks import PARAMS, override, ref_deck
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,10 @@ def mixed_decks():
     return [wd.sedov_rect(37, cstop=40), wd.noh_pie(40, 24, cstop=35, dtinit=1e-4),
             ref_deck("sedov"), ref_deck("noh"), wd.leblanc_rect(64, 8, cstop=50),
             wd.sedov_rect(300, energy=0.7, cstop=25), wd.noh_pie(300, 257, speed=1.7, cstop=30, dtinit=1e-5),
-            ref_deck("nohsmall")]
+            ref_deck("nohsmall"),
+            # per-problem parameters away from the other problems'
+            override(wd.sedov_rect(30, cstop=35), **PARAMS),
+            override(wd.noh_pie(20, 16, cstop=25, dtinit=1e-4), **dict(PARAMS, q1=0.5, q2=1.2, qgamma=1.2))]
 
 
 def single(text, mode, ncyc=10 ** 6):

@@ -24,7 +24,7 @@ import pytest
 
 import pennant_amd as pa
 from oracle_lib import FIELDS, Oracle
-from golden_decks import ref_deck
+from golden_decks import PARAMS, override, ref_deck
 from pennant_amd import decks as wd
 
 pytestmark = pytest.mark.gpu
@@ -57,17 +57,6 @@ def load_oracle_state(pb, o, dt_current):
     pb.set_scalars([s["time"], s["dt"], s["dtrec"], s["cycle"], s["dtnew"], s["dvovmax"],
                     0, 0, dt_current, 0])
 
-
-def override(text, **kv):
-    """The deck with the keys kv set (replaced where present, else added)."""
-    lines = [ln for ln in text.splitlines() if ln.split() and ln.split()[0] not in kv]
-    return "\n".join(lines + [f"{k} {v!r}" for k, v in kv.items()]) + "\n"
-
-
-# Parameters no reference deck sets (gamma, qgamma, q2 and alfa away from
-# their defaults, ssmin binding, a tight cflv, dtmax and dtfac)
-PARAMS = dict(gamma=1.4, qgamma=1.4, q1=0.3, q2=1.5, alfa=0.3, ssmin=0.05, cflv=0.05, dtmax=5e-4,
-              dtfac=1.05)
 
 STEP_DECKS = {
     "sedov32_params": lambda: override(wd.sedov_rect(32, cstop=10 ** 6), **PARAMS),
