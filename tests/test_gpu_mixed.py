@@ -5,7 +5,8 @@ FP64 and computes side forces, corner forces and masses and the whole QCS
 chain in FP32.  It is not bit-exact; these tests bound its error:
 
   * against the CPU oracle on the reference's decks (Sedov 45^2, Noh pie,
-    LeBlanc; 10 cycles each) and against the reference-held golden
+    LeBlanc; 10 cycles each), on rect and pie decks that set the parameters
+    no reference deck sets, and against the reference-held golden
     test/sedov/sedov.xy;
   * against the plain FP64 fused form -- which the parity tests prove
     bit-identical to the oracle -- on runs too long for the oracle: Sedov
@@ -28,7 +29,7 @@ import pennant_amd as pa
 from pennant_amd import decks as wd
 from pennant_amd.accuracy import rel_l1, rel_linf
 from oracle_lib import Oracle
-from golden_decks import HERE, ref_deck
+from golden_decks import HERE, PARAMS, override, ref_deck
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +54,10 @@ ORACLE_DECKS = {
     "sedov45": (lambda: ref_deck("sedov"), 10, 1e-7, 1e-6, 1e-6),
     "noh_pie30x100": (lambda: ref_deck("noh"), 10, 2e-4, 5e-4, 5e-5),
     "leblanc10x90": (lambda: ref_deck("leblanc"), 10, 1e-7, 1e-6, 1e-6),
+    # parameters no reference deck sets (see golden_decks.PARAMS)
+    "sedov40_params": (lambda: override(wd.sedov_rect(40, cstop=40), **PARAMS), 40, 2e-7, 2e-7, 1e-7),
+    "noh_pie20x16_params": (lambda: override(wd.noh_pie(20, 16, cstop=25, dtinit=1e-4),
+                                             **dict(PARAMS, q1=0.5, q2=1.2, qgamma=1.2)), 25, 1e-5, 2e-5, 1e-6),
 }
 
 
