@@ -14,9 +14,9 @@
 //                  setCornerDiv, setQCnForce, setForceQCS, setVelDiff, the
 //                  corner force of sumCrnrForce, calcDtNew's per-zone dt
 //   K2 point pass  corner-mass and corner-force sums (ascending side order;
-//                  K1 stores the corner values in point-corner CSR order, so
-//                  each point reads one contiguous run), applyFixedBC,
-//                  calcAccel, advPosFull
+//                  K1 stores each corner's values at its point's slot for
+//                  that corner, corner-major so that neighbouring points'
+//                  slots are adjacent), applyFixedBC, calcAccel, advPosFull
 //   K3 zone pass   calcCtrs, calcVols (corrector), calcWork, calcWorkRate,
 //                  calcEnergy, calcRho, calcDvol's per-zone ratio
 //   K4 finalize    dt minimum, dvol maximum and bad-side counts over the
