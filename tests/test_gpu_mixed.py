@@ -237,3 +237,29 @@ def test_written_smf_drives_the_fused_cycles():
         say(f"[smf write] {f}: mixed vs FP64 (perturbed) {e:.2e}, perturbed vs not {d:.2e}")
         assert e < 1e-3, f
         assert d > 10 * e, f
+
+
+def test_smf_written_through_device_pointer_matches_set_field():
+    """A store through the device pointer handed out for smf reaches the
+    mixed form's FP32 copy before the next cycle, as set_field does at once:
+    both runs give the same bits."""
+    import hip_rt
+    text = wd.sedov_rect(40, cstop=30)
+
+    def run(via_ptr):
+        with pa.Problem(text) as pb:
+            smf = np.ascontiguousarray(pb.get("smf"))
+            s = 4 * np.arange(0, smf.size // 4, 5)
+            smf[s] += 0.01
+            smf[s + 2] -= 0.01
+            if via_ptr:
+                p = pb.device_ptr("smf")
+                assert hip_rt.hip().hipMemcpy(p, smf.ctypes.data, smf.nbytes, hip_rt.H2D) == 0
+            else:
+                pb.set("smf", smf)
+            pb.run(30, pa.MODE_MIXED)
+            return {f: pb.get(f) for f in ZF}
+
+    a, b = run(True), run(False)
+    for f in ZF:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
