@@ -1358,7 +1358,6 @@ template <int NS, typename T, bool PIPE>
 __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb, SideW sw,
                                                const int (&pp)[NS], double dt, int& bad,
                                                double& dvov, d2 (&x)[NS], ZoneState& st) {
-    using C = Corners<T>;
     double et = a.zetot[z];
     const double zvol0 = a.zvol[z];
     const double zm = a.zm[z];
