@@ -93,7 +93,7 @@ struct FusedArgs {
     unsigned long long* cap_count;
     unsigned long long* cl_stats;
     int* zlist;                // zones for the learned pass, run r at [zbeg_r, zend_r)
-    unsigned* zcount;          // per run
+    unsigned* zcount;          // per run and list slice (list_count)
     int nzruns;
 };
 
@@ -104,6 +104,7 @@ struct FusedArgs {
 // and write neighbouring point-corner slots -- meet in the same L2.
 struct Units {
     int u, end, step;
+    int band;   // the eighth of the units (x) with `xcd`, else 0
 };
 __device__ __forceinline__ Units block_units(int nunits, int xcd) {
     const int G = gridDim.x, b = blockIdx.x;
@@ -113,10 +114,12 @@ __device__ __forceinline__ Units block_units(int nunits, int xcd) {
         r.u = (int)((long long)nunits * x / 8) + (b >> 3);
         r.end = (int)((long long)nunits * (x + 1) / 8);
         r.step = per;
+        r.band = x;
     } else {
         r.u = b;
         r.end = nunits;
         r.step = G;
+        r.band = 0;
     }
     return r;
 }
@@ -131,6 +134,21 @@ template <int NS>
 __device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) pp[k] = a.mapsp1[sb + k];
+}
+
+// The learned pass's zone list of a run is kept in PN_LSLICES slices, one
+// per band of block_units (all in slice 0 without the XCD mapping).  Slice s
+// starts at its band's first zone, list_unit0 units into the run, so its
+// entries fit in the band's own zone range; each slice has its own count on
+// its own 128-byte line, so the flagging waves of the 8 XCDs add to 8
+// counts instead of queueing on one.
+#define PN_LSLICES 8
+#define PN_LSTRIDE 32
+__device__ __forceinline__ int list_unit0(int nunits, int s) {
+    return (int)((long long)nunits * s / 8);
+}
+__device__ __forceinline__ unsigned* list_count(const FusedArgs& a, int run, int s) {
+    return a.zcount + (run * PN_LSLICES + s) * PN_LSTRIDE;
 }
 
 // ---------------------------------------------------------------------------
@@ -836,8 +854,8 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 }
 
 // K1, mixed precision.  With FLAG, zones with a gated corner are appended to
-// the run's zone list (a.zlist[zbeg ..], count a.zcount[run]) for the
-// learned pass; their order in the list is not fixed, their results are.
+// the slice of the run's zone list their unit belongs to (list_count) for
+// the learned pass; their order in the list is not fixed, their results are.
 // With REDO it runs only when the pipelined cycle's finalize asked for the
 // predictor to be redone (Ctl::redo).
 template <int NS, bool FLAG, bool REDO = false>
@@ -873,11 +891,11 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
             const uint64_t m = __ballot(cnt.flag);
             if (m) {
                 unsigned base = 0;
-                if ((threadIdx.x & 63) == 0) base = atomicAdd(a.zcount + run, (unsigned)__popcll(m));
+                if ((threadIdx.x & 63) == 0) base = atomicAdd(list_count(a, run, w.band), (unsigned)__popcll(m));
                 base = __shfl(base, 0, 64);
                 if (cnt.flag) {
                     const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
-                    a.zlist[zbeg + base + (unsigned)__popcll(below)] = z;
+                    a.zlist[zbeg + list_unit0(nunits, w.band) * PN_BLOCK + base + (unsigned)__popcll(below)] = z;
                 }
             }
         }
@@ -900,8 +918,16 @@ template <int NS>
 __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
 k_pred_list(FusedArgs a, int zbeg, int zend, int sbeg, int run) {
     if (a.ctl->done) return;
-    const int count = (int)a.zcount[run];
-    const int nunits = (count + PN_BLOCK - 1) / PN_BLOCK;
+    // the slices' counts, and their units of listed zones (lu0[s] the first)
+    const int runits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    int lcnt[PN_LSLICES], lu0[PN_LSLICES + 1];
+    lu0[0] = 0;
+#pragma unroll
+    for (int s = 0; s < PN_LSLICES; ++s) {
+        lcnt[s] = (int)*list_count(a, run, s);
+        lu0[s + 1] = lu0[s] + (lcnt[s] + PN_BLOCK - 1) / PN_BLOCK;
+    }
+    const int nunits = lu0[PN_LSLICES];
     Units w = block_units(nunits, a.xcd);
     if (w.u >= w.end) return;             // no listed zones for this block
     __shared__ ClosurePack wp;
@@ -918,9 +944,12 @@ k_pred_list(FusedArgs a, int zbeg, int zend, int sbeg, int run) {
     int bad = 0;
     ClCount cnt;
     for (; w.u < w.end; w.u += w.step) {
-        const int i = w.u * PN_BLOCK + threadIdx.x;
-        const bool valid = i < count;
-        const int z = a.zlist[zbeg + (valid ? i : 0)];
+        int s = 0;
+#pragma unroll
+        for (int t = 1; t < PN_LSLICES; ++t) s = (lu0[t] <= w.u) ? t : s;
+        const int i = (w.u - lu0[s]) * PN_BLOCK + threadIdx.x;
+        const bool valid = i < lcnt[s];
+        const int z = a.zlist[zbeg + list_unit0(runits, s) * PN_BLOCK + (valid ? i : 0)];
         (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, side_w(z, zbeg, zend, sbeg, NS),
                                              dt, bad, valid, cnt, zre);
     }
@@ -1336,7 +1365,8 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
 // before K2, and the next predictor appends to them.
 template <typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
-    if (a.zcount && blockIdx.x == 0 && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
+    if (a.zcount && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < a.nzruns * PN_LSLICES; i += PN_BLOCK) a.zcount[i * PN_LSTRIDE] = 0u;
     if (a.ctl->done) return;
     const double dt = a.ctl->dt;
     if ((int)blockIdx.x < a.nheavy) {
@@ -1597,11 +1627,11 @@ k_zone_pipe(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run, DtPara
             const uint64_t m = __ballot(cnt.flag);
             if (m) {
                 unsigned base = 0;
-                if ((threadIdx.x & 63) == 0) base = atomicAdd(a.zcount + run, (unsigned)__popcll(m));
+                if ((threadIdx.x & 63) == 0) base = atomicAdd(list_count(a, run, w.band), (unsigned)__popcll(m));
                 base = __shfl(base, 0, 64);
                 if (cnt.flag) {
                     const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
-                    a.zlist[zbeg + base + (unsigned)__popcll(below)] = z;
+                    a.zlist[zbeg + list_unit0(nunits, w.band) * PN_BLOCK + base + (unsigned)__popcll(below)] = z;
                 }
             }
         }
@@ -1735,7 +1765,8 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize(FusedArgs a, int npre
                                                            double cflv, int cstop, double dtmax,
                                                            double dtfac, double tstop, int zre) {
     // the learned pass of this cycle has consumed its zone lists
-    if (a.zcount && (int)threadIdx.x < a.nzruns) a.zcount[threadIdx.x] = 0u;
+    if (a.zcount)
+        for (int i = threadIdx.x; i < a.nzruns * PN_LSLICES; i += PN_FIN_BLOCK) a.zcount[i * PN_LSTRIDE] = 0u;
     finalize_block(a.ctl, a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, cflv, cstop,
                    dtmax, dtfac, tstop, zre);
 }
@@ -1787,7 +1818,7 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
         c->redo = 1;
         c->la_redos += 1;
         if (a.zcount)
-            for (int r = 0; r < a.nzruns; ++r) a.zcount[r] = 0u;
+            for (int r = 0; r < a.nzruns * PN_LSLICES; ++r) a.zcount[r * PN_LSTRIDE] = 0u;
     }
 }
 
@@ -2184,7 +2215,7 @@ int fused_setup(pennant_problem* pb) {
     pb->cl_counters = (unsigned long long*)al(5, 8);
     pb->zpn = (double*)al((size_t)h.numz, 8);
     pb->zlist = (int*)al((size_t)h.numz, 4);
-    pb->zcount = (unsigned*)al(pb->zruns.size() / 3 + 1, 4);
+    pb->zcount = (unsigned*)al(pb->zruns.size() / 3 * PN_LSLICES * PN_LSTRIDE + 1, 4);
     if (!pb->cl_counters || !pb->zlist || !pb->zcount || !pb->zpn) return (int)hipErrorOutOfMemory;
     pb->part_d = (double*)al((size_t)pb->npart_pred + pb->npart_corr, 8);
     pb->part_i = (int*)al((size_t)pb->npart_pred + pb->npart_corr, 4);
@@ -2352,7 +2383,7 @@ int fused_prologue(pennant_problem* pb, int form) {
     const FusedArgs a = make_args(pb);
     hipStream_t s = pb->stream;
     const int nz = (int)pb->zruns.size() / 3;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pb->zcount, nz);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, pb->zcount, nz * PN_LSLICES * PN_LSTRIDE);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, pb->cl_counters + 3, 2);
     const int npred = launch_lookahead_pred<false>(pb, a, form, s);
     if (npred < 0) return PENNANT_EINVAL;
@@ -2412,7 +2443,7 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
 int reset_zone_lists(pennant_problem* pb) {
     const int nz = (int)pb->zruns.size() / 3;
     if (pb->zcount && nz > 0)
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, pb->stream, pb->zcount, nz);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, pb->stream, pb->zcount, nz * PN_LSLICES * PN_LSTRIDE);
     return (int)hipGetLastError();
 }
 

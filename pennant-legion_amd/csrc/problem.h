@@ -134,7 +134,7 @@ struct pennant_problem {
     // counts of a look-ahead predictor, added to [1], [2] when its cycle ends
     unsigned long long* cl_counters = nullptr;
     int* zlist = nullptr;            // zones for the learned pass (per run, at the run's zone range)
-    unsigned* zcount = nullptr;      // per run
+    unsigned* zcount = nullptr;      // per run and list slice, one 128-byte line each
     pn::Tuning tune;
     hipGraphExec_t gexec = nullptr;  // captured fused cycles of form graph_form
     int graph_form = -1;             // form, + 16 for the pipelined cycle
