@@ -917,38 +917,55 @@ k_pred_mixed(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run) {
 template <int NS>
 __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
 k_pred_list(FusedArgs a, int zbeg, int zend, int sbeg, int run) {
-    if (a.ctl->done) return;
-    // the slices' counts, and their units of listed zones (lu0[s] the first)
-    const int runits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
-    int lcnt[PN_LSLICES], lu0[PN_LSLICES + 1];
-    lu0[0] = 0;
+    // The pass is the latency of a few waves, so everything it starts from
+    // (done flag, dt, the slices' counts, the weights) is loaded at once,
+    // before the first wait; the weights go to LDS before the exit test.
+    const int done = a.ctl->done;
+    const double dt = a.ctl->dt;
+    const bool zre = a.ctl->zre_valid != 0;   // read once: the loop stores
+    int lcnt[PN_LSLICES];
 #pragma unroll
-    for (int s = 0; s < PN_LSLICES; ++s) {
-        lcnt[s] = (int)*list_count(a, run, s);
-        lu0[s + 1] = lu0[s] + (lcnt[s] + PN_BLOCK - 1) / PN_BLOCK;
-    }
-    const int nunits = lu0[PN_LSLICES];
-    Units w = block_units(nunits, a.xcd);
-    if (w.u >= w.end) return;             // no listed zones for this block
+    for (int s = 0; s < PN_LSLICES; ++s) lcnt[s] = (int)*list_count(a, run, s);
+    constexpr int NQ = (int)(sizeof(ClosurePack) / 16);
+    constexpr int NQT = (NQ + PN_BLOCK - 1) / PN_BLOCK;
     __shared__ ClosurePack wp;
     {
         const uint4* src = reinterpret_cast<const uint4*>(a.pack);
         uint4* dst = reinterpret_cast<uint4*>(&wp);
-        for (int i = threadIdx.x; i < (int)(sizeof(ClosurePack) / 16); i += PN_BLOCK) dst[i] = src[i];
+        // indices past the end repeat the last quad (the same value to the
+        // same LDS address), so there is no branch between the loads
+#pragma unroll
+        for (int j = 0; j < NQT; ++j) {
+            const int i = min((int)threadIdx.x + j * PN_BLOCK, NQ - 1);
+            dst[i] = src[i];
+        }
     }
+    // the slices' units of listed zones (lu0[s] the first)
+    const int runits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    int lu0[PN_LSLICES + 1];
+    lu0[0] = 0;
+#pragma unroll
+    for (int s = 0; s < PN_LSLICES; ++s) lu0[s + 1] = lu0[s] + (lcnt[s] + PN_BLOCK - 1) / PN_BLOCK;
+    const int nunits = lu0[PN_LSLICES];
+    Units w = block_units(nunits, a.xcd);
+    if (done || w.u >= w.end) return;     // nothing listed for this block
     __syncthreads();
     FusedArgs b = a;
     b.pack = &wp;
-    const double dt = a.ctl->dt;
-    const bool zre = a.ctl->zre_valid != 0;   // read once: the loop stores
     int bad = 0;
     ClCount cnt;
     for (; w.u < w.end; w.u += w.step) {
-        int s = 0;
+        // the slice holding unit w.u (constant indices only: the arrays stay in registers)
+        int s = 0, sc = lcnt[0], su0 = 0;
 #pragma unroll
-        for (int t = 1; t < PN_LSLICES; ++t) s = (lu0[t] <= w.u) ? t : s;
-        const int i = (w.u - lu0[s]) * PN_BLOCK + threadIdx.x;
-        const bool valid = i < lcnt[s];
+        for (int t = 1; t < PN_LSLICES; ++t)
+            if (lu0[t] <= w.u) {
+                s = t;
+                sc = lcnt[t];
+                su0 = lu0[t];
+            }
+        const int i = (w.u - su0) * PN_BLOCK + threadIdx.x;
+        const bool valid = i < sc;
         const int z = a.zlist[zbeg + list_unit0(runits, s) * PN_BLOCK + (valid ? i : 0)];
         (void)zone_pred_mixed<NS, CL_RECOMP>(b, z, sbeg + (z - zbeg) * NS, side_w(z, zbeg, zend, sbeg, NS),
                                              dt, bad, valid, cnt, zre);
@@ -1536,17 +1553,26 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr(FusedArgs a, int zbeg, int ze
 // The control-block values the end of a cycle and the look-ahead dt start from.
 struct CtlSnap {
     double time, dt, la_dtnew;
-    int cycle, la_bp;
+    int cycle, la_bp, done;
 };
-// Read through vector loads (volatile): the caller stores to the block later.
+// Relaxed atomic loads: vector loads (never scalar-cache loads), issued
+// back to back with no wait between them.
+__device__ __forceinline__ double ld_vec(const double* p) {
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ int ld_vec(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Read through vector loads: the caller stores to the block later.
 __device__ __forceinline__ CtlSnap snap_ctl(const Ctl* c) {
-    const volatile Ctl* w = c;
     CtlSnap v;
-    v.time = w->time;
-    v.dt = w->dt;
-    v.la_dtnew = w->la_dtnew;
-    v.cycle = w->cycle;
-    v.la_bp = w->la_bp;
+    v.time = ld_vec(&c->time);
+    v.dt = ld_vec(&c->dt);
+    v.la_dtnew = ld_vec(&c->la_dtnew);
+    v.cycle = ld_vec(&c->cycle);
+    v.la_bp = ld_vec(&c->la_bp);
+    v.done = ld_vec(&c->done);
     return v;
 }
 
@@ -1587,6 +1613,7 @@ k_zone_pipe(FusedArgs a, int zbeg, int zend, int sbeg, int poff, int run, DtPara
     v.la_dtnew = c->la_dtnew;
     v.cycle = c->cycle;
     v.la_bp = 0;
+    v.done = 0;
     bool last;
     const double dtn = lookahead_dt(v, q, last);
     FusedArgs b = a;
@@ -1687,46 +1714,73 @@ __device__ __forceinline__ int blk_sum_fin(int v) {
 
 // The end of the cycle for one problem, by one block of PN_FIN_BLOCK threads,
 // from npred predictor and ncorr corrector partials.
-__device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
-                                         int cstop, double dtmax, double dtfac, double tstop,
-                                         int zre, double* next_dt = nullptr);
+__device__ __forceinline__ int end_cycle(Ctl* c, const CtlSnap& v, double mn, double mx, int bp,
+                                         int bc, double cflv, int cstop, double dtmax,
+                                         double dtfac, double tstop, int zre,
+                                         double* next_dt = nullptr);
+
+// Each thread's share of the partials (thread t folds t, t + PN_FIN_BLOCK,
+// ...), loaded PN_FIN_PER per thread at a time with no branch between the
+// loads.  Indices past the end read the last partial again: a repeat leaves a
+// min or max unchanged and adds 0 to a count.  npred, ncorr >= 1 (every
+// launch has at least one block, every problem at least one unit).
+#define PN_FIN_PER 4
+__device__ __forceinline__ void fold_partials(const double* part_dt, const int* part_bp, int npred,
+                                              const double* part_dv, const int* part_bc, int ncorr,
+                                              double& mn, int& bp, double& mx, int& bc) {
+    const int n = npred > ncorr ? npred : ncorr;
+    for (int base = 0; base < n; base += PN_FIN_PER * PN_FIN_BLOCK) {
+        double dv[PN_FIN_PER], dd[PN_FIN_PER];
+        int ip[PN_FIN_PER], ic[PN_FIN_PER];
+#pragma unroll
+        for (int j = 0; j < PN_FIN_PER; ++j) {
+            const int i = base + (int)threadIdx.x + j * PN_FIN_BLOCK;
+            const int ipd = min(i, npred - 1), icr = min(i, ncorr - 1);
+            dd[j] = part_dt[ipd];
+            ip[j] = part_bp[ipd];
+            dv[j] = part_dv[icr];
+            ic[j] = part_bc[icr];
+        }
+#pragma unroll
+        for (int j = 0; j < PN_FIN_PER; ++j) {
+            const int i = base + (int)threadIdx.x + j * PN_FIN_BLOCK;
+            mn = (dd[j] < mn) ? dd[j] : mn;
+            bp += (i < npred) ? ip[j] : 0;
+            mx = (dv[j] > mx) ? dv[j] : mx;
+            bc += (i < ncorr) ? ic[j] : 0;
+        }
+    }
+}
 
 __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, const int* part_bp,
                                                int npred, const double* part_dv, const int* part_bc,
                                                int ncorr, double cflv, int cstop, double dtmax,
                                                double dtfac, double tstop, int zre) {
-    if (c->done) return;
+    // The control block and the partials are read at once, so the block
+    // waits for memory once (the partials of a run that has stopped are read
+    // but not used).
+    const CtlSnap v = snap_ctl(c);
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
-    for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
-        const double v = part_dt[i];
-        mn = (v < mn) ? v : mn;
-        bp += part_bp[i];
-    }
-    for (int i = threadIdx.x; i < ncorr; i += PN_FIN_BLOCK) {
-        const double v = part_dv[i];
-        mx = (v > mx) ? v : mx;
-        bc += part_bc[i];
-    }
+    fold_partials(part_dt, part_bp, npred, part_dv, part_bc, ncorr, mn, bp, mx, bc);
     mn = blk_min_fin(mn);
     mx = blk_max_fin(mx);
     bp = blk_sum_fin(bp);
     bc = blk_sum_fin(bc);
-    if (threadIdx.x != 0) return;
-    end_cycle(c, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop, zre);
+    if (threadIdx.x != 0 || v.done) return;
+    end_cycle(c, v, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop, zre);
 }
 
 // The end of a cycle from its predictor's dt limit and bad-side count (mn,
-// bp) and its corrector's dvol ratio and bad-side count (mx, bc).  The
-// control block is read once through vector loads before any store to it (a
+// bp) and its corrector's dvol ratio and bad-side count (mx, bc).  v is the
+// control block as read through vector loads before any store to it (a
 // scalar-cache load is not ordered with a later vector store to the same
 // address).
 // Returns the new done flag; *next_dt (when given) receives the next dt.
 // zre: whether this cycle's corrector stored zr and ze (Ctl::zre_valid).
-__device__ __forceinline__ int end_cycle(Ctl* c, double mn, double mx, int bp, int bc, double cflv,
-                                         int cstop, double dtmax, double dtfac, double tstop,
-                                         int zre, double* next_dt) {
-    const CtlSnap v = snap_ctl(c);
+__device__ __forceinline__ int end_cycle(Ctl* c, const CtlSnap& v, double mn, double mx, int bp,
+                                         int bc, double cflv, int cstop, double dtmax,
+                                         double dtfac, double tstop, int zre, double* next_dt) {
     c->zre_valid = zre;
     double r = 1.e99;
     if (mn < r) r = mn;
@@ -1783,16 +1837,7 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
     if (c->done) return;
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
-    for (int i = threadIdx.x; i < npred; i += PN_FIN_BLOCK) {
-        const double v = a.part_dt[i];
-        mn = (v < mn) ? v : mn;
-        bp += a.part_bp[i];
-    }
-    for (int i = threadIdx.x; i < ncorr; i += PN_FIN_BLOCK) {
-        const double v = a.part_dv[i];
-        mx = (v > mx) ? v : mx;
-        bc += a.part_bc[i];
-    }
+    fold_partials(a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, mn, bp, mx, bc);
     mn = blk_min_fin(mn);
     mx = blk_max_fin(mx);
     bp = blk_sum_fin(bp);
@@ -1808,7 +1853,8 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
     bool last;
     const double dtn = lookahead_dt(v, q, last);
     double dt1 = 0.;
-    if (end_cycle(c, v.la_dtnew, mx, v.la_bp, bc, q.cflv, q.cstop, q.dtmax, q.dtfac, q.tstop, 0, &dt1))
+    if (end_cycle(c, v, v.la_dtnew, mx, v.la_bp, bc, q.cflv, q.cstop, q.dtmax, q.dtfac, q.tstop, 0,
+                  &dt1))
         return;
     if (!last && dt1 == dtn) {
         c->la_dtnew = mn;

@@ -180,3 +180,32 @@ def test_autotune_learned_keeps_state_and_counts():
             assert np.array_equal(bits(a.get(f)), bits(b.get(f))), f
         assert a.closure_stats() == b.closure_stats()
     m.close()
+
+
+@pytest.mark.parametrize("pipeline", [0, 1])
+def test_learned_list_slices_keep_results(pipeline):
+    """The learned pass's zone list holds one slice per XCD band of the unit
+    mapping, and a single slice when the mapping is off (xcd 0, or a grid
+    that is not a multiple of 8).  Every launch shape gives the same state
+    and closure counts, bit for bit, on a mesh with a triangle run and a quad
+    run of 50 units."""
+    m = closure()
+    text = wd.noh_pie(128, 100, cstop=500, dtinit=1e-4)
+    ref = None
+    for cap, xcd in ((2048, 1), (48, 1), (16, 1), (13, 1), (48, 0)):
+        with pa.Problem(text) as pb:
+            pb.set_closure(m)
+            pb.set_tuning("max_blocks", cap)
+            pb.set_tuning("xcd", xcd)
+            pb.set_tuning("pipeline", pipeline)
+            pb.run(40, pa.MODE_LEARNED)
+            out = ([bits(pb.get(f)) for f in STATE], pb.closure_stats())
+        say(f"[list slices] pipeline {pipeline} max_blocks {cap} xcd {xcd}: closure {out[1]}")
+        assert out[1][0] > 0
+        if ref is None:
+            ref = out
+            continue
+        for f, x, y in zip(STATE, ref[0], out[0]):
+            assert np.array_equal(x, y), f
+        assert out[1] == ref[1]
+    m.close()
