@@ -746,7 +746,71 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         crmu[k] = (cdiv[k] > 0.f) ? 0.f : zkur * zrpf * cevol[k];
     }
     if constexpr (CL == CL_FLAG) cnt.flag = comp;
-    if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
+    if constexpr (CL == CL_RECOMP) {
+        // The wave's corners that need the network, packed into as few
+        // evaluations as possible: corner k of a lane goes to slot off_k +
+        // its rank among the lanes needing corner k, in an LDS row of the
+        // wave's own; every evaluation takes 64 slots, and its results come
+        // back through the same slots.  A corner's result does not depend on
+        // the other corners of its evaluation (each is its own MFMA column),
+        // so the bits are those of the corner-by-corner evaluation.
+        __shared__ uint32_t xw_all[PN_BLOCK / 64][NS * 64][5];
+        uint32_t (*xw)[5] = xw_all[threadIdx.x >> 6];
+        const int lane = threadIdx.x & 63;
+        const uint64_t below = (1ull << lane) - 1ull;
+        const bool use = valid && comp && inenv;
+        if (valid && comp) {
+            if (inenv) cnt.learned += 1;
+            else cnt.fallback += 1;
+        }
+        int dest[NS];
+        int total = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const bool need = use && ((gated >> k) & 1u);
+            const uint64_t m = __ballot(need);
+            dest[k] = need ? total + (int)__popcll(m & below) : -1;
+            total += (int)__popcll(m);
+            if (need) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xw[dest[k]][j] = flo[k][j];
+                xw[dest[k]][4] = fhi[k];
+            }
+        }
+        // LDS accesses of one wave are performed in order; the fence keeps
+        // the compiler from moving them across
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int e = 0; e < total; e += 64) {
+            const int sl = e + lane;
+            const bool act = sl < total;
+            const uint64_t am = __ballot(act);
+            u32x4 lo, hi;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lo[j] = act ? xw[sl][j] : 0u;
+            hi[0] = act ? xw[sl][4] : 0u;
+            hi[1] = pack_bf16(1.f, 0.f);
+            hi[2] = 0u;
+            hi[3] = 0u;
+            float y0, y1;
+            mlp_wave(a.pack, lo, hi, (uint32_t)am != 0u, (am >> 32) != 0u, y0, y1);
+            if (act) {
+                xw[sl][0] = __float_as_uint(y0);
+                xw[sl][1] = __float_as_uint(y1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            if (dest[k] >= 0) {
+                const float y0 = __uint_as_float(xw[dest[k]][0]);
+                const float y1 = __uint_as_float(xw[dest[k]][1]);
+                crmu[k] = (q1f * y0 * zssf + gq * y1 * cdu[k]) * zrpf * cevol[k];
+            }
+        }
+    }
+    if constexpr (CL == CL_LEARNED) {
         const bool use = valid && comp && inenv;
         if (valid && comp) {
             if (inenv) cnt.learned += 1;
