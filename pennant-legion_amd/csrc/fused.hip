@@ -51,7 +51,7 @@
 
 using namespace pn;
 
-#define PN_FIN_BLOCK 1024    // finalize block size
+#define PN_FIN_BLOCK 256     // finalize block size
 
 namespace {
 
@@ -1755,16 +1755,6 @@ __device__ __forceinline__ double blk_min_fin(double v) {
         for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v = (part[i] < v) ? part[i] : v;
     return v;
 }
-__device__ __forceinline__ double blk_max_fin(double v) {
-    __shared__ double part[PN_FIN_BLOCK / 64];
-    __syncthreads();   // a previous call's partials may still be read
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v = (part[i] > v) ? part[i] : v;
-    return v;
-}
 __device__ __forceinline__ int blk_sum_fin(int v) {
     __shared__ int part[PN_FIN_BLOCK / 64];
     __syncthreads();   // a previous call's partials may still be read
@@ -1774,6 +1764,33 @@ __device__ __forceinline__ int blk_sum_fin(int v) {
     if (threadIdx.x == 0)
         for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) v += part[i];
     return v;
+}
+
+// The four folds of a finalize in one pass over the block: wave folds, one
+// LDS row per wave, and thread 0 folds the rows in wave order (the result is
+// thread 0's).
+__device__ __forceinline__ void blk_fold_fin(double& mn, double& mx, int& bp, int& bc) {
+    __shared__ double pmn[PN_FIN_BLOCK / 64], pmx[PN_FIN_BLOCK / 64];
+    __shared__ int pbp[PN_FIN_BLOCK / 64], pbc[PN_FIN_BLOCK / 64];
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    bp = wave_sum(bp);
+    bc = wave_sum(bc);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        pmn[w] = mn;
+        pmx[w] = mx;
+        pbp[w] = bp;
+        pbc[w] = bc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < PN_FIN_BLOCK / 64; ++i) {
+            mn = (pmn[i] < mn) ? pmn[i] : mn;
+            mx = (pmx[i] > mx) ? pmx[i] : mx;
+            bp += pbp[i];
+            bc += pbc[i];
+        }
 }
 
 // The end of the cycle for one problem, by one block of PN_FIN_BLOCK threads,
@@ -1788,7 +1805,7 @@ __device__ __forceinline__ int end_cycle(Ctl* c, const CtlSnap& v, double mn, do
 // loads.  Indices past the end read the last partial again: a repeat leaves a
 // min or max unchanged and adds 0 to a count.  npred, ncorr >= 1 (every
 // launch has at least one block, every problem at least one unit).
-#define PN_FIN_PER 4
+#define PN_FIN_PER 8
 __device__ __forceinline__ void fold_partials(const double* part_dt, const int* part_bp, int npred,
                                               const double* part_dv, const int* part_bc, int ncorr,
                                               double& mn, int& bp, double& mx, int& bc) {
@@ -1827,10 +1844,7 @@ __device__ __forceinline__ void finalize_block(Ctl* c, const double* part_dt, co
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
     fold_partials(part_dt, part_bp, npred, part_dv, part_bc, ncorr, mn, bp, mx, bc);
-    mn = blk_min_fin(mn);
-    mx = blk_max_fin(mx);
-    bp = blk_sum_fin(bp);
-    bc = blk_sum_fin(bc);
+    blk_fold_fin(mn, mx, bp, bc);
     if (threadIdx.x != 0 || v.done) return;
     end_cycle(c, v, mn, mx, bp, bc, cflv, cstop, dtmax, dtfac, tstop, zre);
 }
@@ -1902,10 +1916,7 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
     double mn = 1.e99, mx = 1.e-99;
     int bp = 0, bc = 0;
     fold_partials(a.part_dt, a.part_bp, npred, a.part_dv, a.part_bc, ncorr, mn, bp, mx, bc);
-    mn = blk_min_fin(mn);
-    mx = blk_max_fin(mx);
-    bp = blk_sum_fin(bp);
-    bc = blk_sum_fin(bc);
+    blk_fold_fin(mn, mx, bp, bc);
     if (threadIdx.x != 0) return;
     if (a.cl_stats) {
         const unsigned long long nl = atomicExch(a.cl_stats + 2, 0ull);
