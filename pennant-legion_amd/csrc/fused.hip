@@ -918,8 +918,8 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
 }
 
 // K1, mixed precision.  With FLAG, zones with a gated corner are appended to
-// the slice of the run's zone list their unit belongs to (list_count) for
-// the learned pass; their order in the list is not fixed, their results are.
+// their block's band slice of the run's zone list (list_count) for the
+// learned pass; their order in the list is not fixed, their results are.
 // With REDO it runs only when the pipelined cycle's finalize asked for the
 // predictor to be redone (Ctl::redo).
 template <int NS, bool FLAG, bool REDO = false>
