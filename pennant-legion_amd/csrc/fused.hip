@@ -383,11 +383,11 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, S
         const int sl = sw_slot<NS>(sw, k);
         if (k + 1 < NS) {
             const f64x2u vx = {sfw[k].x, sfw[k + 1].x}, vy = {sfw[k].y, sfw[k + 1].y};
-            *reinterpret_cast<f64x2u*>(a.sfwx + sl) = vx;
-            *reinterpret_cast<f64x2u*>(a.sfwy + sl) = vy;
+            __builtin_nontemporal_store(vx, reinterpret_cast<f64x2u*>(a.sfwx + sl));
+            __builtin_nontemporal_store(vy, reinterpret_cast<f64x2u*>(a.sfwy + sl));
         } else {
-            a.sfwx[sl] = sfw[k].x;
-            a.sfwy[sl] = sfw[k].y;
+            st_nt(sfw[k].x, a.sfwx + sl);
+            st_nt(sfw[k].y, a.sfwy + sl);
         }
     }
     // calcDtNew for this zone
@@ -1492,12 +1492,12 @@ __device__ __forceinline__ void corr_zone_core(const FusedArgs& a, int z, int sb
         for (int k = 0; k < NS; k += 2) {
             const int sl = sw_slot<NS>(sw, k);
             if (k + 1 < NS) {
-                const f64x2u vx = *reinterpret_cast<const f64x2u*>(a.sfwx + sl);
-                const f64x2u vy = *reinterpret_cast<const f64x2u*>(a.sfwy + sl);
+                const f64x2u vx = __builtin_nontemporal_load(reinterpret_cast<const f64x2u*>(a.sfwx + sl));
+                const f64x2u vy = __builtin_nontemporal_load(reinterpret_cast<const f64x2u*>(a.sfwy + sl));
                 sfw[k] = mk(vx[0], vy[0]);
                 sfw[k + 1] = mk(vx[1], vy[1]);
             } else {
-                sfw[k] = mk(a.sfwx[sl], a.sfwy[sl]);
+                sfw[k] = mk(ld_nt(a.sfwx + sl), ld_nt(a.sfwy + sl));
             }
         }
     }
