@@ -310,6 +310,25 @@ int pennant_run(pennant_problem* pb, int ncycles, int mode);
 
 const char* pennant_error(pennant_problem* pb);
 
+/* Driver::run (src/Driver.cc:102-187): runs the problem in `mode` to its stop
+ * condition (cstop, or tstop through the device predicate), reporting as the
+ * reference does: after loop cycle 0 and after every cycle whose count is a
+ * multiple of dtreport,
+ *   "End cycle %6d, time = %11.5g, dt = %11.5g, wall = %11lld us\n"
+ * (src/Driver.cc:147-156,330-344; wall = microseconds since the previous
+ * report), then the summary of src/Driver.cc:169-180 ("Run complete", cycle
+ * and cstop, time and tstop, "hydro cycle run time= %14.8g us" between
+ * asterisk rules).  Each piece of text goes to report(text, user), or to
+ * stdout when report is NULL.  The host reads the device control block only
+ * at report boundaries (an event and an asynchronous copy each, up to 32
+ * boundaries in flight), never per cycle.  *run_us (may be NULL) receives
+ * the hydro cycle run time: HIP events from the call's start to the end of
+ * the last cycle that ran.  Returns the number of cycles run, or a negative
+ * error code (PENNANT_EBADSIDES with the message in pennant_error). */
+typedef void (*pennant_report_fn)(const char* text, void* user);
+int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void* user,
+                  double* run_us);
+
 /* Counts: out[0]=nump, out[1]=numz, out[2]=nums, out[3]=number of BC lists. */
 void pennant_sizes(pennant_problem* pb, int64_t* out4);
 

@@ -740,6 +740,137 @@ int pennant_run(pennant_problem* pb, int ncycles, int mode) {
     return pb->ctl_host->cycle - c0;
 }
 
+// Driver::run, src/Driver.cc:102-187: the cycle loop with the report line
+// after loop cycle 0 and after every cycle whose count is a multiple of
+// dtreport (src/Driver.cc:147-156, printed by reportMeasurementTask,
+// src/Driver.cc:330-344), then the end-of-run summary (src/Driver.cc:169-180).
+// The host never waits per cycle: the cycles up to each report boundary are
+// queued, followed by an event and an asynchronous copy of the control block
+// into a pinned slot, with up to PN_DRIVE_WINDOW boundaries in flight.  The
+// device predicate makes cycles queued past the stop condition no-ops, as
+// the reference's predicated launches are.
+#define PN_DRIVE_WINDOW 32
+#define PN_DRIVE_AHEAD 2048   // most cycles queued ahead of the last report read
+
+namespace {
+struct DriveSeg {
+    int last;          // loop cycle index of the boundary
+    hipEvent_t end;    // recorded after the boundary's cycle
+    hipEvent_t copied; // recorded after the control block copy
+};
+
+void emit(pennant_report_fn report, void* user, const char* text) {
+    if (report) {
+        report(text, user);
+    } else {
+        std::fputs(text, stdout);
+        std::fflush(stdout);
+    }
+}
+}  // namespace
+
+int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void* user,
+                  double* run_us) {
+    if (mode < PENNANT_MODE_TASKS || mode > PENNANT_MODE_LEARNED) return PENNANT_EINVAL;
+    if (mode == PENNANT_MODE_LEARNED && !pb->closure) return PENNANT_EINVAL;
+    const Params& q = pb->prm;
+    const int dtreport = q.dtreport > 0 ? q.dtreport : 1;
+    CK(copy_ctl_to_host(pb));
+    const int c0 = pb->ctl_host->cycle;
+    bool done = pb->ctl_host->done != 0;
+    Ctl* slots = nullptr;
+    CK(hipHostMalloc((void**)&slots, sizeof(Ctl) * PN_DRIVE_WINDOW, hipHostMallocDefault));
+    std::vector<DriveSeg> ring(PN_DRIVE_WINDOW);
+    int rc = 0;
+    int ncreated = 0;
+    for (DriveSeg& d : ring) {
+        if (!rc) rc = (int)hipEventCreate(&d.end);
+        if (!rc) rc = (int)hipEventCreate(&d.copied);
+        if (!rc) ++ncreated;
+    }
+    hipEvent_t start = nullptr;
+    if (!rc) rc = (int)hipEventCreate(&start);
+    if (!rc) rc = (int)hipEventRecord(start, pb->stream);
+    hipEvent_t prev = start;        // the last measurement (src/Driver.cc:122-123)
+    hipEvent_t stop = start;        // end of the last cycle that ran
+    int next = c0;                  // next loop cycle index to queue
+    int seen = c0;                  // cycle count of the last snapshot read
+    int head = 0, tail = 0;         // ring of queued boundaries [head, tail)
+    char line[160];
+    while (!rc && (head < tail || (!done && next < q.cstop))) {
+        // queue boundaries while the window allows
+        while (!rc && !done && next < q.cstop && tail - head < PN_DRIVE_WINDOW &&
+               (head == tail || next - ring[head % PN_DRIVE_WINDOW].last <= PN_DRIVE_AHEAD)) {
+            // the report boundary at or after loop cycle `next`
+            int last = (next == 0) ? 0 : ((next + 1 + dtreport - 1) / dtreport) * dtreport - 1;
+            if (last < next) last = next;
+            if (last > q.cstop - 1) last = q.cstop - 1;
+            DriveSeg& d = ring[tail % PN_DRIVE_WINDOW];
+            rc = pennant_launch_cycles(pb, last - next + 1, mode);
+            if (!rc) rc = (int)hipEventRecord(d.end, pb->stream);
+            if (!rc) rc = (int)hipMemcpyAsync(&slots[tail % PN_DRIVE_WINDOW], pb->ctl, sizeof(Ctl),
+                                              hipMemcpyDeviceToHost, pb->stream);
+            if (!rc) rc = (int)hipEventRecord(d.copied, pb->stream);
+            d.last = last;
+            next = last + 1;
+            ++tail;
+        }
+        if (rc || head == tail) break;
+        // read the oldest boundary
+        DriveSeg& d = ring[head % PN_DRIVE_WINDOW];
+        rc = (int)hipEventSynchronize(d.copied);
+        if (rc) break;
+        const Ctl& c = slots[head % PN_DRIVE_WINDOW];
+        const bool report_here = (d.last == 0) || ((d.last + 1) % dtreport == 0);
+        if (c.cycle > seen) {
+            // cycles ran in this segment: the run time extends to its end
+            stop = d.end;
+            seen = c.cycle;
+        }
+        if (c.cycle == d.last + 1) {
+            // this boundary's cycle ran
+            float ms = 0.f;
+            rc = (int)hipEventElapsedTime(&ms, prev, d.end);
+            if (rc) break;
+            if (report_here) {
+                snprintf(line, sizeof line, "End cycle %6d, time = %11.5g, dt = %11.5g, wall = %11lld us\n",
+                         c.cycle, c.time, c.dtlast, (long long)(ms * 1e3));
+                emit(report, user, line);
+                prev = d.end;
+            }
+        }
+        if (c.done) done = true;
+        ++head;
+    }
+    float total_ms = 0.f;
+    if (!rc) rc = (int)hipEventElapsedTime(&total_ms, start, stop);
+    if (!rc) rc = copy_ctl_to_host(pb);
+    if (!rc) rc = bad_sides_error(pb);
+    if (!rc) {
+        const Ctl& c = *pb->ctl_host;
+        snprintf(line, sizeof line, "\nRun complete\n");
+        emit(report, user, line);
+        snprintf(line, sizeof line, "cycle = %6d,        cstop = %8d\n", c.cycle, q.cstop);
+        emit(report, user, line);
+        snprintf(line, sizeof line, "time = %14.6g, tstop = %8.6g\n\n", c.time, q.tstop);
+        emit(report, user, line);
+        emit(report, user, "************************************\n");
+        snprintf(line, sizeof line, "hydro cycle run time= %14.8g us\n", (double)total_ms * 1e3);
+        emit(report, user, line);
+        emit(report, user, "************************************\n");
+        if (run_us) *run_us = (double)total_ms * 1e3;
+    }
+    (void)hipStreamSynchronize(pb->stream);
+    for (int i = 0; i < ncreated; ++i) {
+        (void)hipEventDestroy(ring[i].end);
+        (void)hipEventDestroy(ring[i].copied);
+    }
+    if (start) (void)hipEventDestroy(start);
+    (void)hipHostFree(slots);
+    if (rc) return rc;
+    return pb->ctl_host->cycle - c0;
+}
+
 int pennant_lookahead_redos(pennant_problem* pb, int64_t* count) {
     CK(copy_ctl_to_host(pb));
     *count = pb->ctl_host->la_redos;

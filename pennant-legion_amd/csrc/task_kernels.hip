@@ -12,6 +12,7 @@
 //     one-block finalize kernel (the reference folds warp-32 partials into a
 //     DeferredReduction, src/CudaHelp.hh:14-41).
 #include "common.h"
+#include "reduce.h"
 
 using namespace pn;
 
@@ -228,49 +229,6 @@ __global__ void __launch_bounds__(PN_BLOCK) k_calc_energy(int numz, const double
     if (z >= numz) return;
     const double fuzz = 1.e-99;
     ze[z] = zetot[z] / (zm[z] + fuzz);
-}
-
-// calcDtNewTask, src/Hydro.cc:1591-1619: per-block partial minima
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_dt_new(int numz, const double* zdl,
-                                                          const double* zdu, const double* zss,
-                                                          double cfl, double* part) {
-    const double fuzz = 1.e-99;
-    double acc = 1.e99;
-    for (int z = blockIdx.x * PN_BLOCK + threadIdx.x; z < numz; z += gridDim.x * PN_BLOCK) {
-        const double cdu = smax(zdu[z], smax(zss[z], fuzz));
-        const double zdthyd = zdl[z] * cfl / cdu;
-        acc = (zdthyd < acc) ? zdthyd : acc;
-    }
-    acc = block_min(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-__global__ void __launch_bounds__(PN_BLOCK) k_min_parts(const double* part, int nparts,
-                                                        double* out) {
-    double acc = 1.e99;
-    for (int i = threadIdx.x; i < nparts; i += PN_BLOCK) acc = (part[i] < acc) ? part[i] : acc;
-    acc = block_min(acc);
-    if (threadIdx.x == 0) *out = acc;
-}
-
-// calcDvolTask, src/Hydro.cc:1622-1644
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_dvol(int numz, const double* zvol,
-                                                        const double* zvol0, double* part) {
-    double acc = 1.e-99;
-    for (int z = blockIdx.x * PN_BLOCK + threadIdx.x; z < numz; z += gridDim.x * PN_BLOCK) {
-        const double zdvov = fabs((zvol[z] - zvol0[z]) / zvol0[z]);
-        acc = (zdvov > acc) ? zdvov : acc;
-    }
-    acc = block_max(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-__global__ void __launch_bounds__(PN_BLOCK) k_max_parts(const double* part, int nparts,
-                                                        double* out) {
-    double acc = 1.e-99;
-    for (int i = threadIdx.x; i < nparts; i += PN_BLOCK) acc = (part[i] > acc) ? part[i] : acc;
-    acc = block_max(acc);
-    if (threadIdx.x == 0) *out = acc;
 }
 
 // calcDtTask, src/Hydro.cc:1647-1667

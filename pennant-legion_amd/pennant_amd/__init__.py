@@ -52,6 +52,9 @@ def vec(x, y):
     return _Vec(ctypes.c_void_p(int(x)), ctypes.c_void_p(int(y)))
 
 
+# pennant_report_fn: void (*)(const char* text, void* user)
+REPORT_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
+
 # argument kinds used to declare the task entry points
 _P = ctypes.c_void_p
 _D = ctypes.c_double
@@ -96,6 +99,7 @@ DRIVER_SIGNATURES = {
     "pennant_sync": ([_P], ctypes.c_int),
     "pennant_run": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pennant_error": ([_P], ctypes.c_char_p),
+    "pennant_drive": ([_P, ctypes.c_int, REPORT_FN, _P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_sizes": ([_P, ctypes.POINTER(ctypes.c_int64)], None),
     "pennant_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "pennant_set_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -266,6 +270,25 @@ class Problem:
         if rc < 0:
             self._check(rc)
         return rc
+
+    def drive(self, mode: int = MODE_FUSED, echo: bool = False):
+        """Driver::run: runs to the stop condition with the reference's report
+        lines and end-of-run summary (pennant_drive).  Returns (cycles run,
+        the report text, hydro cycle run time in us); echo also prints it."""
+        parts = []
+
+        def _cb(text, _user):
+            t = text.decode()
+            parts.append(t)
+            if echo:
+                print(t, end="", flush=True)
+
+        cb = REPORT_FN(_cb)
+        us = ctypes.c_double()
+        rc = lib().pennant_drive(self._h, int(mode), cb, None, ctypes.byref(us))
+        if rc < 0:
+            self._check(rc)
+        return rc, "".join(parts), us.value
 
     def launch(self, ncycles: int, mode: int = MODE_FUSED):
         self._check(lib().pennant_launch_cycles(self._h, int(ncycles), int(mode)))

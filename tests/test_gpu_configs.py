@@ -1,0 +1,201 @@
+"""The measured workloads, each run on the GPU at its full size.
+
+  * Noh implosion on the 1024 x 1024 pie mesh (1,048,576 zones, triangles at
+    the centre, a 1024-corner origin point): plain FP64 fused cycles against
+    the CPU oracle, bit for bit;
+  * LeBlanc shock tube on the 4096 x 256 rect mesh to t = 6, mixed precision
+    against the plain FP64 form (which the parity tests prove bit-identical
+    to the oracle), both stopped at the same simulated time by tstop;
+  * the batched ensemble of 256 Sedov/Noh problems (std::mt19937_64 20261202,
+    13.8M zones): the FP64 batch bit-identical to the single runs of a
+    stratified sample (the smallest and largest problem of each type and ten
+    random ones), every problem finite; the mixed batch finite too;
+  * Sedov on the 1440 x 1440 rect mesh, learned closure with mixed precision
+    against plain FP64 at equal simulated time after 200+ cycles: density L1
+    within 1%, and the network actually evaluated.
+
+Error norms are pennant_amd.accuracy's rel_l1 (sum|got-want| / sum|want|)
+and rel_linf (max|got-want| / max|want|).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pennant_amd as pa
+from pennant_amd import decks as wd
+from pennant_amd.accuracy import rel_l1, rel_linf
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("px", "pu", "zr", "ze", "zp", "zetot", "zwrate", "zvol")
+ZF = ("zr", "ze", "zp")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_bits_equal(name, got, want):
+    g, w = bits(got), bits(want)
+    if g.shape != w.shape or not np.array_equal(g, w):
+        bad = np.nonzero(g.reshape(-1) != w.reshape(-1))[0]
+        raise AssertionError(f"{name}: {bad.size} of {g.size} values differ; first at {bad[:4]}")
+
+
+def with_tstop(text: str, tstop: float) -> str:
+    lines = [ln for ln in text.splitlines() if ln.split() and ln.split()[0] not in ("tstop", "cstop")]
+    return "\n".join(lines + [f"tstop {tstop!r}"]) + "\n"
+
+
+def test_noh_pie_1024_fused_against_oracle():
+    text = wd.noh_pie_1024(cstop=2)
+    say("[noh pie 1024^2] oracle, 2 cycles")
+    o = Oracle(text)
+    assert o.run(10) == 2
+    say("[noh pie 1024^2] fused FP64, 2 cycles")
+    with pa.Problem(text) as pb:
+        assert (pb.numz, pb.nums, pb.nump) == (1048576, 4193280, 1049601)
+        assert pb.run(10, pa.MODE_FUSED) == 2
+        for f in STATE + ("zm",):
+            assert_bits_equal(f, pb.get(f), o.get(f))
+        so, sp = o.scalars(), pb.scalars()
+        for key in ("time", "dt", "dtrec", "dtnew", "dvovmax"):
+            assert_bits_equal(key, sp[key], so[key])
+
+
+# Bounds set about 10x above the errors measured on MI355X (see the printout).
+LEBLANC_L1 = {"zr": 2e-5, "ze": 2e-5, "zp": 2e-5}
+LEBLANC_LINF = {"zr": 2e-3, "ze": 2e-3, "zp": 2e-3}
+
+
+def test_leblanc_4096x256_mixed_against_fp64_to_t6():
+    text = wd.leblanc_4096x256()
+    res = {}
+    for name, mode in (("fp64", pa.MODE_FUSED), ("mixed", pa.MODE_MIXED)):
+        say(f"[leblanc 4096x256] {name} to t=6")
+        with pa.Problem(text) as pb:
+            assert (pb.numz, pb.nump) == (1048576, 1052929)
+            n = 0
+            while not pb.scalars()["done"]:
+                n += pb.run(4096, mode)
+            sc = pb.scalars()
+            res[name] = (n, sc, {f: pb.get(f) for f in ZF + ("px",)})
+    (n64, s64, ref), (nmx, smx, got) = res["fp64"], res["mixed"]
+    say(f"  cycles fp64 {n64} mixed {nmx}; t {s64['time']!r} / {smx['time']!r}")
+    assert s64["time"] == 6.0 and smx["time"] == 6.0
+    assert abs(nmx - n64) <= max(2, n64 // 1000)
+    for f in ZF:
+        assert np.all(np.isfinite(got[f]))
+        e1, ei = rel_l1(got[f], ref[f]), rel_linf(got[f], ref[f])
+        say(f"  {f}: rel_l1 {e1:.3e} rel_linf {ei:.3e}")
+        assert e1 <= LEBLANC_L1[f], (f, e1)
+        assert ei <= LEBLANC_LINF[f], (f, ei)
+    # the shock tube stays one-dimensional: every row of zones is the same
+    zr = got["zr"].reshape(256, 4096)
+    assert np.max(np.abs(zr - zr[0])) <= 1e-6 * np.max(zr)
+
+
+def _stratified_sample(n, seed=11):
+    """Indices: smallest and largest Sedov (even k) and Noh (odd k) problem,
+    plus 10 other problems drawn at random."""
+    idx = np.arange(n.size)
+    pick = set()
+    for par in (0, 1):
+        sub = idx[idx % 2 == par]
+        pick.add(int(sub[np.argmin(n[sub])]))
+        pick.add(int(sub[np.argmax(n[sub])]))
+    rest = np.array(sorted(set(idx.tolist()) - pick))
+    pick.update(int(k) for k in np.random.default_rng(seed).choice(rest, 10, replace=False))
+    return sorted(pick)
+
+
+ENSEMBLE_CYCLES = 24
+
+
+def test_ensemble256_fp64_batch_bit_identical_to_single_runs():
+    decks, n, par = wd.ensemble(256, cstop=10 ** 7)
+    nz = int((n.astype(np.int64) ** 2).sum())
+    say(f"[ensemble 256] {nz} zones; creating the batch")
+    assert 256 * 32 * 32 <= nz
+    sample = _stratified_sample(n)
+    say(f"  sample {sample} (n = {[int(n[k]) for k in sample]})")
+    with pa.Batch(decks) as b:
+        assert b.count == 256
+        say(f"  batch FP64, {ENSEMBLE_CYCLES} cycles")
+        b.run(ENSEMBLE_CYCLES, pa.MODE_FUSED)
+        assert b.stopped() == []
+        for i in range(256):
+            sc = b.scalars(i)
+            assert sc["cycle"] == ENSEMBLE_CYCLES, i
+            for f in ("px", "pu", "zr", "ze", "zp"):
+                assert np.all(np.isfinite(b.get(i, f))), (i, f)
+        for k in sample:
+            with pa.Problem(decks[k]) as pb:
+                assert pb.run(ENSEMBLE_CYCLES, pa.MODE_FUSED) == ENSEMBLE_CYCLES
+                got, want = b.scalars(k), pb.scalars()
+                for key in ("time", "dt", "dtrec", "dtnew", "dvovmax"):
+                    assert bits(got[key]) == bits(want[key]), (k, key)
+                for f in STATE:
+                    assert_bits_equal(f"problem {k} {f}", b.get(k, f), pb.get(f))
+        say("  sample bit-identical")
+
+
+def test_ensemble256_mixed_batch_runs_finite():
+    decks, n, par = wd.ensemble(256, cstop=10 ** 7)
+    with pa.Batch(decks) as b:
+        b.autotune(pa.MODE_MIXED, cycles=2)
+        say(f"[ensemble 256 mixed] max_blocks {b.get_tuning('max_blocks')} xcd {b.get_tuning('xcd')}")
+        b.run(64, pa.MODE_MIXED)
+        assert b.stopped() == []
+        for i in range(256):
+            assert b.scalars(i)["cycle"] == 64
+            for f in ("px", "zr", "ze", "zp"):
+                assert np.all(np.isfinite(b.get(i, f))), (i, f)
+        # one problem against its own mixed run, bit for bit
+        k = int(np.argmax(n))
+        with pa.Problem(decks[k]) as pb:
+            pb.run(64, pa.MODE_MIXED)
+            for f in STATE:
+                assert_bits_equal(f"problem {k} {f}", b.get(k, f), pb.get(f))
+
+
+def test_sedov1440_learned_mixed_against_fp64_at_equal_time():
+    from pennant_amd.closure import Mlp
+    from pennant_amd.train_closure import DEFAULT_OUT
+    ncyc = 220
+    say(f"[sedov 1440^2] FP64, {ncyc} cycles")
+    with pa.Problem(wd.sedov_1440(cstop=10 ** 7)) as pb:
+        assert pb.run(ncyc, pa.MODE_FUSED) == ncyc
+        t_end = pb.scalars()["time"]
+        ref = {f: pb.get(f) for f in ZF}
+    text = with_tstop(wd.sedov_1440(), t_end)
+    m = Mlp.load(DEFAULT_OUT)
+    out = {}
+    for name, mode in (("mixed", pa.MODE_MIXED), ("learned", pa.MODE_LEARNED)):
+        say(f"[sedov 1440^2] {name} to t={t_end!r}")
+        with pa.Problem(text) as pb:
+            if mode == pa.MODE_LEARNED:
+                pb.set_closure(m)
+            n = pb.run(10 ** 6, mode)
+            sc = pb.scalars()
+            assert sc["done"] == 1 and abs(sc["time"] - t_end) <= 1e-12 * t_end
+            out[name] = (n, {f: pb.get(f) for f in ZF},
+                         pb.closure_stats() if mode == pa.MODE_LEARNED else None)
+    m.close()
+    for name in ("mixed", "learned"):
+        n, st, stats = out[name]
+        errs = {f: rel_l1(st[f], ref[f]) for f in ZF}
+        say(f"  {name}: {n} cycles; L1 " + " ".join(f"{f} {e:.3e}" for f, e in errs.items()) +
+            (f"; closure zones learned/fallback {stats}" if stats else ""))
+        assert n >= 200
+        assert errs["zr"] <= 1e-2
+    assert out["learned"][2][0] > 0
+    # the exact-QCS mixed form is the tighter one
+    assert rel_l1(out["mixed"][1]["zr"], ref["zr"]) <= 1e-4
