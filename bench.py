@@ -8,15 +8,25 @@ oracle) and the mixed form with the exact QCS on the same workload.  A step
 is one hydro cycle over the whole mesh; the state is resident in HBM before
 the timed region starts.
 
+Each form is first advanced, untimed, to mid-problem (--mid-time, t = 0.5 of
+the run to t = 1), so the timed cycles see the developed shock front that
+the learned closure works on; after timing every form runs on to tstop, and
+the forms are compared there, at equal simulated time.
+
 Prints ONE JSON line on stdout:
   value       zone-cycles/s = zones * cycles / wall (HIP events on the
               library's stream around exactly --steps cycles)
-  roofline    dominant kernel: algorithmic bytes per launch / its average
-              duration (HIP events between the launches in the same timed
-              region) against MI355X's 8 TB/s
+  roofline    dominant kernel of the timed form: algorithmic bytes per launch
+              / its average duration (HIP events between the launches, in
+              further cycles right after the timed ones) against 8 TB/s;
+              roofline_fp64: the same for the plain FP64 form
+  accuracy    speedup over plain FP64; density/energy/pressure L1 and Linf
+              errors, shock radius at tstop; the 1% density gate
+  energy      total energy (internal + kinetic) at t = 0 and at tstop, and
+              its relative drift, per form
   cpu_baseline  the CPU oracle (a serial C++ restatement of the reference's
               CPU task bodies, and its OpenMP variant) on a bounded sample of
-              the same workload, on this host
+              the same workload, on this host (lscpu model, sockets, cores)
 Diagnostics go to stderr.
 """
 from __future__ import annotations
@@ -129,39 +139,6 @@ def bench_ensemble(args, pa, decks):
     print(json.dumps(line), flush=True)
 
 
-def physics_check(pa, decks, mlp, n):
-    """Sedov n x n to t = 1 in the plain FP64, mixed and learned forms: density
-    L1 error, shock radius (densest zone) and total energy against FP64."""
-    from pennant_amd.accuracy import rel_l1
-    from pennant_amd.diagnostics import shock_radius, zone_centroids
-    text = decks.sedov_rect(n)
-    hm = pa.HostMesh(text)
-    znump, mapsp1 = hm.get("znump"), hm.get("mapsp1")
-    res = {}
-    for name, mode in (("fp64", pa.MODE_FUSED), ("mixed", pa.MODE_MIXED), ("learned", pa.MODE_LEARNED)):
-        with pa.Problem(text) as pb:
-            if mode == pa.MODE_LEARNED:
-                pb.set_closure(mlp)
-            cyc = 0
-            while not pb.scalars()["done"]:
-                cyc += pb.run(4096, mode)
-            zr, px = pb.get("zr"), pb.get("px")
-            ei, ek = pb.energy()
-            res[name] = (zr, shock_radius(zr, zone_centroids(znump, mapsp1, px)), ei + ek, cyc,
-                         pb.closure_stats() if mode == pa.MODE_LEARNED else None)
-    e0 = res["fp64"][2]
-    out = {"workload": f"sedov-rect-{n}x{n} to t=1", "cycles": res["fp64"][3],
-           "shock_radius_fp64": round(res["fp64"][1], 6), "energy_fp64": e0}
-    for name in ("mixed", "learned"):
-        zr, rs, et, cyc, st = res[name]
-        out[name] = {"rel_l1_zr": float(f"{rel_l1(zr, res['fp64'][0]):.3e}"), "shock_radius": round(rs, 6),
-                     "rel_energy_diff": float(f"{(et - e0) / e0:.3e}"), "cycles": cyc}
-        if st is not None:
-            out[name]["closure_zone_evaluations"] = {"learned": st[0], "exact_fallback": st[1]}
-    log(f"physics check {out}")
-    return out
-
-
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
     "mixed": {"zone_pred": "k_pred_mixed<4, false, false>", "point": "k_point<float>",
@@ -186,6 +163,25 @@ def profiled_traffic(workload: str, kernel: str, form: str = "fp64"):
     return None
 
 
+def host_cpu():
+    """The host's CPU as lscpu reports it: model, sockets, cores per socket,
+    threads per core, and the logical CPUs this process may use."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except Exception as ex:            # lscpu missing: report what os knows
+        info["lscpu_error"] = str(ex)
+    return info
+
+
 def cpu_baseline(deck_text: str, numz: int, budget_s: float):
     """Times the oracle (serial, then OpenMP) on whole cycles of the same
     workload after setup, for about budget_s seconds each."""
@@ -206,6 +202,16 @@ def cpu_baseline(deck_text: str, numz: int, budget_s: float):
     return out, threads
 
 
+def kernel_roofline(kb: dict, per_ms: dict, names) -> dict:
+    """Roofline of the dominant (longest) kernel: its algorithmic bytes per
+    launch over its average launch duration, against 8 TB/s."""
+    dom = max(kb, key=lambda n: per_ms[n])
+    achieved = kb[dom] / (per_ms[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": kb[dom],
+            "avg_launch_us": round(per_ms[dom] * 1e3, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +220,14 @@ def main():
     ap.add_argument("--phase-steps", type=int, default=20,
                     help="cycles after the timed ones, with events between the kernels (direct "
                          "launches), for the per-kernel durations (0 disables)")
+    ap.add_argument("--mid-time", type=float, default=0.5,
+                    help="simulated time each form is advanced to (untimed, graph replay) before "
+                         "the warm-up, so that the timed cycles run mid-problem, where the shock "
+                         "front and the learned closure's work are developed (0: time from the "
+                         "start)")
+    ap.add_argument("--to-end", type=int, default=1,
+                    help="1: after timing, run every form on to tstop and compare them there, at "
+                         "equal simulated time (density L1, shock radius, total-energy drift)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pennant_autotune picks each form's launch shape (grid cap, XCD "
                          "mapping, K1 layout, pipelined cycle) after the warm-up; 0: defaults or "
@@ -230,9 +244,6 @@ def main():
     ap.add_argument("--closure", default=None,
                     help="trained closure (.npz from pennant_amd.train_closure); default: the "
                          "package's closure_ensemble.npz")
-    ap.add_argument("--physics-n", type=int, default=360,
-                    help="Sedov n x n run to t=1 comparing shock radius and energy of the forms "
-                         "(0 disables)")
     ap.add_argument("--copy-mb", type=int, default=512,
                     help="size of the streaming-copy calibration (0 disables)")
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
@@ -249,6 +260,7 @@ def main():
     import pennant_amd as pa
     from pennant_amd import decks
     from pennant_amd.accuracy import field_errors
+    from pennant_amd.diagnostics import shock_radius, zone_centroids
 
     if args.workload == "ensemble256":
         bench_ensemble(args, pa, decks)
@@ -264,8 +276,13 @@ def main():
         if not path.exists():
             raise SystemExit(f"no trained closure at {path}: run python -m pennant_amd.train_closure")
         mlp = Mlp.load(path)
+    hm = pa.HostMesh(text)
+    znump, mapsp1 = hm.get("znump"), hm.get("mapsp1")
 
     def measure(mode):
+        """One form: set up, advance untimed to --mid-time, warm up, tune,
+        time --steps cycles (then --phase-steps with per-kernel events), and
+        with --to-end run on to tstop.  Returns the timings and the states."""
         t0 = time.perf_counter()
         pb = pa.Problem(text)
         if mode == pa.MODE_LEARNED:
@@ -278,7 +295,15 @@ def main():
             pb.set_tuning("xcd", args.xcd)
         if args.pipeline is not None:
             pb.set_tuning("pipeline", args.pipeline)
+        e0 = sum(pb.energy())
         log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
+        t0 = time.perf_counter()
+        while pb.scalars()["time"] < args.mid_time and not pb.scalars()["done"]:
+            pb.run(1024, mode)
+        sc_mid = pb.scalars()
+        if args.mid_time > 0:
+            log(f"advanced to t={sc_mid['time']:.6f} (cycle {int(sc_mid['cycle'])}) in "
+                f"{time.perf_counter() - t0:.1f}s")
         if args.warmup > 0:
             pb.run(args.warmup, mode)
         explicit = any(v is not None for v in (args.k1, args.max_blocks, args.xcd, args.pipeline))
@@ -290,89 +315,76 @@ def main():
             log(f"autotune: {ms / tune_cycles * 1e3:.1f} us/cycle at the best shape")
         tune = {k: pb.get_tuning(k) for k in ("k1_layout", "max_blocks", "xcd", "pipeline", "graph")}
         log(f"tuning {tune}")
+        sc0 = pb.scalars()
+        cl0 = pb.closure_stats() if mode == pa.MODE_LEARNED else None
         # the timed cycles run as pennant_launch_cycles runs them (16-cycle
         # graphs); the per-kernel durations come from further cycles with
         # events between the kernels
         total_ms, _ = pb.time_cycles(args.steps, mode, phases=False)
+        sc1 = pb.scalars()
+        assert sc1["cycle"] == sc0["cycle"] + args.steps and not sc1["done"], sc1
+        cl1 = pb.closure_stats() if mode == pa.MODE_LEARNED else None
         phase_ms = None
         if mode != pa.MODE_TASKS and args.phase_steps > 0:
             _, phase_ms = pb.time_cycles(args.phase_steps, mode, phases=True)
-        sc = pb.scalars()
-        assert sc["cycle"] == args.warmup + args.steps + (args.phase_steps if phase_ms else 0), sc
-        state = {f: pb.get(f) for f in ("zr", "ze", "zp")}
-        if mode == pa.MODE_LEARNED:
-            state["closure_stats"] = pb.closure_stats()
-        dims = (pb.numz, pb.nums, pb.nump)
+        res = {"total_ms": total_ms, "phase_ms": phase_ms, "tune": tune, "e0": e0,
+               "t_window": (sc0["time"], sc1["time"]), "cycle_window": (int(sc0["cycle"]), int(sc1["cycle"])),
+               "closure_window": None if cl0 is None else (cl1[0] - cl0[0], cl1[1] - cl0[1]),
+               "dims": (pb.numz, pb.nums, pb.nump)}
+        if args.to_end:
+            t0 = time.perf_counter()
+            while not pb.scalars()["done"]:
+                pb.run(8192, mode)
+            sc = pb.scalars()
+            res["end"] = {"time": sc["time"], "cycles": int(sc["cycle"]), "e": sum(pb.energy()),
+                          "state": {f: pb.get(f) for f in ("zr", "ze", "zp")}, "px": pb.get("px"),
+                          "closure": pb.closure_stats() if mode == pa.MODE_LEARNED else None}
+            log(f"  ran on to t={sc['time']!r} (cycle {int(sc['cycle'])}) in {time.perf_counter() - t0:.1f}s")
         pb.close()
-        return total_ms, phase_ms, sc, state, dims, tune
+        return res
 
-    mode = modes[args.mode]
-    total_ms, phase_ms, sc, state, (Z, S, P), tune = measure(mode)
-    ms_per_step = total_ms / args.steps
-    value = Z * args.steps / (total_ms * 1e-3)
-    log(f"{args.mode}: {args.steps} cycles in {total_ms:.3f} ms -> {ms_per_step * 1e3:.1f} us/cycle, "
-        f"{value:.4e} zone-cycles/s; t={sc['time']:.6e}")
+    def rate(r):
+        return r["dims"][0] * args.steps / (r["total_ms"] * 1e-3)
 
-    accuracy = None
-    if mode in (pa.MODE_MIXED, pa.MODE_LEARNED):
-        # the plain FP64 form on the same workload, same cycle count: its rate
-        # is the denominator of the speedup and its state the accuracy reference
-        p_ms, _, p_sc, p_state, _, p_tune = measure(pa.MODE_FUSED)
-        plain = Z * args.steps / (p_ms * 1e-3)
-        err = field_errors(state, p_state)
-        log(f"plain fp64 fused: {p_ms / args.steps * 1e3:.1f} us/cycle, {plain:.4e} zone-cycles/s; "
-            f"speedup {value / plain:.3f}; " +
-            " ".join(f"{k} L1 {v[0]:.2e} Linf {v[1]:.2e}" for k, v in err.items()))
-        accuracy = {"reference": "plain FP64 fused form (bit-identical to the CPU oracle)",
-                    "cycles": args.warmup + args.steps,
-                    "time": sc["time"], "time_plain": p_sc["time"],
-                    "rel_l1": {k: float(f"{v[0]:.3e}") for k, v in err.items()},
-                    "rel_linf": {k: float(f"{v[1]:.3e}") for k, v in err.items()},
-                    "plain_fp64_value": round(plain, 1),
-                    "plain_fp64_tuning": p_tune,
-                    "speedup_vs_plain_fp64": round(value / plain, 4)}
-        if mode == pa.MODE_LEARNED:
-            m_ms, _, _, m_state, _, _ = measure(pa.MODE_MIXED)
-            mixed_v = Z * args.steps / (m_ms * 1e-3)
-            nl, nf = state["closure_stats"]
-            log(f"mixed exact QCS: {m_ms / args.steps * 1e3:.1f} us/cycle, {mixed_v:.4e} zone-cycles/s; "
-                f"closure zones learned {nl} fallback {nf}")
-            accuracy["mixed_exact_qcs_value"] = round(mixed_v, 1)
-            accuracy["closure_zone_evaluations"] = {"learned": nl, "exact_fallback": nf}
-            accuracy["rel_l1_learned_vs_mixed_exact"] = float(f"{field_errors(state, m_state)['zr'][0]:.3e}")
-            if args.physics_n > 0:
-                accuracy["physics"] = physics_check(pa, decks, mlp, args.physics_n)
-
-    roofline = None
-    if phase_ms is not None:
-        mixed = (mode in (pa.MODE_MIXED, pa.MODE_LEARNED))
-        pipelined = mixed and tune["pipeline"] == 1
+    def phase_roofline(r, mixed):
+        if r["phase_ms"] is None:
+            return None, None
+        Z, S, P = r["dims"]
+        pipelined = mixed and r["tune"]["pipeline"] == 1
         kb = kernel_bytes(S, Z, P, mixed, pipelined)
         names = (("zone", "point", "unused", "finalize", "learned_pass") if pipelined
                  else ("zone_pred", "point", "zone_corr", "finalize", "learned_pass"))
-        per = {n: phase_ms[i] / args.phase_steps for i, n in enumerate(names)}
+        per = {n: r["phase_ms"][i] / args.phase_steps for i, n in enumerate(names)}
         for n in names:
-            if n == "unused" or (n == "learned_pass" and mode != pa.MODE_LEARNED):
+            if n == "unused":
                 continue
             gbs = kb[n] / (per[n] * 1e-3) / 1e9 if n in kb else 0.0
             log(f"  {n:12s} {per[n] * 1e3:8.1f} us/launch" +
                 (f"  {kb[n] / 1e6:8.1f} MB  {gbs:8.1f} GB/s" if n in kb else ""))
-        dom = max(kb, key=lambda n: per[n])
-        achieved = kb[dom] / (per[dom] * 1e-3) / 1e9
+        return kernel_roofline(kb, per, names), (kb, per, names, pipelined)
+
+    mode = modes[args.mode]
+    main_r = measure(mode)
+    Z, S, P = main_r["dims"]
+    value = rate(main_r)
+    ms_per_step = main_r["total_ms"] / args.steps
+    log(f"{args.mode}: {args.steps} cycles in {main_r['total_ms']:.3f} ms -> {ms_per_step * 1e3:.1f} us/cycle, "
+        f"{value:.4e} zone-cycles/s; t window {main_r['t_window'][0]:.6f}..{main_r['t_window'][1]:.6f}")
+
+    mixed = mode in (pa.MODE_MIXED, pa.MODE_LEARNED)
+    roofline, rdetail = phase_roofline(main_r, mixed)
+    pform = {pa.MODE_LEARNED: "learned", pa.MODE_MIXED: "mixed"}.get(mode, "fp64")
+    if roofline is not None:
+        kb, per, names, pipelined = rdetail
+        roofline["traffic"] = (profiled_traffic(args.workload, roofline["kernel"], pform)
+                               if main_r["tune"]["k1_layout"] == 0 and not pipelined else None)
+        roofline["phase_us"] = {n: round(per[n] * 1e3, 2) for n in names
+                                if n != "unused" and (n != "learned_pass" or mode == pa.MODE_LEARNED)}
+        roofline["cycle_alg_GBs"] = round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1)
         # the reference's task decomposition moves 688 S + 536 Z + 512 P bytes
         # per cycle; at the fused cycle's rate that is this effective bandwidth
-        b_tasks = 688 * S + 536 * Z + 512 * P
-        pform = {pa.MODE_LEARNED: "learned", pa.MODE_MIXED: "mixed"}.get(mode, "fp64")
-        traffic = (profiled_traffic(args.workload, dom, pform)
-                   if tune["k1_layout"] == 0 and not pipelined else None)
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic,
-                    "alg_bytes_per_launch": kb[dom], "avg_launch_us": round(per[dom] * 1e3, 2),
-                    "phase_us": {n: round(per[n] * 1e3, 2) for n in names
-                                 if n != "unused" and (n != "learned_pass" or mode == pa.MODE_LEARNED)},
-                    "cycle_alg_GBs": round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1),
-                    "task_decomposition_effective_GBs": round(b_tasks / (ms_per_step * 1e-3) / 1e9, 1)}
+        roofline["task_decomposition_effective_GBs"] = round(
+            (688 * S + 536 * Z + 512 * P) / (ms_per_step * 1e-3) / 1e9, 1)
         if args.copy_mb > 0:
             import ctypes
             ms = ctypes.c_double()
@@ -382,6 +394,72 @@ def main():
                 log(f"  streaming copy of {args.copy_mb} MiB: {copy_gbs:.1f} GB/s (read + write)")
                 roofline["copy_GBs"] = round(copy_gbs, 1)
 
+    forms = {args.mode: main_r}
+    roofline_fp64 = None
+    accuracy = None
+    if mixed:
+        # the plain FP64 form on the same workload: its mid-run rate is the
+        # denominator of the speedup and its state at tstop the accuracy
+        # reference; the mixed form with the exact QCS beside them
+        forms["fused"] = measure(pa.MODE_FUSED)
+        if mode == pa.MODE_LEARNED:
+            forms["mixed"] = measure(pa.MODE_MIXED)
+        plain = rate(forms["fused"])
+        roofline_fp64, _ = phase_roofline(forms["fused"], False)
+        if roofline_fp64 is not None:
+            roofline_fp64["traffic"] = (profiled_traffic(args.workload, roofline_fp64["kernel"], "fp64")
+                                        if forms["fused"]["tune"]["k1_layout"] == 0 else None)
+        log(f"plain fp64 fused: {forms['fused']['total_ms'] / args.steps * 1e3:.1f} us/cycle, "
+            f"{plain:.4e} zone-cycles/s; speedup {value / plain:.3f}")
+        accuracy = {"reference": "plain FP64 fused form (bit-identical to the CPU oracle)",
+                    "plain_fp64_value": round(plain, 1),
+                    "plain_fp64_tuning": forms["fused"]["tune"],
+                    "speedup_vs_plain_fp64": round(value / plain, 4),
+                    "timed_window": {k: {"t": [round(v, 6) for v in r["t_window"]],
+                                         "cycles": list(r["cycle_window"])} for k, r in forms.items()}}
+        if "mixed" in forms and args.mode != "mixed":
+            mixed_v = rate(forms["mixed"])
+            log(f"mixed exact QCS: {forms['mixed']['total_ms'] / args.steps * 1e3:.1f} us/cycle, "
+                f"{mixed_v:.4e} zone-cycles/s")
+            accuracy["mixed_exact_qcs_value"] = round(mixed_v, 1)
+            accuracy["learned_vs_mixed_exact_speed"] = round(value / mixed_v, 4)
+        if mode == pa.MODE_LEARNED:
+            accuracy["closure_zone_evaluations_in_window"] = dict(
+                zip(("learned", "exact_fallback"), main_r["closure_window"]))
+        if args.to_end:
+            ref = forms["fused"]["end"]
+            cen = zone_centroids(znump, mapsp1, ref["px"])
+            end = {"t": ref["time"], "fp64": {"cycles": ref["cycles"],
+                                              "shock_radius": round(shock_radius(ref["state"]["zr"], cen), 6)}}
+            for k, r in forms.items():
+                if k == "fused":
+                    continue
+                e = r["end"]
+                err = field_errors(e["state"], ref["state"])
+                end[k] = {"cycles": e["cycles"],
+                          "rel_l1": {f: float(f"{v[0]:.3e}") for f, v in err.items()},
+                          "rel_linf": {f: float(f"{v[1]:.3e}") for f, v in err.items()},
+                          "shock_radius": round(shock_radius(e["state"]["zr"],
+                                                             zone_centroids(znump, mapsp1, e["px"])), 6)}
+                if e["closure"] is not None:
+                    end[k]["closure_zone_evaluations"] = dict(zip(("learned", "exact_fallback"), e["closure"]))
+                log(f"  {k} at t={e['time']!r}: " +
+                    " ".join(f"{f} L1 {v[0]:.2e} Linf {v[1]:.2e}" for f, v in err.items()))
+            accuracy["at_tstop"] = end
+            gate = end[args.mode]["rel_l1"]["zr"] if args.mode in end else None
+            accuracy["density_l1_at_tstop"] = gate
+            accuracy["density_l1_gate_1pct_ok"] = (gate is not None and gate <= 0.01)
+    # total energy E = internal + kinetic (per radian, as zm and zvol), at
+    # the start and at tstop, per form
+    if args.to_end:
+        energy = {}
+        for k, r in forms.items():
+            e0, e1 = r["e0"], r["end"]["e"]
+            energy[k] = {"E0": e0, "E_end": e1, "t_end": r["end"]["time"], "drift": float(f"{e1 / e0 - 1:.4e}")}
+            log(f"  energy {k}: E0 {e0!r} E(t={r['end']['time']}) {e1!r} drift {e1 / e0 - 1:+.4e}")
+    else:
+        energy = None
+
     cpu = None
     if args.cpu_budget > 0:
         res, threads = cpu_baseline(text, Z, args.cpu_budget)
@@ -390,9 +468,9 @@ def main():
         log(f"cpu oracle serial {v_ser:.3e} zone-cycles/s ({n_ser} cycles, {t_ser:.1f}s); "
             f"omp x{threads} {v_omp:.3e} ({n_omp} cycles, {t_omp:.1f}s)")
         cpu = {"value": round(v_omp, 1), "unit": "zone-cycles/s", "cores": threads, "kind": "port",
-               "sample": f"{n_omp} cycles of {wname} after setup (OpenMP variant); serial "
-                         f"variant {v_ser:.4g} zone-cycles/s on 1 core over {n_ser} cycles",
-               "serial_value": round(v_ser, 1)}
+               "sample": f"{n_omp} cycles of {wname} after setup (OpenMP variant, {threads} threads); "
+                         f"serial variant {v_ser:.4g} zone-cycles/s on 1 core over {n_ser} cycles",
+               "serial_value": round(v_ser, 1), "host": host_cpu()}
 
     form = {"learned": "mixed-fp32-learned-qcs-fused", "mixed": "mixed-fp32-qcs-fused",
             "fused": "plain-fp64-fused", "tasks": "plain-fp64-tasks"}
@@ -404,11 +482,15 @@ def main():
         "dtype": {pa.MODE_LEARNED: "bf16+f32+f64", pa.MODE_MIXED: "f32+f64"}.get(mode, "f64"),
         "data": "synthetic",
         "config": {"workload": wname, "form": form[args.mode], "zones": Z, "sides": S, "points": P,
-                   "tuning": tune},
+                   "timed_from_t": round(main_r["t_window"][0], 6), "tuning": main_r["tune"]},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if roofline_fp64 is not None:
+        line["roofline_fp64"] = roofline_fp64
     if accuracy is not None:
         line["accuracy"] = accuracy
+    if energy is not None:
+        line["energy"] = energy
     print(json.dumps(line), flush=True)
 
 
