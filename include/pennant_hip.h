@@ -170,6 +170,19 @@ int pennant_hip_calc_dt_new(const pennant_mesh* m, const double* zdl, const doub
                             const double* zss, double cfl, double* dtnew,
                             void* workspace, hipStream_t stream);
 
+/* pennant_hip_calc_dt_new without a workspace argument.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bits as pennant_hip_calc_dt_new; the
+ * per-block partials live in a library-owned device array of PN_RED_BLOCKS
+ * (1024) doubles per device, which the bounded reduction grid never exceeds
+ * for any numz.  Allocates nothing.  Calls that overlap on one device race on
+ * that array: the caller orders them.  PENNANT_EINVAL for a NULL mesh or
+ * output, or numz < 0. */
+int pennant_hip_calc_dt_new_noalloc(const pennant_mesh* m, const double* zdl, const double* zdu,
+                                    const double* zss, double cfl, double* dtnew,
+                                    hipStream_t stream);
+
 /* TID_CALCDVOL: *dvovmax = max(1e-99, max_z |(zvol-zvol0)/zvol0|).
  * Replaces gpu_calc_dvol, src/Hydro.cu:628-681 (src/Hydro.cc:1622-1644). */
 int pennant_hip_calc_dvol(const pennant_mesh* m, const double* zvol, const double* zvol0,

@@ -79,6 +79,7 @@ TASK_SIGNATURES = {
     "pennant_hip_calc_work_rate": [_M, _P, _P, _P, _P, _P, _P, _P],
     "pennant_hip_calc_energy": [_M, _P, _P, _P, _P],
     "pennant_hip_calc_dt_new": [_M, _P, _P, _P, _D, _P, _P, _P],
+    "pennant_hip_calc_dt_new_noalloc": [_M, _P, _P, _P, _D, _P, _P],
     "pennant_hip_calc_dvol": [_M, _P, _P, _P, _P, _P],
     "pennant_hip_calc_dt": [_P, _P, _P, _D, _P, _P],
     "pennant_hip_calc_state_half": [_M, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P],
