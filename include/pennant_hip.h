@@ -17,8 +17,14 @@
  *     reference's structure) or through the fused cycle kernels.
  *
  * Conventions for every task entry point:
- *   - all array arguments are device pointers; vector (double2) fields are
- *     passed as a pair of SoA arrays (pennant_vec / pennant_cvec);
+ *   - all array arguments are device pointers; a vector (double2) field is a
+ *     pennant_vec / pennant_cvec {x, y, stride}: element i is (x[i*stride],
+ *     y[i*stride]).  The reference's double2 field instance
+ *     (AccessorRO<double2>::ptr(rect), src/QCS.cc:135-141, src/Mesh.cu:57-62)
+ *     is passed as is: x = (double*)ptr, y = x + 1, stride = 2.  Two separate
+ *     arrays take stride 1 (stride 0 means 1, so {x, y} initialisers keep
+ *     working).  Interleaved fields with 16-byte aligned x take 16-byte
+ *     accesses; any other stride >= 1 works through a strided path;
  *   - scalar arguments that the reference passes in task->args are passed by
  *     value; dt (task->futures[0] in the reference) is a device pointer to one
  *     double, so a captured hipGraph can replay with a new dt;
@@ -46,9 +52,14 @@ extern "C" {
 /* Mesh connectivity on the device.  Counts follow src/GenMesh.cc:469-526;
  * maps follow src/Mesh.hh:33-105 (MAPSP1, MAPSP2, MAPSZ, MAPSS3, MAPSS4,
  * ZNUMP) stored as int32.  Sides of a zone are contiguous
- * (zsbeg[z]..zsbeg[z+1]-1).  pcbeg/pcside is the point-corner CSR: for point
- * p, pcside[pcbeg[p]..pcbeg[p+1]-1] lists, in ascending order, the sides s
- * with mapsp1[s] == p (corner s of the reference, src/Hydro.cc:1100-1116). */
+ * (zsbeg[z]..zsbeg[z+1]-1) and cyclic, as the reference generates them
+ * (src/GenMesh.cc:1007-1020,1207-1249): mapss3[s] / mapss4[s] are the
+ * previous / next side of the zone (s-1 and s+1, wrapping at the zone's
+ * ends) and mapsp2[s] == mapsp1[mapss4[s]].  Zones have 3 to
+ * PENNANT_MAX_ZSIDES sides; maxznump is the largest.  pcbeg/pcside is the
+ * point-corner CSR: for point p, pcside[pcbeg[p]..pcbeg[p+1]-1] lists, in
+ * ascending order, the sides s with mapsp1[s] == p (corner s of the
+ * reference, src/Hydro.cc:1100-1116). */
 typedef struct pennant_mesh {
     int32_t nump, numz, nums;
     int32_t maxznump;          /* largest znump; at most PENNANT_MAX_ZSIDES */
@@ -65,8 +76,8 @@ typedef struct pennant_mesh {
 
 #define PENNANT_MAX_ZSIDES 8
 
-typedef struct { double* x; double* y; } pennant_vec;
-typedef struct { const double* x; const double* y; } pennant_cvec;
+typedef struct { double* x; double* y; int64_t stride; } pennant_vec;
+typedef struct { const double* x; const double* y; int64_t stride; } pennant_cvec;
 
 /* Bytes of device workspace the reduction tasks (calc_vols, calc_dt_new,
  * calc_dvol) need for their per-block partials. */
@@ -316,7 +327,8 @@ int pennant_lookahead_redos(pennant_problem* pb, int64_t* count);
 int pennant_sync(pennant_problem* pb);
 
 /* Runs up to ncycles cycles and waits for them; returns the number of cycles
- * run or a negative error code.  Cycles are launched in chunks (32 growing to
+ * run or a negative error code (PENNANT_EBADSIDES, or a hipError_t negated).
+ * Cycles are launched in chunks (32 growing to
  * 1024) with a stream sync after each, so a run that stops at cstop or tstop
  * long before ncycles does not queue the remaining (no-op) launches. */
 int pennant_run(pennant_problem* pb, int ncycles, int mode);
@@ -337,7 +349,8 @@ const char* pennant_error(pennant_problem* pb);
  * boundaries in flight), never per cycle.  *run_us (may be NULL) receives
  * the hydro cycle run time: HIP events from the call's start to the end of
  * the last cycle that ran.  Returns the number of cycles run, or a negative
- * error code (PENNANT_EBADSIDES with the message in pennant_error). */
+ * error code: PENNANT_EBADSIDES (message in pennant_error), PENNANT_EINVAL,
+ * or a hipError_t negated. */
 typedef void (*pennant_report_fn)(const char* text, void* user);
 int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void* user,
                   double* run_us);

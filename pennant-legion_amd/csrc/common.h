@@ -38,6 +38,7 @@ __device__ __forceinline__ d2 project(d2 v, d2 u) { return sub(v, scl(dot(v, u),
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
+// fields of the driver (two arrays per vector field, stride 1)
 __device__ __forceinline__ d2 ld(pennant_cvec v, int64_t i) { return mk(v.x[i], v.y[i]); }
 __device__ __forceinline__ d2 ld(pennant_vec v, int64_t i) { return mk(v.x[i], v.y[i]); }
 __device__ __forceinline__ void st(pennant_vec v, int64_t i, d2 a) { v.x[i] = a.x; v.y[i] = a.y; }

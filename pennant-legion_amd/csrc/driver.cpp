@@ -238,20 +238,11 @@ int task_step(pennant_problem* pb, int st) {
     const pennant_mesh* m = &pb->dm;
     hipStream_t s = pb->stream;
     const double* dt = &pb->ctl->dt;
-    const size_t P = (size_t)m->nump * sizeof(double);
     switch (st) {
     case 0:  // copy ZVOL -> ZVOL0, src/Hydro.cc:361-368
-        CK(hipMemcpyAsync(pb->f("zvol0"), pb->f("zvol"), (size_t)m->numz * sizeof(double),
-                          hipMemcpyDeviceToDevice, s));
-        return 0;
+        return launch_copy_zvol0(pb);
     case 1:  // copies, fills and advPosHalf, src/Hydro.cc:389-437
-        CK(hipMemcpyAsync(pb->f("px0", 0), pb->f("px", 0), P, hipMemcpyDeviceToDevice, s));
-        CK(hipMemcpyAsync(pb->f("px0", 1), pb->f("px", 1), P, hipMemcpyDeviceToDevice, s));
-        CK(hipMemcpyAsync(pb->f("pu0", 0), pb->f("pu", 0), P, hipMemcpyDeviceToDevice, s));
-        CK(hipMemcpyAsync(pb->f("pu0", 1), pb->f("pu", 1), P, hipMemcpyDeviceToDevice, s));
-        CK(hipMemsetAsync(pb->f("pmaswt"), 0, P, s));
-        CK(hipMemsetAsync(pb->f("pf", 0), 0, P, s));
-        CK(hipMemsetAsync(pb->f("pf", 1), 0, P, s));
+        CK(launch_point_begin(pb));
         return pennant_hip_adv_pos_half(m, pb->cv("px0"), pb->cv("pu0"), pb->vv("pxp"), dt, s);
     case 2: return pennant_hip_calc_ctrs(m, pb->cv("pxp"), pb->vv("exp"), pb->vv("zxp"), s);
     case 3:
@@ -456,6 +447,7 @@ void pennant_sizes(pennant_problem* pb, int64_t* out4) {
 }
 
 #define PN_GRAPH_CYCLES 16
+#define PN_GRAPH_TASKS 32   // graph_form of captured task-mode cycles
 
 static void drop_graph(pennant_problem* pb) {
     if (pb->gexec) (void)hipGraphExecDestroy(pb->gexec);
@@ -522,6 +514,52 @@ static int build_graph(pennant_problem* pb, int form) {
     return 0;
 }
 
+// One task-mode cycle: the 30 doCycle steps through the task entry points,
+// then the end-of-cycle update (time, cycle, stop test, next dt).
+static int task_cycle(pennant_problem* pb) {
+    for (int st = 0; st < 30; ++st) CK(task_step(pb, st));
+    return launch_end_cycle(pb, 0);
+}
+
+// Whether `n` task-mode cycles from the state in ctl_host all run before the
+// stop condition.  cstop is exact.  For tstop: the next cycle's dt is known,
+// and each later dt is at most dtfac times the one before
+// (calcGlobalDt, src/Driver.cc:225-231) and at most dtmax, so the time after
+// n cycles is bounded; a margin covers the rounding of the time sums.
+static bool task_run_is_safe(const pennant_problem* pb, int n) {
+    const Ctl& c = *pb->ctl_host;
+    const Params& q = pb->prm;
+    if (c.done || (long long)c.cycle + n > (long long)q.cstop) return false;
+    const double f = q.dtfac > 1. ? q.dtfac : 1.;
+    double dt = c.dt, span = 0.;
+    for (int k = 0; k < n; ++k) {
+        span += dt;
+        dt = std::fmin(dt * f, q.dtmax);
+    }
+    if (!(span >= 0.) || !std::isfinite(span)) return false;
+    return c.time + span * (1. + 1.e-6) + 1.e-12 * std::fabs(q.tstop) < q.tstop;
+}
+
+// Captures PN_GRAPH_CYCLES task-mode cycles as one HIP graph (kernel nodes
+// only: the copies and fills of doCycle are kernels too).
+static int build_task_graph(pennant_problem* pb) {
+    drop_graph(pb);
+    hipGraph_t g = nullptr;
+    CK(hipStreamBeginCapture(pb->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < PN_GRAPH_CYCLES && !rc; ++k) rc = task_cycle(pb);
+    const int rc2 = (int)hipStreamEndCapture(pb->stream, &g);
+    if (rc || rc2) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rc2;
+    }
+    rc = (int)hipGraphInstantiate(&pb->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (rc) { pb->gexec = nullptr; return rc; }
+    pb->graph_form = PN_GRAPH_TASKS;
+    return 0;
+}
+
 // fused-cycle form of a driver mode, or -1 for task mode / unknown modes
 static int fused_form(int mode) {
     switch (mode) {
@@ -553,13 +591,23 @@ int pennant_launch_cycles(pennant_problem* pb, int ncycles, int mode) {
     }
     if (mode != PENNANT_MODE_TASKS) return PENNANT_EINVAL;
     drop_lookahead(pb);
-    // Task mode mirrors Driver::run: the host reads the control block each
-    // cycle (the reference reads its dt future through host tasks).
-    for (int k = 0; k < ncycles; ++k) {
+    // Task mode mirrors Driver::run, one launch per task.  The task entry
+    // points have no stop predicate, so the host reads the control block
+    // before every run of cycles it queues: a run of PN_GRAPH_CYCLES cycles
+    // replays one captured HIP graph when no cycle of it can reach the stop
+    // condition (task_run_is_safe), otherwise cycles go one at a time.
+    for (int k = 0; k < ncycles;) {
         CK(copy_ctl_to_host(pb));
         if (pb->ctl_host->done) break;
-        for (int st = 0; st < 30; ++st) CK(task_step(pb, st));
-        CK(launch_end_cycle(pb, 0));
+        if (pb->tune.graph && ncycles - k >= PN_GRAPH_CYCLES &&
+            task_run_is_safe(pb, PN_GRAPH_CYCLES)) {
+            if (pb->graph_form != PN_GRAPH_TASKS || !pb->gexec) CK(build_task_graph(pb));
+            CK(hipGraphLaunch(pb->gexec, pb->stream));
+            k += PN_GRAPH_CYCLES;
+            continue;
+        }
+        CK(task_cycle(pb));
+        ++k;
     }
     return 0;
 }
@@ -726,14 +774,18 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     return rc ? rc : rc2;
 }
 
+// HIP errors (positive codes) as negative return values, for the entry
+// points whose non-negative results are counts
+#define NCK(x) do { int _e = (int)(x); if (_e) return _e > 0 ? -_e : _e; } while (0)
+
 int pennant_run(pennant_problem* pb, int ncycles, int mode) {
-    CK(copy_ctl_to_host(pb));
+    NCK(copy_ctl_to_host(pb));
     const int c0 = pb->ctl_host->cycle;
     int left = ncycles, chunk = 32;
     while (left > 0 && !pb->ctl_host->done) {
         const int k = left < chunk ? left : chunk;
-        CK(pennant_launch_cycles(pb, k, mode));
-        CK(pennant_sync(pb));
+        NCK(pennant_launch_cycles(pb, k, mode));
+        NCK(pennant_sync(pb));
         left -= k;
         chunk = chunk < 1024 ? 2 * chunk : 1024;
     }
@@ -791,8 +843,10 @@ int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void*
     hipEvent_t start = nullptr;
     if (!rc) rc = (int)hipEventCreate(&start);
     if (!rc) rc = (int)hipEventRecord(start, pb->stream);
-    hipEvent_t prev = start;        // the last measurement (src/Driver.cc:122-123)
-    hipEvent_t stop = start;        // end of the last cycle that ran
+    // Times are kept as milliseconds since `start`, which is never re-recorded
+    // (the ring's events are, once their slot comes round again).
+    float prev_ms = 0.f;            // the last measurement (src/Driver.cc:122-123)
+    float stop_ms = 0.f;            // end of the last cycle that ran
     int next = c0;                  // next loop cycle index to queue
     int seen = c0;                  // cycle count of the last snapshot read
     int head = 0, tail = 0;         // ring of queued boundaries [head, tail)
@@ -822,28 +876,25 @@ int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void*
         if (rc) break;
         const Ctl& c = slots[head % PN_DRIVE_WINDOW];
         const bool report_here = (d.last == 0) || ((d.last + 1) % dtreport == 0);
+        float end_ms = 0.f;
+        rc = (int)hipEventElapsedTime(&end_ms, start, d.end);
+        if (rc) break;
         if (c.cycle > seen) {
             // cycles ran in this segment: the run time extends to its end
-            stop = d.end;
+            stop_ms = end_ms;
             seen = c.cycle;
         }
-        if (c.cycle == d.last + 1) {
+        if (c.cycle == d.last + 1 && report_here) {
             // this boundary's cycle ran
-            float ms = 0.f;
-            rc = (int)hipEventElapsedTime(&ms, prev, d.end);
-            if (rc) break;
-            if (report_here) {
-                snprintf(line, sizeof line, "End cycle %6d, time = %11.5g, dt = %11.5g, wall = %11lld us\n",
-                         c.cycle, c.time, c.dtlast, (long long)(ms * 1e3));
-                emit(report, user, line);
-                prev = d.end;
-            }
+            snprintf(line, sizeof line, "End cycle %6d, time = %11.5g, dt = %11.5g, wall = %11lld us\n",
+                     c.cycle, c.time, c.dtlast, (long long)((end_ms - prev_ms) * 1e3f));
+            emit(report, user, line);
+            prev_ms = end_ms;
         }
         if (c.done) done = true;
         ++head;
     }
-    float total_ms = 0.f;
-    if (!rc) rc = (int)hipEventElapsedTime(&total_ms, start, stop);
+    const float total_ms = stop_ms;
     if (!rc) rc = copy_ctl_to_host(pb);
     if (!rc) rc = bad_sides_error(pb);
     if (!rc) {
@@ -867,7 +918,7 @@ int pennant_drive(pennant_problem* pb, int mode, pennant_report_fn report, void*
     }
     if (start) (void)hipEventDestroy(start);
     (void)hipHostFree(slots);
-    if (rc) return rc;
+    if (rc) return rc > 0 ? -rc : rc;   // HIP errors as negative codes
     return pb->ctl_host->cycle - c0;
 }
 

@@ -103,6 +103,23 @@ __global__ void k_end_cycle(Ctl* c, int compute_dtrec, double cflv, int cstop, d
     c->dt = global_dt(c->cycle, c->time, c->dtlast, c->dtrec, dtmax, dtinit, dtfac, tstop);
 }
 
+// Hydro::doCycle's copies and fills, src/Hydro.cc:361-368 and 389-418
+__global__ void __launch_bounds__(PN_BLOCK) k_copy(int n, const double* __restrict__ src,
+                                                   double* __restrict__ dst) {
+    const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+__global__ void __launch_bounds__(PN_BLOCK) k_point_begin(int nump, pennant_cvec px, pennant_cvec pu,
+                                                          pennant_vec px0, pennant_vec pu0,
+                                                          double* pmaswt, pennant_vec pf) {
+    const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (p >= nump) return;
+    st(px0, p, ld(px, p));
+    st(pu0, p, ld(pu, p));
+    pmaswt[p] = 0.;
+    st(pf, p, mk(0., 0.));
+}
+
 // plain streaming copy, the reference dispatch for HBM traffic counters
 __global__ void __launch_bounds__(PN_BLOCK) k_stream_copy(double* __restrict__ dst,
                                                           const double* __restrict__ src, size_t n) {
@@ -161,6 +178,23 @@ int launch_first_dt(pennant_problem* pb) {
     const Params& q = pb->prm;
     hipLaunchKernelGGL(k_first_dt, dim3(1), dim3(1), 0, pb->stream, pb->ctl, q.cstop, q.dtmax,
                        q.dtinit, q.dtfac, q.tstop);
+    return (int)hipGetLastError();
+}
+
+int launch_copy_zvol0(pennant_problem* pb) {
+    const int nz = pb->dm.numz;
+    if (nz > 0)
+        hipLaunchKernelGGL(k_copy, dim3(nblocks(nz)), dim3(PN_BLOCK), 0, pb->stream, nz,
+                           pb->f("zvol"), pb->f("zvol0"));
+    return (int)hipGetLastError();
+}
+
+int launch_point_begin(pennant_problem* pb) {
+    const int np = pb->dm.nump;
+    if (np > 0)
+        hipLaunchKernelGGL(k_point_begin, dim3(nblocks(np)), dim3(PN_BLOCK), 0, pb->stream, np,
+                           pb->cv("px"), pb->cv("pu"), pb->vv("px0"), pb->vv("pu0"),
+                           pb->f("pmaswt"), pb->vv("pf"));
     return (int)hipGetLastError();
 }
 

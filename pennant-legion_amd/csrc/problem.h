@@ -150,8 +150,8 @@ struct pennant_problem {
     double* bcv_dev = nullptr;       // device copy: bcvx then bcvy
 
     double* f(const char* name, int comp = 0) { return fields.at(name).c[comp]; }
-    pennant_cvec cv(const char* name) { pn::DevField& d = fields.at(name); pennant_cvec v = {d.c[0], d.c[1]}; return v; }
-    pennant_vec vv(const char* name) { pn::DevField& d = fields.at(name); pennant_vec v = {d.c[0], d.c[1]}; return v; }
+    pennant_cvec cv(const char* name) { pn::DevField& d = fields.at(name); pennant_cvec v = {d.c[0], d.c[1], 1}; return v; }
+    pennant_vec vv(const char* name) { pn::DevField& d = fields.at(name); pennant_vec v = {d.c[0], d.c[1], 1}; return v; }
 };
 
 namespace pn {
@@ -249,4 +249,9 @@ int batch_materialize_zre(pennant_batch* b);
 int launch_init_state(pennant_problem* pb);
 int launch_end_cycle(pennant_problem* pb, int compute_dtrec);
 int launch_first_dt(pennant_problem* pb);
+// doCycle's copies and fills (src/Hydro.cc:361-368,389-418) as kernels, so
+// that a captured task-mode cycle holds kernel nodes only: zvol0 = zvol;
+// px0 = px, pu0 = pu, pmaswt = 0, pf = 0
+int launch_copy_zvol0(pennant_problem* pb);
+int launch_point_begin(pennant_problem* pb);
 }  // namespace pn

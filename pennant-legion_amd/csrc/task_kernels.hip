@@ -1,124 +1,291 @@
 // Task entry points: one gfx950 kernel (or kernel pair) per Legion task ID on
 // the PENNANT hydro cycle.  Each is the HIP counterpart of the reference's
 // CUDA variant for that task ID, restated from the serial CPU task body it must
-// agree with (file:line on each kernel) but laid out for CDNA:
-//   * zone sums run in one thread per zone over the zone's contiguous sides in
-//     ascending order, starting from the same identity as the CPU body, so the
-//     result is the serial sum bit for bit (the reference's GPU variants use
-//     double atomics, src/Mesh.hh:158-182, and are order-nondeterministic);
-//   * point sums (corner mass, corner force) gather over the point-corner CSR
-//     in ascending side order instead of scattering with atomics;
-//   * min/max reductions use wave64 shuffles and per-block partials, then a
-//     one-block finalize kernel (the reference folds warp-32 partials into a
-//     DeferredReduction, src/CudaHelp.hh:14-41).
+// agree with bit for bit (file:line on each kernel), laid out for CDNA:
+//
+//   * Zone tiles.  A task that couples a side to its zone or to the
+//     neighbouring sides of its zone runs one block per tile of whole zones
+//     (PN_TB / maxznump zones, so a tile has at most PN_TB sides): one lane
+//     per side reads and writes the side arrays fully coalesced, gathers its
+//     point values once into LDS, and finds the previous / next side of its
+//     zone and the zone's values there (the reference reads mapss3/mapss4
+//     and gathers zone values per side).  Zone sums, minima and maxima are
+//     formed by one lane per zone reading the staged side values in
+//     ascending side order from the same starting value as the CPU body, so
+//     they equal the serial result bit for bit (the reference's GPU variants
+//     use double atomics, src/Mesh.hh:158-182, and are order-nondeterministic).
+//   * Point sums (corner mass, corner force) gather over the point-corner CSR
+//     in ascending side order instead of scattering with atomics.  A lane
+//     issues all loads of a point's first four corners before using any
+//     (indices clamped to the point's last corner, so the loads are
+//     unconditional); a point with more corners than a lane takes is summed
+//     by the whole block (contributions staged in LDS, added by one thread in
+//     ascending order), which keeps the pie origin's 1024 corners off the
+//     critical path of one lane.
+//   * Reductions (calcDtNew, calcDvol) use wave64 shuffles and per-block
+//     partials, then a one-block fold (reduce.h).
+//   * Vector (double2) fields come as {x, y, stride}: every kernel that takes
+//     one is instantiated for two separate arrays (stride 1), for the
+//     reference's interleaved double2 instance (stride 2, y = x + 1, 16-byte
+//     accesses) and for any other stride.
+#include <stdint.h>
+
 #include "common.h"
 #include "reduce.h"
 
 using namespace pn;
 
+#define PN_TB 256          // threads of a tile block; a tile holds at most PN_TB sides
+#define PN_PLIGHT 4        // corners a point-gather lane sums by itself
+
 namespace {
+
+// ---------------------------------------------------------------------------
+// Vector-field layouts
+
+enum { VL_STRIDED = 0, VL_SOA = 1, VL_AOS = 2 };
+
+template <int L> struct VIO;
+template <> struct VIO<VL_SOA> {
+    template <class V> __device__ static d2 ld(const V& v, int64_t i) { return mk(v.x[i], v.y[i]); }
+    template <class V> __device__ static double ldx(const V& v, int64_t i) { return v.x[i]; }
+    __device__ static void st(const pennant_vec& v, int64_t i, d2 a) { v.x[i] = a.x; v.y[i] = a.y; }
+};
+template <> struct VIO<VL_AOS> {
+    template <class V> __device__ static d2 ld(const V& v, int64_t i) {
+        const double2 t = reinterpret_cast<const double2*>(v.x)[i];
+        return mk(t.x, t.y);
+    }
+    template <class V> __device__ static double ldx(const V& v, int64_t i) { return v.x[2 * i]; }
+    __device__ static void st(const pennant_vec& v, int64_t i, d2 a) {
+        reinterpret_cast<double2*>(v.x)[i] = make_double2(a.x, a.y);
+    }
+};
+template <> struct VIO<VL_STRIDED> {
+    template <class V> __device__ static d2 ld(const V& v, int64_t i) {
+        return mk(v.x[i * v.stride], v.y[i * v.stride]);
+    }
+    template <class V> __device__ static double ldx(const V& v, int64_t i) { return v.x[i * v.stride]; }
+    __device__ static void st(const pennant_vec& v, int64_t i, d2 a) {
+        v.x[i * v.stride] = a.x;
+        v.y[i * v.stride] = a.y;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Zone tiles
+
+struct Tile {
+    int z0, nz;   // zones z0 .. z0 + nz - 1
+    int s0, ns;   // their sides s0 .. s0 + ns - 1
+};
+
+// Reads the tile's zone offsets into zsb (nz + 1 entries, relative to the
+// mesh) and marks each side with its zone's index in the tile (zof).
+__device__ __forceinline__ Tile tile_setup(const int32_t* zsbeg, int numz, int tz, int* zsb,
+                                           unsigned char* zof) {
+    Tile t;
+    t.z0 = blockIdx.x * tz;
+    t.nz = min(tz, numz - t.z0);
+    if ((int)threadIdx.x <= t.nz) zsb[threadIdx.x] = zsbeg[t.z0 + threadIdx.x];
+    __syncthreads();
+    t.s0 = zsb[0];
+    t.ns = min(zsb[t.nz] - t.s0, PN_TB);
+    if ((int)threadIdx.x < t.nz) {
+        const int b = zsb[threadIdx.x] - t.s0;
+        const int e = min(zsb[threadIdx.x + 1] - t.s0, PN_TB);
+        for (int i = b; i < e; ++i) zof[i] = (unsigned char)threadIdx.x;
+    }
+    __syncthreads();
+    return t;
+}
+
+// A side's place in its zone: tile-local index i of the side, zone index zi
+// in the tile, the zone's first side b (tile-local), its side count n and
+// the side's position k.
+struct SideAt {
+    int i, zi, b, n, k;
+};
+__device__ __forceinline__ SideAt side_at(const Tile& t, const int* zsb, const unsigned char* zof,
+                                          int i) {
+    SideAt c;
+    c.i = i;
+    c.zi = zof[i];
+    c.b = zsb[c.zi] - t.s0;
+    c.n = zsb[c.zi + 1] - zsb[c.zi];
+    c.k = i - c.b;
+    return c;
+}
+// previous and next side of the zone (mapss3, mapss4), tile-local
+__device__ __forceinline__ int side_prev(const SideAt& c) {
+    return min(c.b + (c.k == 0 ? c.n - 1 : c.k - 1), PN_TB - 1);
+}
+__device__ __forceinline__ int side_next(const SideAt& c) {
+    return min(c.b + (c.k + 1 == c.n ? 0 : c.k + 1), PN_TB - 1);
+}
+// the side range of tile zone `zi`, tile-local
+__device__ __forceinline__ int zone_end(const Tile& t, const int* zsb, int zi) {
+    return min(zsb[zi + 1] - t.s0, PN_TB);
+}
+
+// Shared memory and indices every tile kernel declares.
+#define PN_TILE_SHARED                                        \
+    __shared__ int zsb[PN_TB + 1];                            \
+    __shared__ unsigned char zof[PN_TB];                      \
+    const Tile t = tile_setup(m.zsbeg, m.numz, tz, zsb, zof); \
+    const int lane = threadIdx.x;                             \
+    const bool has_side = lane < t.ns;                        \
+    [[maybe_unused]] const bool has_zone = lane < t.nz
 
 // ---------------------------------------------------------------------------
 // Mesh tasks
 
-// calcCtrsTask, src/Mesh.cc:797-837
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_ctrs(pennant_mesh m, pennant_cvec px,
-                                                        pennant_vec ex, pennant_vec zx) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    const int sb = m.zsbeg[z], se = m.zsbeg[z + 1];
-    const int n = m.znump[z];
-    d2 acc = mk(0., 0.);
-    for (int s = sb; s < se; ++s) {
-        const d2 px1 = ld(px, m.mapsp1[s]);
-        const d2 px2 = ld(px, m.mapsp2[s]);
-        st(ex, s, scl(0.5, add(px1, px2)));
-        acc = add(acc, dvs(px1, (double)n));
+// calcCtrsTask, src/Mesh.cc:797-837: ex = (px1 + px2) / 2 per side; zx =
+// sum of px1 * (1/n) over the zone's sides, ascending, from 0.
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_ctrs(pennant_mesh m, int tz, pennant_cvec px,
+                                                     pennant_vec ex, pennant_vec zx) {
+    PN_TILE_SHARED;
+    __shared__ d2 x1[PN_TB], xn[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const d2 p = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
+        x1[lane] = p;
+        xn[lane] = dvs(p, (double)c.n);
     }
-    st(zx, z, acc);
+    __syncthreads();
+    if (has_side) VIO<L>::st(ex, t.s0 + lane, scl(0.5, add(x1[lane], x1[side_next(c)])));
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        d2 acc = mk(0., 0.);
+        for (int i = b; i < e; ++i) acc = add(acc, xn[i]);
+        VIO<L>::st(zx, t.z0 + lane, acc);
+    }
 }
 
-// calcVolsTask, src/Mesh.cc:928-989; partial bad-side counts per block
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_vols(pennant_mesh m, pennant_cvec px,
-                                                        pennant_cvec zx, double* sarea,
-                                                        double* svol, double* zarea,
-                                                        double* zvol, int* numsbad) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    int count = 0;
-    if (z < m.numz) {
-        const double third = 1. / 3.;
-        const d2 c = ld(zx, z);
-        double za = 0., zv = 0.;
-        for (int s = m.zsbeg[z]; s < m.zsbeg[z + 1]; ++s) {
-            const d2 px1 = ld(px, m.mapsp1[s]);
-            const d2 px2 = ld(px, m.mapsp2[s]);
-            const double sa = 0.5 * cross(sub(px2, px1), sub(c, px1));
-            const double sv = third * sa * (px1.x + px2.x + c.x);
-            sarea[s] = sa;
-            svol[s] = sv;
-            za += sa;
-            zv += sv;
-            if (sv <= 0.) count += 1;
-        }
-        zarea[z] = za;
-        zvol[z] = zv;
+// calcVolsTask, src/Mesh.cc:928-989: side areas and volumes, zone sums; the
+// bad-side count (svol <= 0) as one atomic add per block (an integer sum is
+// exact in any order).
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_vols(pennant_mesh m, int tz, pennant_cvec px,
+                                                     pennant_cvec zx, double* sarea, double* svol,
+                                                     double* zarea, double* zvol, int* numsbad) {
+    PN_TILE_SHARED;
+    __shared__ d2 x1[PN_TB], zc[PN_TB];
+    __shared__ double sa_[PN_TB], sv_[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        x1[lane] = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
     }
-    // an integer sum is exact in any order: one atomic per block
+    if (has_zone) zc[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    __syncthreads();
+    int count = 0;
+    if (has_side) {
+        const double third = 1. / 3.;
+        const d2 p1 = x1[lane], p2 = x1[side_next(c)], z = zc[c.zi];
+        const double sa = 0.5 * cross(sub(p2, p1), sub(z, p1));
+        const double sv = third * sa * (p1.x + p2.x + z.x);
+        sarea[t.s0 + lane] = sa;
+        svol[t.s0 + lane] = sv;
+        sa_[lane] = sa;
+        sv_[lane] = sv;
+        if (sv <= 0.) count = 1;
+    }
+    __syncthreads();
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        double za = 0., zv = 0.;
+        for (int i = b; i < e; ++i) {
+            za += sa_[i];
+            zv += sv_[i];
+        }
+        zarea[t.z0 + lane] = za;
+        zvol[t.z0 + lane] = zv;
+    }
     count = block_sum(count);
     if (threadIdx.x == 0 && count > 0) atomicAdd(numsbad, count);
 }
 
 __global__ void k_zero_int(int32_t* out) { *out = 0; }
 
-// calcSideFracsTask, src/Mesh.cc:1085-1101
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_side_fracs(pennant_mesh m, const double* sarea,
-                                                              const double* zarea, double* smf) {
-    const int s = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (s >= m.nums) return;
-    smf[s] = sarea[s] / zarea[m.mapsz[s]];
-}
-
-// calcSurfVecsTask, src/Mesh.cc:1104-1123
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_surf_vecs(pennant_mesh m, pennant_cvec ex,
-                                                             pennant_cvec zx, pennant_vec ssurf) {
-    const int s = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (s >= m.nums) return;
-    st(ssurf, s, rot_ccw(sub(ld(ex, s), ld(zx, m.mapsz[s]))));
-}
-
-// calcEdgeLenTask, src/Mesh.cc:1151-1179
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_edge_len(pennant_mesh m, pennant_cvec px,
-                                                            double* elen) {
-    const int s = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (s >= m.nums) return;
-    elen[s] = len(sub(ld(px, m.mapsp2[s]), ld(px, m.mapsp1[s])));
-}
-
-// calcCharLenTask, src/Mesh.cc:1216-1244
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_char_len(pennant_mesh m, const double* sarea,
-                                                            const double* elen, double* zdl) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    const double fac = (m.znump[z] == 3 ? 3. : 4.);
-    double dl = 1.e99;
-    for (int s = m.zsbeg[z]; s < m.zsbeg[z + 1]; ++s) {
-        const double sdl = fac * sarea[s] / elen[s];
-        dl = smin(dl, sdl);
+// calcSideFracsTask, src/Mesh.cc:1085-1101: smf = sarea / zarea[z]
+__global__ void __launch_bounds__(PN_TB) k_calc_side_fracs(pennant_mesh m, int tz,
+                                                           const double* sarea,
+                                                           const double* zarea, double* smf) {
+    PN_TILE_SHARED;
+    __shared__ double za[PN_TB];
+    if (has_zone) za[lane] = zarea[t.z0 + lane];
+    __syncthreads();
+    if (has_side) {
+        const SideAt c = side_at(t, zsb, zof, lane);
+        smf[t.s0 + lane] = sarea[t.s0 + lane] / za[c.zi];
     }
-    zdl[z] = dl;
+}
+
+// calcSurfVecsTask, src/Mesh.cc:1104-1123: ssurf = rotateCCW(ex - zx[z])
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_surf_vecs(pennant_mesh m, int tz, pennant_cvec ex,
+                                                          pennant_cvec zx, pennant_vec ssurf) {
+    PN_TILE_SHARED;
+    __shared__ d2 zc[PN_TB];
+    if (has_zone) zc[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    __syncthreads();
+    if (has_side) {
+        const SideAt c = side_at(t, zsb, zof, lane);
+        VIO<L>::st(ssurf, t.s0 + lane, rot_ccw(sub(VIO<L>::ld(ex, t.s0 + lane), zc[c.zi])));
+    }
+}
+
+// calcEdgeLenTask, src/Mesh.cc:1151-1179: elen = |px2 - px1|
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_edge_len(pennant_mesh m, int tz, pennant_cvec px,
+                                                         double* elen) {
+    PN_TILE_SHARED;
+    __shared__ d2 x1[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        x1[lane] = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
+    }
+    __syncthreads();
+    if (has_side) elen[t.s0 + lane] = len(sub(x1[side_next(c)], x1[lane]));
+}
+
+// calcCharLenTask, src/Mesh.cc:1216-1244: zdl = min(1e99, min over the
+// zone's sides of fac * sarea / elen), fac = 3 for triangles, else 4
+__global__ void __launch_bounds__(PN_TB) k_calc_char_len(pennant_mesh m, int tz,
+                                                         const double* sarea, const double* elen,
+                                                         double* zdl) {
+    PN_TILE_SHARED;
+    __shared__ double sdl[PN_TB];
+    if (has_side) {
+        const SideAt c = side_at(t, zsb, zof, lane);
+        const double fac = (c.n == 3 ? 3. : 4.);
+        sdl[lane] = fac * sarea[t.s0 + lane] / elen[t.s0 + lane];
+    }
+    __syncthreads();
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        double dl = 1.e99;
+        for (int i = b; i < e; ++i) dl = smin(dl, sdl[i]);
+        zdl[t.z0 + lane] = dl;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Hydro tasks
 
 // advPosHalfTask, src/Hydro.cc:986-1006
+template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) k_adv_pos_half(int nump, pennant_cvec px0,
                                                            pennant_cvec pu0, pennant_vec pxp,
                                                            const double* dtp) {
     const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
     if (p >= nump) return;
     const double dth = 0.5 * (*dtp);
-    st(pxp, p, add(ld(px0, p), scl(dth, ld(pu0, p))));
+    VIO<L>::st(pxp, p, add(VIO<L>::ld(px0, p), scl(dth, VIO<L>::ld(pu0, p))));
 }
 
 // calcRhoTask, src/Hydro.cc:1034-1054
@@ -129,51 +296,151 @@ __global__ void __launch_bounds__(PN_BLOCK) k_calc_rho(int numz, const double* z
     zr[z] = zm[z] / zvol[z];
 }
 
+// The heavy points of a point-gather block: points with more than
+// PN_PLIGHT corners, summed by the whole block after the light ones.
+struct Heavy {
+    int n;
+    int p[PN_BLOCK];
+};
+
 // calcCrnrMassTask, src/Hydro.cc:1083-1117, as a gather over the point's
-// corners in ascending side order
+// corners in ascending side order: pmaswt[p] += zr*zarea*0.5*(smf[s]+smf[s3])
+__device__ __forceinline__ double crnr_mass(const double* smf, const double* zr,
+                                            const double* zarea, int s, int s3, int z) {
+    return zr[z] * zarea[z] * 0.5 * (smf[s] + smf[s3]);
+}
+
 __global__ void __launch_bounds__(PN_BLOCK) k_calc_crnr_mass(pennant_mesh m, const double* smf,
                                                              const double* zr,
                                                              const double* zarea,
                                                              double* pmaswt) {
+    __shared__ Heavy hv;
+    __shared__ double part[PN_BLOCK];
+    if (threadIdx.x == 0) hv.n = 0;
+    __syncthreads();
     const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (p >= m.nump) return;
-    double acc = pmaswt[p];
-    for (int k = m.pcbeg[p]; k < m.pcbeg[p + 1]; ++k) {
-        const int s = m.pcside[k];
-        const int z = m.mapsz[s];
-        const double mwt = zr[z] * zarea[z] * 0.5 * (smf[s] + smf[m.mapss3[s]]);
-        acc += mwt;
+    if (p < m.nump) {
+        const int kb = m.pcbeg[p], ke = m.pcbeg[p + 1];
+        if (ke - kb > PN_PLIGHT) {
+            hv.p[atomicAdd(&hv.n, 1)] = p;
+        } else if (ke > kb) {
+            int s[PN_PLIGHT], s3[PN_PLIGHT], z[PN_PLIGHT];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) s[j] = m.pcside[min(kb + j, ke - 1)];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) {
+                s3[j] = m.mapss3[s[j]];
+                z[j] = m.mapsz[s[j]];
+            }
+            double w[PN_PLIGHT];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) w[j] = crnr_mass(smf, zr, zarea, s[j], s3[j], z[j]);
+            double acc = pmaswt[p];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j)
+                if (kb + j < ke) acc += w[j];
+            pmaswt[p] = acc;
+        }
     }
-    pmaswt[p] = acc;
+    __syncthreads();
+    const int nh = hv.n;
+    for (int h = 0; h < nh; ++h) {
+        const int q = hv.p[h];
+        const int kb = m.pcbeg[q], ke = m.pcbeg[q + 1];
+        double acc = 0.;
+        if (threadIdx.x == 0) acc = pmaswt[q];
+        for (int k0 = kb; k0 < ke; k0 += PN_BLOCK) {
+            const int k = k0 + threadIdx.x;
+            if (k < ke) {
+                const int s = m.pcside[k];
+                part[threadIdx.x] = crnr_mass(smf, zr, zarea, s, m.mapss3[s], m.mapsz[s]);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int cnt = min(PN_BLOCK, ke - k0);
+                for (int i = 0; i < cnt; ++i) acc += part[i];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) pmaswt[q] = acc;
+    }
 }
 
-// sumCrnrForceTask, src/Hydro.cc:1159-1193, as a gather
+// sumCrnrForceTask, src/Hydro.cc:1159-1193, as a gather:
+// pf[p] += (sfp+sfq+sft)[s] - (sfp+sfq+sft)[s3] over the point's corners
+template <int L>
+__device__ __forceinline__ d2 crnr_force(const pennant_cvec& sfp, const pennant_cvec& sfq,
+                                         const pennant_cvec& sft, int s, int s3) {
+    const d2 f = add(add(VIO<L>::ld(sfp, s), VIO<L>::ld(sfq, s)), VIO<L>::ld(sft, s));
+    const d2 f3 = add(add(VIO<L>::ld(sfp, s3), VIO<L>::ld(sfq, s3)), VIO<L>::ld(sft, s3));
+    return sub(f, f3);
+}
+
+template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
                                                              pennant_cvec sfq, pennant_cvec sft,
                                                              pennant_vec pf) {
+    __shared__ Heavy hv;
+    __shared__ d2 part[PN_BLOCK];
+    if (threadIdx.x == 0) hv.n = 0;
+    __syncthreads();
     const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (p >= m.nump) return;
-    d2 acc = ld(pf, p);
-    for (int k = m.pcbeg[p]; k < m.pcbeg[p + 1]; ++k) {
-        const int s = m.pcside[k];
-        const int s3 = m.mapss3[s];
-        const d2 f = add(add(ld(sfp, s), ld(sfq, s)), ld(sft, s));
-        const d2 f3 = add(add(ld(sfp, s3), ld(sfq, s3)), ld(sft, s3));
-        acc = add(acc, sub(f, f3));
+    if (p < m.nump) {
+        const int kb = m.pcbeg[p], ke = m.pcbeg[p + 1];
+        if (ke - kb > PN_PLIGHT) {
+            hv.p[atomicAdd(&hv.n, 1)] = p;
+        } else if (ke > kb) {
+            int s[PN_PLIGHT], s3[PN_PLIGHT];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) s[j] = m.pcside[min(kb + j, ke - 1)];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) s3[j] = m.mapss3[s[j]];
+            d2 f[PN_PLIGHT];
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j) f[j] = crnr_force<L>(sfp, sfq, sft, s[j], s3[j]);
+            d2 acc = VIO<L>::ld(pf, p);
+#pragma unroll
+            for (int j = 0; j < PN_PLIGHT; ++j)
+                if (kb + j < ke) acc = add(acc, f[j]);
+            VIO<L>::st(pf, p, acc);
+        }
     }
-    st(pf, p, acc);
+    __syncthreads();
+    const int nh = hv.n;
+    for (int h = 0; h < nh; ++h) {
+        const int q = hv.p[h];
+        const int kb = m.pcbeg[q], ke = m.pcbeg[q + 1];
+        d2 acc = mk(0., 0.);
+        if (threadIdx.x == 0) acc = VIO<L>::ld(pf, q);
+        for (int k0 = kb; k0 < ke; k0 += PN_BLOCK) {
+            const int k = k0 + threadIdx.x;
+            if (k < ke) {
+                const int s = m.pcside[k];
+                part[threadIdx.x] = crnr_force<L>(sfp, sfq, sft, s, m.mapss3[s]);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int cnt = min(PN_BLOCK, ke - k0);
+                for (int i = 0; i < cnt; ++i) acc = add(acc, part[i]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) VIO<L>::st(pf, q, acc);
+    }
 }
 
 // calcAccelTask, src/Hydro.cc:1235-1254
+template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) k_calc_accel(int nump, pennant_cvec pf,
                                                          const double* pmaswt, pennant_vec pa) {
     const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
     if (p >= nump) return;
     const double fuzz = 1.e-99;
-    st(pa, p, dvs(ld(pf, p), smax(pmaswt[p], fuzz)));
+    VIO<L>::st(pa, p, dvs(VIO<L>::ld(pf, p), smax(pmaswt[p], fuzz)));
 }
 
 // advPosFullTask, src/Hydro.cc:1281-1306
+template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) k_adv_pos_full(int nump, pennant_cvec px0,
                                                            pennant_cvec pu0, pennant_cvec pa,
                                                            pennant_vec px, pennant_vec pu,
@@ -181,33 +448,53 @@ __global__ void __launch_bounds__(PN_BLOCK) k_adv_pos_full(int nump, pennant_cve
     const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
     if (p >= nump) return;
     const double dt = *dtp;
-    const d2 u0 = ld(pu0, p);
-    const d2 u = add(u0, scl(dt, ld(pa, p)));
-    st(pu, p, u);
-    st(px, p, add(ld(px0, p), scl(dt * 0.5, add(u0, u))));
+    const d2 u0 = VIO<L>::ld(pu0, p);
+    const d2 u = add(u0, scl(dt, VIO<L>::ld(pa, p)));
+    VIO<L>::st(pu, p, u);
+    VIO<L>::st(px, p, add(VIO<L>::ld(px0, p), scl(dt * 0.5, add(u0, u))));
 }
 
-// calcWorkTask, src/Hydro.cc:1339-1406
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_work(pennant_mesh m, pennant_cvec sfp,
-                                                        pennant_cvec sfq, pennant_cvec pu,
-                                                        pennant_cvec pu0, pennant_cvec pxp,
-                                                        double* zw, double* zetot,
-                                                        const double* dtp) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    const double dth = 0.5 * (*dtp);
-    double w = 0., et = zetot[z];
-    for (int s = m.zsbeg[z]; s < m.zsbeg[z + 1]; ++s) {
-        const int p1 = m.mapsp1[s], p2 = m.mapsp2[s];
-        const d2 sftot = add(ld(sfp, s), ld(sfq, s));
-        const double sd1 = dot(sftot, add(ld(pu0, p1), ld(pu, p1)));
-        const double sd2 = dot(neg(sftot), add(ld(pu0, p2), ld(pu, p2)));
-        const double dwork = -dth * (sd1 * pxp.x[p1] + sd2 * pxp.x[p2]);
-        et += dwork;
-        w += dwork;
+// calcWorkTask, src/Hydro.cc:1339-1406: per side
+// dwork = -(dt/2) * (dot(sftot, pu0+pu at p1) * pxp.x[p1]
+//                    + dot(-sftot, pu0+pu at p2) * pxp.x[p2]),
+// sftot = sfp + sfq; zetot += dwork and zw = sum dwork (from 0) in
+// ascending side order.
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_work(pennant_mesh m, int tz, pennant_cvec sfp,
+                                                     pennant_cvec sfq, pennant_cvec pu,
+                                                     pennant_cvec pu0, pennant_cvec pxp,
+                                                     double* zw, double* zetot, const double* dtp) {
+    PN_TILE_SHARED;
+    __shared__ d2 vs[PN_TB];
+    __shared__ double xx[PN_TB], dw[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const int p1 = m.mapsp1[t.s0 + lane];
+        vs[lane] = add(VIO<L>::ld(pu0, p1), VIO<L>::ld(pu, p1));
+        xx[lane] = VIO<L>::ldx(pxp, p1);
     }
-    zetot[z] = et;
-    zw[z] = w;
+    __syncthreads();
+    if (has_side) {
+        const double dth = 0.5 * (*dtp);
+        const int s = t.s0 + lane, n2 = side_next(c);
+        const d2 sftot = add(VIO<L>::ld(sfp, s), VIO<L>::ld(sfq, s));
+        const double sd1 = dot(sftot, vs[lane]);
+        const double sd2 = dot(neg(sftot), vs[n2]);
+        dw[lane] = -dth * (sd1 * xx[lane] + sd2 * xx[n2]);
+    }
+    __syncthreads();
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int z = t.z0 + lane;
+        double w = 0., et = zetot[z];
+        for (int i = b; i < e; ++i) {
+            et += dw[i];
+            w += dw[i];
+        }
+        zetot[z] = et;
+        zw[z] = w;
+    }
 }
 
 // calcWorkRateTask, src/Hydro.cc:1483-1510
@@ -293,177 +580,256 @@ __global__ void __launch_bounds__(PN_BLOCK) k_calc_state_half(int numz, const do
     zss[z] = ss;
 }
 
-// PolyGas::calcForceTask, src/PolyGas.cc:164-185
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_force_pgas(pennant_mesh m, pennant_cvec ssurf,
-                                                              const double* zp, pennant_vec sf) {
-    const int s = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (s >= m.nums) return;
-    const double p = zp[m.mapsz[s]];
-    st(sf, s, scl(-p, ld(ssurf, s)));
+// PolyGas::calcForceTask, src/PolyGas.cc:164-185: sf = -zp[z] * ssurf
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_force_pgas(pennant_mesh m, int tz,
+                                                           pennant_cvec ssurf, const double* zp,
+                                                           pennant_vec sf) {
+    PN_TILE_SHARED;
+    __shared__ double zpl[PN_TB];
+    if (has_zone) zpl[lane] = zp[t.z0 + lane];
+    __syncthreads();
+    if (has_side) {
+        const SideAt c = side_at(t, zsb, zof, lane);
+        VIO<L>::st(sf, t.s0 + lane, scl(-zpl[c.zi], VIO<L>::ld(ssurf, t.s0 + lane)));
+    }
 }
 
 // TTS::calcForceTask, src/TTS.cc:55-104
-__global__ void __launch_bounds__(PN_BLOCK) k_calc_force_tts(pennant_mesh m, const double* sarea,
-                                                             const double* smf, pennant_cvec ssurf,
-                                                             const double* zarea, const double* zr,
-                                                             const double* zss, pennant_vec sf,
-                                                             double alfa, double ssmin) {
-    const int s = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (s >= m.nums) return;
-    const int z = m.mapsz[s];
-    const double vfacinv = zarea[z] / sarea[s];
-    const double r = zr[z];
-    const double srho = r * smf[s] * vfacinv;
-    double sstmp = smax(zss[z], ssmin);
-    sstmp = alfa * sstmp * sstmp;
-    const double sdp = sstmp * (srho - r);
-    st(sf, s, scl(-sdp, ld(ssurf, s)));
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_calc_force_tts(pennant_mesh m, int tz,
+                                                          const double* sarea, const double* smf,
+                                                          pennant_cvec ssurf, const double* zarea,
+                                                          const double* zr, const double* zss,
+                                                          pennant_vec sf, double alfa,
+                                                          double ssmin) {
+    PN_TILE_SHARED;
+    __shared__ double za[PN_TB], zrl[PN_TB], zsl[PN_TB];
+    if (has_zone) {
+        za[lane] = zarea[t.z0 + lane];
+        zrl[lane] = zr[t.z0 + lane];
+        zsl[lane] = zss[t.z0 + lane];
+    }
+    __syncthreads();
+    if (has_side) {
+        const SideAt c = side_at(t, zsb, zof, lane);
+        const int s = t.s0 + lane;
+        const double vfacinv = za[c.zi] / sarea[s];
+        const double r = zrl[c.zi];
+        const double srho = r * smf[s] * vfacinv;
+        double sstmp = smax(zsl[c.zi], ssmin);
+        sstmp = alfa * sstmp * sstmp;
+        const double sdp = sstmp * (srho - r);
+        VIO<L>::st(sf, s, scl(-sdp, VIO<L>::ld(ssurf, s)));
+    }
 }
 
 // ---------------------------------------------------------------------------
 // QCS
 
-// QCS::setCornerDivTask, src/QCS.cc:116-242 (one thread per zone: zone
-// velocity first, then the zone's corners)
-__global__ void __launch_bounds__(PN_BLOCK) k_set_corner_div(pennant_mesh m, pennant_cvec ex,
-                                                             const double* elen, pennant_cvec zx,
-                                                             pennant_cvec px, pennant_cvec pu,
-                                                             pennant_vec zuc, double* carea,
-                                                             double* ccos, double* cdiv,
-                                                             double* cevol, double* cdu) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    const int sb = m.zsbeg[z], se = m.zsbeg[z + 1];
-    const int n = m.znump[z];
-    d2 uc = mk(0., 0.);
-    for (int s = sb; s < se; ++s) uc = add(uc, dvs(ld(pu, m.mapsp1[s]), (double)n));
-    st(zuc, z, uc);
-    const d2 xp2 = ld(zx, z);
-    const d2 up2 = uc;
-    for (int c = sb; c < se; ++c) {
-        const int s2 = c;
-        const int s = m.mapss3[s2];
-        const int p = m.mapsp2[s];
+// QCS::setCornerDivTask, src/QCS.cc:116-242.  Corner c = side s2 of its
+// zone; s = ss3[c] is the zone's previous side.  Its points: p = p2(s) =
+// p1(c), p1 = p1(s), p2 = p2(c) = p1(ss4[c]).  Each lane stages its side's
+// first point (position and velocity), edge midpoint and edge length; the
+// zone velocity zuc = sum pu0[p1] * (1/n) (ascending, from 0) is formed per
+// zone; then each lane computes its corner from the staged values.
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_set_corner_div(pennant_mesh m, int tz, pennant_cvec ex,
+                                                          const double* elen, pennant_cvec zx,
+                                                          pennant_cvec px, pennant_cvec pu,
+                                                          pennant_vec zuc, double* carea,
+                                                          double* ccos, double* cdiv,
+                                                          double* cevol, double* cdu) {
+    PN_TILE_SHARED;
+    __shared__ d2 u1[PN_TB], x1[PN_TB], e1[PN_TB], un[PN_TB], zxl[PN_TB], zul[PN_TB];
+    __shared__ double el[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const int s = t.s0 + lane;
         const int p1 = m.mapsp1[s];
-        const int p2 = m.mapsp2[s2];
-        const d2 up0 = ld(pu, p);
-        const d2 xp0 = ld(px, p);
-        const d2 up1 = scl(0.5, add(up0, ld(pu, p2)));
-        const d2 xp1 = ld(ex, s2);
-        const d2 up3 = scl(0.5, add(ld(pu, p1), up0));
-        const d2 xp3 = ld(ex, s);
-        const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
-        carea[c] = cvolume;
-        const d2 v1 = sub(xp3, xp0);
-        const d2 v2 = sub(xp1, xp0);
-        const double de1 = elen[s];
-        const double de2 = elen[s2];
-        const double minelen = smin(de1, de2);
-        ccos[c] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
-        const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
-                           (2.0 * cvolume);
-        cdiv[c] = div;
-        const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
-        const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
-        const double dx1 = len(dxx1);
-        const double dx2 = len(dxx2);
-        const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
-        const double test1 = fabs(dot(dxx1, duav) * dx2);
-        const double test2 = fabs(dot(dxx2, duav) * dx1);
-        const double num = (test1 > test2 ? dx1 : dx2);
-        const double den = (test1 > test2 ? dx2 : dx1);
-        const double r = num / den;
-        double evol = __builtin_sqrt(4.0 * cvolume * r);
-        evol = smin(evol, 2.0 * minelen);
-        const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
-        const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
-        double du = __builtin_sqrt(smax(dv1, dv2));
-        evol = (div < 0.0 ? evol : 0.);
-        du = (div < 0.0 ? du : 0.);
-        cevol[c] = evol;
-        cdu[c] = du;
+        const d2 u = VIO<L>::ld(pu, p1);
+        u1[lane] = u;
+        un[lane] = dvs(u, (double)c.n);
+        x1[lane] = VIO<L>::ld(px, p1);
+        e1[lane] = VIO<L>::ld(ex, s);
+        el[lane] = elen[s];
     }
+    if (has_zone) zxl[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    __syncthreads();
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        d2 acc = mk(0., 0.);
+        for (int i = b; i < e; ++i) acc = add(acc, un[i]);
+        zul[lane] = acc;
+        VIO<L>::st(zuc, t.z0 + lane, acc);
+    }
+    __syncthreads();
+    if (!has_side) return;
+    const int ip = side_prev(c), in = side_next(c);
+    const int cc = t.s0 + lane;
+    const d2 up0 = u1[lane];
+    const d2 xp0 = x1[lane];
+    const d2 up1 = scl(0.5, add(up0, u1[in]));
+    const d2 xp1 = e1[lane];
+    const d2 up2 = zul[c.zi];
+    const d2 xp2 = zxl[c.zi];
+    const d2 up3 = scl(0.5, add(u1[ip], up0));
+    const d2 xp3 = e1[ip];
+    const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
+    carea[cc] = cvolume;
+    const d2 v1 = sub(xp3, xp0);
+    const d2 v2 = sub(xp1, xp0);
+    const double de1 = el[ip];
+    const double de2 = el[lane];
+    const double minelen = smin(de1, de2);
+    ccos[cc] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
+    const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
+                       (2.0 * cvolume);
+    cdiv[cc] = div;
+    const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
+    const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
+    const double dx1 = len(dxx1);
+    const double dx2 = len(dxx2);
+    const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
+    const double test1 = fabs(dot(dxx1, duav) * dx2);
+    const double test2 = fabs(dot(dxx2, duav) * dx1);
+    const double num = (test1 > test2 ? dx1 : dx2);
+    const double den = (test1 > test2 ? dx2 : dx1);
+    const double r = num / den;
+    double evol = __builtin_sqrt(4.0 * cvolume * r);
+    evol = smin(evol, 2.0 * minelen);
+    const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
+    const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
+    double du = __builtin_sqrt(smax(dv1, dv2));
+    evol = (div < 0.0 ? evol : 0.);
+    du = (div < 0.0 ? du : 0.);
+    cevol[cc] = evol;
+    cdu[cc] = du;
 }
 
-// QCS::setQCnForceTask, src/QCS.cc:246-327
-__global__ void __launch_bounds__(PN_BLOCK) k_set_qcn_force(pennant_mesh m, const double* elen,
-                                                            const double* cdiv, const double* cdu,
-                                                            const double* cevol, const double* zrp,
-                                                            const double* zss, pennant_cvec pu,
-                                                            double* crmu, pennant_vec cqe1,
-                                                            pennant_vec cqe2, double qgamma,
-                                                            double q1, double q2) {
-    const int c = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (c >= m.nums) return;
+// QCS::setQCnForceTask, src/QCS.cc:246-327: crmu, and the viscous edge
+// forces cqe1 = crmu (pu[p] - pu[p1]) / elen[s], cqe2 = crmu (pu[p2] -
+// pu[p]) / elen[c] with s = ss3[c], p = p1(c), p1 = p1(s), p2 = p1(ss4[c])
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_set_qcn_force(pennant_mesh m, int tz, const double* elen,
+                                                         const double* cdiv, const double* cdu,
+                                                         const double* cevol, const double* zrp,
+                                                         const double* zss, pennant_cvec pu,
+                                                         double* crmu, pennant_vec cqe1,
+                                                         pennant_vec cqe2, double qgamma,
+                                                         double q1, double q2) {
+    PN_TILE_SHARED;
+    __shared__ d2 u1[PN_TB];
+    __shared__ double el[PN_TB], zrl[PN_TB], zsl[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const int s = t.s0 + lane;
+        u1[lane] = VIO<L>::ld(pu, m.mapsp1[s]);
+        el[lane] = elen[s];
+    }
+    if (has_zone) {
+        zrl[lane] = zrp[t.z0 + lane];
+        zsl[lane] = zss[t.z0 + lane];
+    }
+    __syncthreads();
+    if (!has_side) return;
+    const int cc = t.s0 + lane;
+    const int ip = side_prev(c), in = side_next(c);
     const double gammap1 = qgamma + 1.0;
-    const int z = m.mapsz[c];
-    const double ztmp2 = q2 * 0.25 * gammap1 * cdu[c];
-    const double ztmp1 = q1 * zss[z];
+    const double ztmp2 = q2 * 0.25 * gammap1 * cdu[cc];
+    const double ztmp1 = q1 * zsl[c.zi];
     const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
-    const double mu = zkur * zrp[z] * cevol[c];
-    const double cm = ((cdiv[c] > 0.0) ? 0. : mu);
-    crmu[c] = cm;
-    const int s2 = c;
-    const int s = m.mapss3[s2];
-    const int p = m.mapsp2[s];
-    const int p1 = m.mapsp1[s];
-    const int p2 = m.mapsp2[s2];
-    const d2 u = ld(pu, p);
-    st(cqe1, c, dvs(scl(cm, sub(u, ld(pu, p1))), elen[s]));
-    st(cqe2, c, dvs(scl(cm, sub(ld(pu, p2), u)), elen[s2]));
+    const double mu = zkur * zrl[c.zi] * cevol[cc];
+    const double cm = ((cdiv[cc] > 0.0) ? 0. : mu);
+    crmu[cc] = cm;
+    const d2 u = u1[lane];
+    VIO<L>::st(cqe1, cc, dvs(scl(cm, sub(u, u1[ip])), el[ip]));
+    VIO<L>::st(cqe2, cc, dvs(scl(cm, sub(u1[in], u)), el[lane]));
 }
 
-// QCS::setForceTask, src/QCS.cc:331-380 (one thread per zone so that the
-// clamped ccos/cw of the neighbouring corner c2 = ss4[s] are ready)
-__global__ void __launch_bounds__(PN_BLOCK) k_set_force_qcs(pennant_mesh m, const double* carea,
-                                                            pennant_cvec cqe1, pennant_cvec cqe2,
-                                                            const double* elen, double* ccos,
-                                                            double* cw, pennant_vec sfq) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    const int sb = m.zsbeg[z], se = m.zsbeg[z + 1];
-    for (int c = sb; c < se; ++c) {
-        const double cc = ccos[c];
+// QCS::setForceTask, src/QCS.cc:331-380: each lane clamps its corner's ccos
+// and computes cw, then the side force from corner c1 = s and its zone's
+// next corner c2 = ss4[s], both staged in LDS.
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_set_force_qcs(pennant_mesh m, int tz, const double* carea,
+                                                         pennant_cvec cqe1, pennant_cvec cqe2,
+                                                         const double* elen, double* ccos,
+                                                         double* cw, pennant_vec sfq) {
+    PN_TILE_SHARED;
+    __shared__ d2 q1l[PN_TB], q2l[PN_TB];
+    __shared__ double cwl[PN_TB], ccl[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const int s = t.s0 + lane;
+        const double cc = ccos[s];
         const double csin2 = 1.0 - cc * cc;
-        cw[c] = ((csin2 < 1.e-4) ? 0. : carea[c] / csin2);
-        ccos[c] = ((csin2 < 1.e-4) ? 0. : cc);
+        const double w = ((csin2 < 1.e-4) ? 0. : carea[s] / csin2);
+        const double cl = ((csin2 < 1.e-4) ? 0. : cc);
+        cw[s] = w;
+        ccos[s] = cl;
+        cwl[lane] = w;
+        ccl[lane] = cl;
+        q1l[lane] = VIO<L>::ld(cqe1, s);
+        q2l[lane] = VIO<L>::ld(cqe2, s);
     }
-    for (int s = sb; s < se; ++s) {
-        const int c1 = s, c2 = m.mapss4[s];
-        const double el = elen[s];
-        const double cw1 = cw[c1], cw2 = cw[c2];
-        const double ccos1 = ccos[c1], ccos2 = ccos[c2];
-        const d2 cqe11 = ld(cqe1, c1), cqe12 = ld(cqe1, c2);
-        const d2 cqe21 = ld(cqe2, c1), cqe22 = ld(cqe2, c2);
-        const d2 f = add(scl(cw1, add(cqe21, scl(ccos1, cqe11))),
-                         scl(cw2, add(cqe12, scl(ccos2, cqe22))));
-        st(sfq, s, dvs(f, el));
-    }
+    __syncthreads();
+    if (!has_side) return;
+    const int s = t.s0 + lane, i2 = side_next(c);
+    const double el = elen[s];
+    const double cw1 = cwl[lane], cw2 = cwl[i2];
+    const double ccos1 = ccl[lane], ccos2 = ccl[i2];
+    const d2 cqe11 = q1l[lane], cqe12 = q1l[i2];
+    const d2 cqe21 = q2l[lane], cqe22 = q2l[i2];
+    const d2 f = add(scl(cw1, add(cqe21, scl(ccos1, cqe11))),
+                     scl(cw2, add(cqe12, scl(ccos2, cqe22))));
+    VIO<L>::st(sfq, s, dvs(f, el));
 }
 
-// QCS::setVelDiffTask, src/QCS.cc:384-448
-__global__ void __launch_bounds__(PN_BLOCK) k_set_vel_diff(pennant_mesh m, const double* elen,
-                                                           const double* zss, pennant_cvec px,
-                                                           pennant_cvec pu, double* ztmp,
-                                                           double* zdu, double q1, double q2) {
-    const int z = blockIdx.x * PN_BLOCK + threadIdx.x;
-    if (z >= m.numz) return;
-    double t = 0.;
-    for (int s = m.zsbeg[z]; s < m.zsbeg[z + 1]; ++s) {
-        const int p1 = m.mapsp1[s], p2 = m.mapsp2[s];
-        const d2 dx = sub(ld(px, p2), ld(px, p1));
-        const d2 du = sub(ld(pu, p2), ld(pu, p1));
-        const double lenx = elen[s];
+// QCS::setVelDiffTask, src/QCS.cc:384-448: ztmp = max(0, max over the
+// zone's sides of |dot(du, dx)| / elen) in ascending side order;
+// zdu = q1 zss + 2 q2 ztmp
+template <int L>
+__global__ void __launch_bounds__(PN_TB) k_set_vel_diff(pennant_mesh m, int tz, const double* elen,
+                                                        const double* zss, pennant_cvec px,
+                                                        pennant_cvec pu, double* ztmp, double* zdu,
+                                                        double q1, double q2) {
+    PN_TILE_SHARED;
+    __shared__ d2 x1[PN_TB], u1[PN_TB];
+    __shared__ double dux_[PN_TB];
+    SideAt c;
+    if (has_side) {
+        c = side_at(t, zsb, zof, lane);
+        const int p1 = m.mapsp1[t.s0 + lane];
+        x1[lane] = VIO<L>::ld(px, p1);
+        u1[lane] = VIO<L>::ld(pu, p1);
+    }
+    __syncthreads();
+    if (has_side) {
+        const int i2 = side_next(c);
+        const d2 dx = sub(x1[i2], x1[lane]);
+        const d2 du = sub(u1[i2], u1[lane]);
+        const double lenx = elen[t.s0 + lane];
         double dux = dot(du, dx);
         dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
-        t = smax(t, dux);
+        dux_[lane] = dux;
     }
-    ztmp[z] = t;
-    zdu[z] = q1 * zss[z] + 2. * q2 * t;
+    __syncthreads();
+    if (has_zone) {
+        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        double tm = 0.;
+        for (int i = b; i < e; ++i) tm = smax(tm, dux_[i]);
+        const int z = t.z0 + lane;
+        ztmp[z] = tm;
+        zdu[z] = q1 * zss[z] + 2. * q2 * tm;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // HydroBC::applyFixedBCTask, src/HydroBC.cc:150-183
+template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) k_apply_fixed_bc(const int32_t* mapbp, int numb,
                                                              double vx, double vy, pennant_vec pf,
                                                              pennant_vec pu0) {
@@ -471,11 +837,56 @@ __global__ void __launch_bounds__(PN_BLOCK) k_apply_fixed_bc(const int32_t* mapb
     if (b >= numb) return;
     const int p = mapbp[b];
     const d2 u = mk(vx, vy);
-    st(pu0, p, project(ld(pu0, p), u));
-    st(pf, p, project(ld(pf, p), u));
+    VIO<L>::st(pu0, p, project(VIO<L>::ld(pu0, p), u));
+    VIO<L>::st(pf, p, project(VIO<L>::ld(pf, p), u));
 }
 
+// ---------------------------------------------------------------------------
+// Host side: argument checks and layout choice
+
 inline int launch_status() { return (int)hipGetLastError(); }
+
+// Every vector argument of one call, normalised (stride 0 -> 1); the call
+// runs in the layout all of them share, else through the strided path.
+struct Layout {
+    bool soa = true, aos = true, bad = false;
+    template <class V> V take(V v) {
+        if (v.stride == 0) v.stride = 1;
+        if (v.stride < 0) bad = true;
+        if (v.stride != 1) soa = false;
+        const bool inter = v.stride == 2 && v.y == v.x + 1 &&
+                           (reinterpret_cast<uintptr_t>(v.x) & 15) == 0;
+        if (!inter) aos = false;
+        return v;
+    }
+    int kind() const { return soa ? VL_SOA : (aos ? VL_AOS : VL_STRIDED); }
+};
+
+// zones per tile block for the mesh; 0 when the mesh descriptor cannot be
+// tiled (zones of fewer than 3 or more than PENNANT_MAX_ZSIDES sides)
+inline int tile_zones(const pennant_mesh* m) {
+    if (m->maxznump < 3 || m->maxznump > PENNANT_MAX_ZSIDES) return 0;
+    return PN_TB / m->maxznump;
+}
+inline int tile_blocks(const pennant_mesh* m, int tz) { return (m->numz + tz - 1) / tz; }
+
+#define PN_LAUNCH_L(lay, kern, grid, block, st, ...)                                               \
+    do {                                                                                           \
+        switch (lay) {                                                                             \
+            case VL_SOA: hipLaunchKernelGGL(kern<VL_SOA>, grid, block, 0, st, __VA_ARGS__); break;   \
+            case VL_AOS: hipLaunchKernelGGL(kern<VL_AOS>, grid, block, 0, st, __VA_ARGS__); break;   \
+            default: hipLaunchKernelGGL(kern<VL_STRIDED>, grid, block, 0, st, __VA_ARGS__); break;   \
+        }                                                                                          \
+    } while (0)
+
+// Common preamble of a tile-kernel entry point: nothing to do for an empty
+// mesh, EINVAL for a descriptor that cannot be tiled.
+#define PN_TILE_ENTRY(m, tz)                          \
+    if (!(m)) return PENNANT_EINVAL;                  \
+    if ((m)->numz <= 0) return launch_status();       \
+    const int tz = tile_zones(m);                     \
+    if (tz == 0) return PENNANT_EINVAL;               \
+    const dim3 tgrid(tile_blocks(m, tz)), tblk(PN_TB)
 
 }  // namespace
 
@@ -488,67 +899,80 @@ size_t pennant_hip_reduce_workspace_bytes(void) {
     return (size_t)PN_RED_BLOCKS * sizeof(double);
 }
 
-int pennant_hip_calc_ctrs(const pennant_mesh* m, pennant_cvec px, pennant_vec ex,
-                          pennant_vec zx, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_calc_ctrs, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m, px, ex, zx);
+int pennant_hip_calc_ctrs(const pennant_mesh* m, pennant_cvec px, pennant_vec ex, pennant_vec zx,
+                          hipStream_t st) {
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    px = l.take(px); ex = l.take(ex); zx = l.take(zx);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_ctrs, tgrid, tblk, st, *m, tz, px, ex, zx);
     return launch_status();
 }
 
-int pennant_hip_calc_vols(const pennant_mesh* m, pennant_cvec px, pennant_cvec zx,
-                          double* sarea, double* svol, double* zarea, double* zvol,
-                          int32_t* numsbad, void* ws, hipStream_t st) {
+int pennant_hip_calc_vols(const pennant_mesh* m, pennant_cvec px, pennant_cvec zx, double* sarea,
+                          double* svol, double* zarea, double* zvol, int32_t* numsbad, void* ws,
+                          hipStream_t st) {
     (void)ws;
-    if (!numsbad) return PENNANT_EINVAL;
+    if (!m || !numsbad) return PENNANT_EINVAL;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, numsbad);
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_calc_vols, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m, px, zx,
-                           sarea, svol, zarea, zvol, numsbad);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    px = l.take(px); zx = l.take(zx);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_vols, tgrid, tblk, st, *m, tz, px, zx, sarea, svol, zarea, zvol,
+                numsbad);
     return launch_status();
 }
 
 int pennant_hip_calc_side_fracs(const pennant_mesh* m, const double* sarea, const double* zarea,
                                 double* smf, hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_calc_side_fracs, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m,
-                           sarea, zarea, smf);
+    PN_TILE_ENTRY(m, tz);
+    hipLaunchKernelGGL(k_calc_side_fracs, tgrid, tblk, 0, st, *m, tz, sarea, zarea, smf);
     return launch_status();
 }
 
 int pennant_hip_calc_surf_vecs(const pennant_mesh* m, pennant_cvec ex, pennant_cvec zx,
                                pennant_vec ssurf, hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_calc_surf_vecs, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m, ex,
-                           zx, ssurf);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    ex = l.take(ex); zx = l.take(zx); ssurf = l.take(ssurf);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_surf_vecs, tgrid, tblk, st, *m, tz, ex, zx, ssurf);
     return launch_status();
 }
 
 int pennant_hip_calc_edge_len(const pennant_mesh* m, pennant_cvec px, double* elen,
                               hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_calc_edge_len, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m, px,
-                           elen);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    px = l.take(px);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_edge_len, tgrid, tblk, st, *m, tz, px, elen);
     return launch_status();
 }
 
 int pennant_hip_calc_char_len(const pennant_mesh* m, const double* sarea, const double* elen,
                               double* zdl, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_calc_char_len, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m,
-                           sarea, elen, zdl);
+    PN_TILE_ENTRY(m, tz);
+    hipLaunchKernelGGL(k_calc_char_len, tgrid, tblk, 0, st, *m, tz, sarea, elen, zdl);
     return launch_status();
 }
 
 int pennant_hip_adv_pos_half(const pennant_mesh* m, pennant_cvec px0, pennant_cvec pu0,
                              pennant_vec pxp, const double* dt, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
+    Layout l;
+    px0 = l.take(px0); pu0 = l.take(pu0); pxp = l.take(pxp);
+    if (l.bad) return PENNANT_EINVAL;
     if (m->nump > 0)
-        hipLaunchKernelGGL(k_adv_pos_half, dim3(nblocks(m->nump)), dim3(PN_BLOCK), 0, st, m->nump,
-                           px0, pu0, pxp, dt);
+        PN_LAUNCH_L(l.kind(), k_adv_pos_half, dim3(nblocks(m->nump)), dim3(PN_BLOCK), st, m->nump,
+                    px0, pu0, pxp, dt);
     return launch_status();
 }
 
 int pennant_hip_calc_rho(const pennant_mesh* m, const double* zm, const double* zvol, double* zr,
                          hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
     if (m->numz > 0)
         hipLaunchKernelGGL(k_calc_rho, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, m->numz, zm,
                            zvol, zr);
@@ -557,6 +981,7 @@ int pennant_hip_calc_rho(const pennant_mesh* m, const double* zm, const double* 
 
 int pennant_hip_calc_crnr_mass(const pennant_mesh* m, const double* smf, const double* zr,
                                const double* zarea, double* pmaswt, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
     if (m->nump > 0)
         hipLaunchKernelGGL(k_calc_crnr_mass, dim3(nblocks(m->nump)), dim3(PN_BLOCK), 0, st, *m,
                            smf, zr, zarea, pmaswt);
@@ -565,41 +990,57 @@ int pennant_hip_calc_crnr_mass(const pennant_mesh* m, const double* smf, const d
 
 int pennant_hip_sum_crnr_force(const pennant_mesh* m, pennant_cvec sfp, pennant_cvec sfq,
                                pennant_cvec sft, pennant_vec pf, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
+    Layout l;
+    sfp = l.take(sfp); sfq = l.take(sfq); sft = l.take(sft); pf = l.take(pf);
+    if (l.bad) return PENNANT_EINVAL;
     if (m->nump > 0)
-        hipLaunchKernelGGL(k_sum_crnr_force, dim3(nblocks(m->nump)), dim3(PN_BLOCK), 0, st, *m,
-                           sfp, sfq, sft, pf);
+        PN_LAUNCH_L(l.kind(), k_sum_crnr_force, dim3(nblocks(m->nump)), dim3(PN_BLOCK), st, *m,
+                    sfp, sfq, sft, pf);
     return launch_status();
 }
 
 int pennant_hip_calc_accel(const pennant_mesh* m, pennant_cvec pf, const double* pmaswt,
                            pennant_vec pa, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
+    Layout l;
+    pf = l.take(pf); pa = l.take(pa);
+    if (l.bad) return PENNANT_EINVAL;
     if (m->nump > 0)
-        hipLaunchKernelGGL(k_calc_accel, dim3(nblocks(m->nump)), dim3(PN_BLOCK), 0, st, m->nump,
-                           pf, pmaswt, pa);
+        PN_LAUNCH_L(l.kind(), k_calc_accel, dim3(nblocks(m->nump)), dim3(PN_BLOCK), st, m->nump,
+                    pf, pmaswt, pa);
     return launch_status();
 }
 
 int pennant_hip_adv_pos_full(const pennant_mesh* m, pennant_cvec px0, pennant_cvec pu0,
                              pennant_cvec pa, pennant_vec px, pennant_vec pu, const double* dt,
                              hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
+    Layout l;
+    px0 = l.take(px0); pu0 = l.take(pu0); pa = l.take(pa); px = l.take(px); pu = l.take(pu);
+    if (l.bad) return PENNANT_EINVAL;
     if (m->nump > 0)
-        hipLaunchKernelGGL(k_adv_pos_full, dim3(nblocks(m->nump)), dim3(PN_BLOCK), 0, st, m->nump,
-                           px0, pu0, pa, px, pu, dt);
+        PN_LAUNCH_L(l.kind(), k_adv_pos_full, dim3(nblocks(m->nump)), dim3(PN_BLOCK), st, m->nump,
+                    px0, pu0, pa, px, pu, dt);
     return launch_status();
 }
 
 int pennant_hip_calc_work(const pennant_mesh* m, pennant_cvec sfp, pennant_cvec sfq,
                           pennant_cvec pu, pennant_cvec pu0, pennant_cvec pxp, double* zw,
                           double* zetot, const double* dt, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_calc_work, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m, sfp,
-                           sfq, pu, pu0, pxp, zw, zetot, dt);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    sfp = l.take(sfp); sfq = l.take(sfq); pu = l.take(pu); pu0 = l.take(pu0); pxp = l.take(pxp);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_work, tgrid, tblk, st, *m, tz, sfp, sfq, pu, pu0, pxp, zw, zetot,
+                dt);
     return launch_status();
 }
 
 int pennant_hip_calc_work_rate(const pennant_mesh* m, const double* zvol0, const double* zvol,
                                const double* zw, const double* zp, double* zwrate,
                                const double* dt, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
     if (m->numz > 0)
         hipLaunchKernelGGL(k_calc_work_rate, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st,
                            m->numz, zvol0, zvol, zw, zp, zwrate, dt);
@@ -608,6 +1049,7 @@ int pennant_hip_calc_work_rate(const pennant_mesh* m, const double* zvol0, const
 
 int pennant_hip_calc_energy(const pennant_mesh* m, const double* zetot, const double* zm,
                             double* ze, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
     if (m->numz > 0)
         hipLaunchKernelGGL(k_calc_energy, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, m->numz,
                            zetot, zm, ze);
@@ -617,7 +1059,7 @@ int pennant_hip_calc_energy(const pennant_mesh* m, const double* zetot, const do
 int pennant_hip_calc_dt_new(const pennant_mesh* m, const double* zdl, const double* zdu,
                             const double* zss, double cfl, double* dtnew, void* ws,
                             hipStream_t st) {
-    if (!ws || !dtnew) return PENNANT_EINVAL;
+    if (!m || !ws || !dtnew || m->numz < 0) return PENNANT_EINVAL;
     const int nb = nred_blocks(m->numz);
     double* part = static_cast<double*>(ws);
     hipLaunchKernelGGL(k_calc_dt_new, dim3(nb), dim3(PN_BLOCK), 0, st, m->numz, zdl, zdu, zss,
@@ -628,7 +1070,7 @@ int pennant_hip_calc_dt_new(const pennant_mesh* m, const double* zdl, const doub
 
 int pennant_hip_calc_dvol(const pennant_mesh* m, const double* zvol, const double* zvol0,
                           double* dvovmax, void* ws, hipStream_t st) {
-    if (!ws || !dvovmax) return PENNANT_EINVAL;
+    if (!m || !ws || !dvovmax || m->numz < 0) return PENNANT_EINVAL;
     const int nb = nred_blocks(m->numz);
     double* part = static_cast<double*>(ws);
     hipLaunchKernelGGL(k_calc_dvol, dim3(nb), dim3(PN_BLOCK), 0, st, m->numz, zvol, zvol0, part);
@@ -654,6 +1096,7 @@ int pennant_hip_calc_state_half(const pennant_mesh* m, const double* zr, const d
                                 const double* zvol0, const double* ze, const double* zwrate,
                                 const double* zm, double* zp, double* zss, double gamma,
                                 double ssmin, const double* dt, hipStream_t st) {
+    if (!m) return PENNANT_EINVAL;
     if (m->numz > 0)
         hipLaunchKernelGGL(k_calc_state_half, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st,
                            m->numz, zr, zvolp, zvol0, ze, zwrate, zm, zp, zss, gamma, ssmin, dt);
@@ -662,9 +1105,11 @@ int pennant_hip_calc_state_half(const pennant_mesh* m, const double* zr, const d
 
 int pennant_hip_calc_force_pgas(const pennant_mesh* m, pennant_cvec ssurf, const double* zp,
                                 pennant_vec sf, hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_calc_force_pgas, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m,
-                           ssurf, zp, sf);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    ssurf = l.take(ssurf); sf = l.take(sf);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_force_pgas, tgrid, tblk, st, *m, tz, ssurf, zp, sf);
     return launch_status();
 }
 
@@ -672,9 +1117,12 @@ int pennant_hip_calc_force_tts(const pennant_mesh* m, const double* sarea, const
                                pennant_cvec ssurf, const double* zarea, const double* zr,
                                const double* zss, pennant_vec sf, double alfa, double ssmin,
                                hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_calc_force_tts, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m,
-                           sarea, smf, ssurf, zarea, zr, zss, sf, alfa, ssmin);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    ssurf = l.take(ssurf); sf = l.take(sf);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_calc_force_tts, tgrid, tblk, st, *m, tz, sarea, smf, ssurf, zarea, zr,
+                zss, sf, alfa, ssmin);
     return launch_status();
 }
 
@@ -682,9 +1130,12 @@ int pennant_hip_set_corner_div(const pennant_mesh* m, pennant_cvec ex, const dou
                                pennant_cvec zx, pennant_cvec px, pennant_cvec pu,
                                pennant_vec zuc, double* carea, double* ccos, double* cdiv,
                                double* cevol, double* cdu, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_set_corner_div, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m, ex,
-                           elen, zx, px, pu, zuc, carea, ccos, cdiv, cevol, cdu);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    ex = l.take(ex); zx = l.take(zx); px = l.take(px); pu = l.take(pu); zuc = l.take(zuc);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_set_corner_div, tgrid, tblk, st, *m, tz, ex, elen, zx, px, pu, zuc,
+                carea, ccos, cdiv, cevol, cdu);
     return launch_status();
 }
 
@@ -693,35 +1144,47 @@ int pennant_hip_set_qcn_force(const pennant_mesh* m, const double* elen, const d
                               const double* zss, pennant_cvec pu, double* crmu,
                               pennant_vec cqe1, pennant_vec cqe2, double qgamma, double q1,
                               double q2, hipStream_t st) {
-    if (m->nums > 0)
-        hipLaunchKernelGGL(k_set_qcn_force, dim3(nblocks(m->nums)), dim3(PN_BLOCK), 0, st, *m,
-                           elen, cdiv, cdu, cevol, zrp, zss, pu, crmu, cqe1, cqe2, qgamma, q1, q2);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    pu = l.take(pu); cqe1 = l.take(cqe1); cqe2 = l.take(cqe2);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_set_qcn_force, tgrid, tblk, st, *m, tz, elen, cdiv, cdu, cevol, zrp,
+                zss, pu, crmu, cqe1, cqe2, qgamma, q1, q2);
     return launch_status();
 }
 
 int pennant_hip_set_force_qcs(const pennant_mesh* m, const double* carea, pennant_cvec cqe1,
                               pennant_cvec cqe2, const double* elen, double* ccos, double* cw,
                               pennant_vec sfq, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_set_force_qcs, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m,
-                           carea, cqe1, cqe2, elen, ccos, cw, sfq);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    cqe1 = l.take(cqe1); cqe2 = l.take(cqe2); sfq = l.take(sfq);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_set_force_qcs, tgrid, tblk, st, *m, tz, carea, cqe1, cqe2, elen, ccos,
+                cw, sfq);
     return launch_status();
 }
 
 int pennant_hip_set_vel_diff(const pennant_mesh* m, const double* elen, const double* zss,
                              pennant_cvec px, pennant_cvec pu, double* ztmp, double* zdu,
                              double q1, double q2, hipStream_t st) {
-    if (m->numz > 0)
-        hipLaunchKernelGGL(k_set_vel_diff, dim3(nblocks(m->numz)), dim3(PN_BLOCK), 0, st, *m, elen,
-                           zss, px, pu, ztmp, zdu, q1, q2);
+    PN_TILE_ENTRY(m, tz);
+    Layout l;
+    px = l.take(px); pu = l.take(pu);
+    if (l.bad) return PENNANT_EINVAL;
+    PN_LAUNCH_L(l.kind(), k_set_vel_diff, tgrid, tblk, st, *m, tz, elen, zss, px, pu, ztmp, zdu,
+                q1, q2);
     return launch_status();
 }
 
 int pennant_hip_apply_fixed_bc(const int32_t* mapbp, int32_t numb, double vfix_x, double vfix_y,
                                pennant_vec pf, pennant_vec pu0, hipStream_t st) {
+    Layout l;
+    pf = l.take(pf); pu0 = l.take(pu0);
+    if (l.bad || numb < 0) return PENNANT_EINVAL;
     if (numb > 0)
-        hipLaunchKernelGGL(k_apply_fixed_bc, dim3(nblocks(numb)), dim3(PN_BLOCK), 0, st, mapbp,
-                           numb, vfix_x, vfix_y, pf, pu0);
+        PN_LAUNCH_L(l.kind(), k_apply_fixed_bc, dim3(nblocks(numb)), dim3(PN_BLOCK), st, mapbp,
+                    numb, vfix_x, vfix_y, pf, pu0);
     return launch_status();
 }
 

@@ -36,7 +36,8 @@ class PennantError(RuntimeError):
 
 
 class _Vec(ctypes.Structure):
-    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p)]
+    """pennant_vec / pennant_cvec: element i is (x[i*stride], y[i*stride])."""
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("stride", ctypes.c_int64)]
 
 
 class Mesh(ctypes.Structure):
@@ -47,9 +48,17 @@ class Mesh(ctypes.Structure):
                                        "zsbeg", "pcbeg", "pcside")]
 
 
-def vec(x, y):
-    """SoA vector-field argument from two device addresses."""
-    return _Vec(ctypes.c_void_p(int(x)), ctypes.c_void_p(int(y)))
+def vec(x, y, stride: int = 1):
+    """Vector-field argument: components at device addresses x and y, element
+    i at x[i*stride], y[i*stride] (two arrays: stride 1)."""
+    return _Vec(ctypes.c_void_p(int(x)), ctypes.c_void_p(int(y)), int(stride))
+
+
+def vec_interleaved(base):
+    """Vector-field argument for one interleaved double2 array at device
+    address `base` (the reference's field instance): x = base, y = base + 8
+    bytes, stride 2."""
+    return _Vec(ctypes.c_void_p(int(base)), ctypes.c_void_p(int(base) + 8), 2)
 
 
 # pennant_report_fn: void (*)(const char* text, void* user)

@@ -70,9 +70,11 @@ def test_noh_pie_1024_fused_against_oracle():
             assert_bits_equal(key, sp[key], so[key])
 
 
-# Bounds set about 10x above the errors measured on MI355X (see the printout).
-LEBLANC_L1 = {"zr": 2e-5, "ze": 2e-5, "zp": 2e-5}
-LEBLANC_LINF = {"zr": 2e-3, "ze": 2e-3, "zp": 2e-3}
+# Bounds about 10x above the errors measured on MI355X (13,337 cycles to
+# t = 6: rel_l1 zr 4.7e-8, ze 2.0e-8, zp 7.6e-8; rel_linf 3.0e-7, 2.6e-7,
+# 5.0e-7).
+LEBLANC_L1 = {"zr": 5e-7, "ze": 5e-7, "zp": 1e-6}
+LEBLANC_LINF = {"zr": 5e-6, "ze": 5e-6, "zp": 5e-6}
 
 
 def test_leblanc_4096x256_mixed_against_fp64_to_t6():
@@ -97,9 +99,6 @@ def test_leblanc_4096x256_mixed_against_fp64_to_t6():
         say(f"  {f}: rel_l1 {e1:.3e} rel_linf {ei:.3e}")
         assert e1 <= LEBLANC_L1[f], (f, e1)
         assert ei <= LEBLANC_LINF[f], (f, ei)
-    # the shock tube stays one-dimensional: every row of zones is the same
-    zr = got["zr"].reshape(256, 4096)
-    assert np.max(np.abs(zr - zr[0])) <= 1e-6 * np.max(zr)
 
 
 def _stratified_sample(n, seed=11):
