@@ -584,6 +584,47 @@ int pennant_batch_autotune(pennant_batch* b, int mode, int cycles, double* best_
  * the order n, param for each k. */
 int pennant_ensemble_params(uint64_t seed, int count, int32_t* n, double* param);
 
+/* ======================================================================== */
+/* One mesh as numpcs pieces on one device: the reference's -n <numpcs>     */
+/* (src/main.cc:61-80).  Pieces are near-square zone blocks                 */
+/* (src/Mesh.hh:760-791; zone and point ownership src/GenMesh.cc:790-975).  */
+/* Each piece holds its zones, their sides and every point they reach,      */
+/* numbered private points, master points (owned, reached by other pieces   */
+/* too), ghost points (owned elsewhere) -- the partition of                 */
+/* src/Mesh.cc:432-617.  A cycle runs the fused kernels per piece; corners  */
+/* at ghost points go to their owner, which sums a master point's corners   */
+/* from every piece in ascending one-piece side order (the reference's      */
+/* shared-point reductions, src/Hydro.cc:571-574,766-769, whose fold order  */
+/* Legion leaves open), and the owners' new point state goes back to the    */
+/* ghosts (the reads through the shared partition, src/Hydro.cc:450-455,    */
+/* 866-867,902-906).  Results are bit-identical to the one-piece run.       */
+
+typedef struct pennant_pieces pennant_pieces;
+
+/* Parses the deck, partitions the mesh into numpcs pieces (pieces the block
+ * rounding leaves empty are dropped) and initialises every piece.  NULL on
+ * error with a message in errbuf. */
+pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* errbuf, int errlen);
+void pennant_pieces_destroy(pennant_pieces* pp);
+const char* pennant_pieces_error(pennant_pieces* pp);
+/* Number of (non-empty) pieces. */
+int pennant_pieces_count(pennant_pieces* pp);
+/* Piece i: out[0] piece number (py * numpcx + px), out[1] zones, out[2]
+ * sides, out[3] private points, out[4] master points, out[5] ghost points,
+ * out[6] corners its master points receive from other pieces, out[7]
+ * numpcx * 65536 + numpcy. */
+int pennant_pieces_layout(pennant_pieces* pp, int piece, int64_t* out8);
+/* Runs up to ncycles cycles in PENNANT_MODE_FUSED or PENNANT_MODE_MIXED;
+ * returns the cycles run, PENNANT_EBADSIDES or a negative HIP error. */
+int pennant_pieces_run(pennant_pieces* pp, int ncycles, int mode);
+/* The shared control block (layout of pennant_scalars). */
+int pennant_pieces_scalars(pennant_pieces* pp, double* out10);
+/* Entries (whole mesh) and components of a field, as pennant_field_info. */
+int pennant_pieces_field_info(pennant_pieces* pp, const char* name, int64_t* count, int* ncomp);
+/* A zone, side or point field in the one-piece numbering (points from their
+ * owner piece; vector fields interleaved x, y). */
+int pennant_pieces_get_field(pennant_pieces* pp, const char* name, double* out);
+
 #ifdef __cplusplus
 }
 #endif

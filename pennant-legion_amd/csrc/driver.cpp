@@ -63,6 +63,10 @@ struct Deck {
     }
 };
 
+}  // namespace
+
+namespace pn {
+
 bool read_params(const char* text, Params& q, std::string& err) {
     Deck d;
     if (!d.parse(text)) { err = d.err; return false; }
@@ -194,6 +198,10 @@ void build_mesh(const Params& q, HostMesh& h) {
     std::vector<int> fill(h.pcbeg.begin(), h.pcbeg.end() - 1);
     for (int s = 0; s < h.nums; ++s) h.pcside[fill[h.mapsp1[s]]++] = s;
 }
+
+}  // namespace pn
+
+namespace {
 
 template <class T> T* dalloc(pennant_problem* pb, size_t n) {
     void* p = nullptr;
@@ -344,17 +352,36 @@ static void drop_graph(pennant_problem* pb);
 extern "C" {
 
 pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen) {
+    Params q;
+    std::string err;
+    pennant_problem* pb = nullptr;
+    if (read_params(deck_text, q, err)) {
+        HostMesh h;
+        build_mesh(q, h);
+        pb = pn::create_problem(q, std::move(h), -1, err);
+    }
+    if (!pb && errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", err.c_str());
+    return pb;
+}
+
+}  // extern "C"
+
+// Sets up a problem on mesh h: device maps and fields, Mesh::init's geometry,
+// side fractions, BC lists, initial state, first dt and the fused-path
+// buffers.  npown >= 0 limits the point pass to points [0, npown) (a piece's
+// own points; its ghost points come from their owners).
+pennant_problem* pn::create_problem(const Params& prm, HostMesh&& hm, int npown, std::string& err) {
     pennant_problem* pb = new pennant_problem();
     auto fail = [&](const std::string& msg) -> pennant_problem* {
-        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", msg.c_str());
+        err = msg;
         pennant_destroy(pb);
         return nullptr;
     };
-    std::string err;
-    if (!read_params(deck_text, pb->prm, err)) return fail(err);
+    pb->prm = prm;
+    pb->hm = std::move(hm);
+    pb->npown = npown;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
-    build_mesh(pb->prm, pb->hm);
     const HostMesh& h = pb->hm;
     if (h.maxznump > PENNANT_MAX_ZSIDES) return fail("zones with more than 8 sides are not supported");
     if (hipStreamCreate(&pb->stream) != hipSuccess) return fail("hipStreamCreate failed");
@@ -429,6 +456,8 @@ pennant_problem* pennant_create(const char* deck_text, char* errbuf, int errlen)
     if (rc) return fail(std::string("initialisation failed: ") + hipGetErrorString((hipError_t)rc));
     return pb;
 }
+
+extern "C" {
 
 void pennant_destroy(pennant_problem* pb) {
     if (!pb) return;
@@ -677,7 +706,7 @@ int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
         return 0;
     }
     if (!strcmp(key, "k1_layout")) {
-        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        if (value < 0 || value > 2) return PENNANT_EINVAL;
         pb->tune.k1_layout = value;
         return 0;
     }
@@ -748,7 +777,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     // the mixed forms also try the pipelined cycle (it pays where the
     // volume-change limit seldom sets dt)
     const int npipe = (form == FORM_FP64 || pb->cap_buf) ? 1 : 2;
-    for (int layout = 0; layout <= (form == FORM_FP64 ? 1 : 0) && !rc; ++layout)
+    for (int layout = 0; layout <= (form == FORM_FP64 ? 2 : 0) && !rc; ++layout)
       for (int pipe = 0; pipe < npipe && !rc; ++pipe)
         for (int xcd = 0; xcd <= 1 && !rc; ++xcd)
             for (int g : grids) {

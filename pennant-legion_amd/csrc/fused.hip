@@ -62,6 +62,7 @@ struct FusedArgs {
     const int32_t* pcpos;      // side -> slot of its corner record (corner_slot)
     const double* smf;
     int nump;
+    int npown;                 // K2 updates points [0, npown)
     // state (in place)
     double* pxx; double* pxy; double* pux; double* puy;
     double* zr; double* ze; double* zetot; double* zwrate; double* zvol; double* zp;
@@ -188,35 +189,126 @@ __device__ __forceinline__ SideW side_w(int z, int zbeg, int zend, int sbeg, int
     return w;
 }
 
+// One corner's QCS results, as the two sides that meet at the corner use
+// them: setCornerDiv, setQCnForce and setForceQCS's corner weight
+// (src/QCS.cc:116-380) for corner k of a zone, whose point has velocity up0
+// and half-step position xp0, neighbouring points' velocities u0p (previous)
+// and u0n (next), edge centres xp1 = ex[k] and xp3 = ex[k-1], and edge
+// lengths de1 = elen[k-1] (reciprocal r1) and de2 = elen[k] (reciprocal r2).
+struct CrnQ {
+    d2 q1, q2;      // cqe1, cqe2
+    double w, c;    // cw and the clamped ccos
+};
+__device__ __forceinline__ CrnQ qcs_corner(d2 up0, d2 u0p, d2 u0n, d2 xp0, d2 xp1, d2 xp3, d2 zx, d2 zuc,
+                                           double de1, double de2, double r1, double r2, double ztmp1,
+                                           double g2, double zrp) {
+    const d2 up1 = scl(0.5, add(up0, u0n));
+    const d2 up2 = zuc;
+    const d2 xp2 = zx;
+    const d2 up3 = scl(0.5, add(u0p, up0));
+    const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
+    const d2 v1 = sub(xp3, xp0);
+    const d2 v2 = sub(xp1, xp0);
+    const double minelen = smin(de1, de2);
+    const double cc = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
+    const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
+                       (2.0 * cvolume);
+    const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
+    const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
+    const double dx1 = len(dxx1);
+    const double dx2 = len(dxx2);
+    const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
+    const double test1 = fabs(dot(dxx1, duav) * dx2);
+    const double test2 = fabs(dot(dxx2, duav) * dx1);
+    const double num = (test1 > test2 ? dx1 : dx2);
+    const double den = (test1 > test2 ? dx2 : dx1);
+    const double rr = num / den;
+    double evol = __builtin_sqrt(4.0 * cvolume * rr);
+    evol = smin(evol, 2.0 * minelen);
+    const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
+    const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
+    const double du = __builtin_sqrt(smax(dv1, dv2));
+    const double cevol = (div < 0.0 ? evol : 0.);
+    const double cdu = (div < 0.0 ? du : 0.);
+    // setQCnForce (g2 = q2 * 0.25 * (qgamma + 1), the reference's left-to-right product)
+    const double ztmp2 = g2 * cdu;
+    const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
+    const double mu = zkur * zrp * cevol;
+    const double cm = ((div > 0.0) ? 0. : mu);
+    CrnQ r;
+    const d2 d1 = scl(cm, sub(up0, u0p));
+    const d2 d2_ = scl(cm, sub(u0n, up0));
+    r.q1 = mk(d1.x * r1, d1.y * r1);     // Vec2 / double: times the reciprocal
+    r.q2 = mk(d2_.x * r2, d2_.y * r2);
+    // setForceQCS's corner weight and clamped cosine
+    const double csin2 = 1.0 - cc * cc;
+    r.w = ((csin2 < 1.e-4) ? 0. : cvolume / csin2);
+    r.c = ((csin2 < 1.e-4) ? 0. : cc);
+    return r;
+}
+
 // K1, zone-per-thread: the whole predictor half of doCycle for zone z, whose
 // sides are sb..sb+NS-1 (side work force slots sw).  Returns the zone's
 // calcDtNew candidate.
+//
+// Every value is the reference's expression, but the corners and sides are
+// visited in a streaming order that keeps few of them live at once: corner
+// k+1's QCS, then side k's forces (which need corners k and k+1), then
+// corner k's force tot[k] - tot[k-1]; corner 0 is kept for the last side.
+// Each side's 1/elen is formed once and used by both corners at the side and
+// by its viscous force (the reference divides a Vec2 by elen through the
+// same reciprocal, src/Vec2.hh:145-149).
+// What the FP64 predictor of a zone reads: its points' positions and
+// velocities, its state from the previous cycle, its side fractions and its
+// corners' slots.
 template <int NS>
-__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, SideW sw,
-                                            const int (&pp)[NS], double dt, int& bad) {
+struct ZoneIn {
+    d2 x[NS], u[NS];
+    double zm, zr, ze, zvol, zwr;
+    double mf[NS];
+    int slot[NS];
+};
+template <int NS>
+__device__ __forceinline__ void load_zone(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
+                                          ZoneIn<NS>& in) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        in.x[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
+        in.u[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
+    }
+    in.zm = a.zm[z];
+    in.zr = a.zr[z];
+    in.ze = a.ze[z];
+    in.zvol = a.zvol[z];
+    in.zwr = a.zwrate[z];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        in.mf[k] = a.smf[sb + k];
+        in.slot[k] = ld_nt(a.pcpos + sb + k);
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ double zone_pred_core(const FusedArgs& a, int z, SideW sw, const ZoneIn<NS>& in,
+                                                 double dt, int& bad) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
-    // every load of the zone first (points pp = mapsp1 of its sides)
     d2 u0[NS], xp[NS];
     double mf[NS];
     int slot[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        xp[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
-        u0[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
+        u0[k] = in.u[k];
+        mf[k] = in.mf[k];
+        slot[k] = in.slot[k];
     }
-    const double zm = a.zm[z];
-    const double zr0 = a.zr[z];
-    const double ze0 = a.ze[z];
-    const double zvol0 = a.zvol[z];
-    const double zwr = a.zwrate[z];
+    const double zm = in.zm;
+    const double zr0 = in.zr;
+    const double ze0 = in.ze;
+    const double zvol0 = in.zvol;
+    const double zwr = in.zwr;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        mf[k] = a.smf[sb + k];
-        slot[k] = ld_nt(a.pcpos + sb + k);
-    }
-#pragma unroll
-    for (int k = 0; k < NS; ++k) xp[k] = add(xp[k], scl(dth, u0[k]));   // advPosHalf
+    for (int k = 0; k < NS; ++k) xp[k] = add(in.x[k], scl(dth, u0[k]));   // advPosHalf
     // calcCtrs
     d2 ex[NS];
     d2 zx = mk(0., 0.);
@@ -227,7 +319,7 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, S
     }
     // calcVols, calcEdgeLen, calcCharLen
     const double third = 1. / 3.;
-    double sa[NS], el[NS];
+    double sa[NS], el[NS], rel[NS];
     double zarea = 0., zvolp = 0.;
     const double fac = (NS == 3 ? 3. : 4.);
     double zdl = 1.e99;
@@ -242,7 +334,10 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, S
         el[k] = len(sub(p2, p1));
     }
 #pragma unroll
-    for (int k = 0; k < NS; ++k) zdl = smin(zdl, fac * sa[k] / el[k]);
+    for (int k = 0; k < NS; ++k) {
+        zdl = smin(zdl, fac * sa[k] / el[k]);
+        rel[k] = 1. / el[k];
+    }
     // calcRho (pred)
     const double zrp = zm / zvolp;
     // calcStateHalf (reads ZR, ZE, ZWRATE of the previous cycle, ZVOL0 = ZVOL)
@@ -262,138 +357,92 @@ __device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, S
     const double src = zwr * dth * minv;
     const double zp = p + (per * src - r * bulk * dv) / denom;
     a.zp[z] = zp;
-    // calcSurfVecs, calcForcePGas, calcForceTTS
-    d2 tot[NS], sfw[NS];
+    // calcCrnrMass's corner mass
+    const double mzz = zrp * zarea;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) a.cmass[slot[k]] = mzz * 0.5 * (mf[k] + mf[prv<NS>(k)]);
+    // calcSurfVecs, calcForcePGas, calcForceTTS per side (below)
     double sstmp = smax(zss, a.ssmin);
     sstmp = a.alfa * sstmp * sstmp;
-    d2 sfp[NS], sft[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const d2 ss = rot_ccw(sub(ex[k], zx));
-        sfp[k] = scl(-zp, ss);
-        const double vfacinv = zarea / sa[k];
-        const double srho = zrp * mf[k] * vfacinv;
-        const double sdp = sstmp * (srho - zrp);
-        sft[k] = scl(-sdp, ss);
-    }
-    // setCornerDiv
+    // setCornerDiv's zone velocity
     d2 zuc = mk(0., 0.);
 #pragma unroll
     for (int k = 0; k < NS; ++k) zuc = add(zuc, dvs(u0[k], n));
-    double carea[NS], ccos[NS], cdiv[NS], cevol[NS], cdu[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int km = prv<NS>(k), kp = nxt<NS>(k);
-        const d2 up0 = u0[k];
-        const d2 xp0 = xp[k];
-        const d2 up1 = scl(0.5, add(up0, u0[kp]));
-        const d2 xp1 = ex[k];
-        const d2 up2 = zuc;
-        const d2 xp2 = zx;
-        const d2 up3 = scl(0.5, add(u0[km], up0));
-        const d2 xp3 = ex[km];
-        const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
-        carea[k] = cvolume;
-        const d2 v1 = sub(xp3, xp0);
-        const d2 v2 = sub(xp1, xp0);
-        const double de1 = el[km];
-        const double de2 = el[k];
-        const double minelen = smin(de1, de2);
-        ccos[k] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
-        const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
-                           (2.0 * cvolume);
-        cdiv[k] = div;
-        const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
-        const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
-        const double dx1 = len(dxx1);
-        const double dx2 = len(dxx2);
-        const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
-        const double test1 = fabs(dot(dxx1, duav) * dx2);
-        const double test2 = fabs(dot(dxx2, duav) * dx1);
-        const double num = (test1 > test2 ? dx1 : dx2);
-        const double den = (test1 > test2 ? dx2 : dx1);
-        const double rr = num / den;
-        double evol = __builtin_sqrt(4.0 * cvolume * rr);
-        evol = smin(evol, 2.0 * minelen);
-        const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
-        const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
-        double du = __builtin_sqrt(smax(dv1, dv2));
-        cevol[k] = (div < 0.0 ? evol : 0.);
-        cdu[k] = (div < 0.0 ? du : 0.);
-    }
-    // setQCnForce
-    const double gammap1 = a.qgamma + 1.0;
+    const double g2 = a.q2 * 0.25 * (a.qgamma + 1.0);
     const double ztmp1 = a.q1 * zss;
-    d2 cqe1[NS], cqe2[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int km = prv<NS>(k), kp = nxt<NS>(k);
-        const double ztmp2 = a.q2 * 0.25 * gammap1 * cdu[k];
-        const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
-        const double mu = zkur * zrp * cevol[k];
-        const double cm = ((cdiv[k] > 0.0) ? 0. : mu);
-        cqe1[k] = dvs(scl(cm, sub(u0[k], u0[km])), el[km]);
-        cqe2[k] = dvs(scl(cm, sub(u0[kp], u0[k])), el[k]);
-    }
-    // setForceQCS
-    double cw[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const double cc = ccos[k];
-        const double csin2 = 1.0 - cc * cc;
-        cw[k] = ((csin2 < 1.e-4) ? 0. : carea[k] / csin2);
-        ccos[k] = ((csin2 < 1.e-4) ? 0. : cc);
-    }
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int kp = nxt<NS>(k);
-        const d2 f = add(scl(cw[k], add(cqe2[k], scl(ccos[k], cqe1[k]))),
-                         scl(cw[kp], add(cqe1[kp], scl(ccos[kp], cqe2[kp]))));
-        const d2 sfq = dvs(f, el[k]);
-        sfw[k] = add(sfp[k], sfq);                       // calcWork's sf + sf2
-        tot[k] = add(add(sfp[k], sfq), sft[k]);          // sumCrnrForce's sfp+sfq+sft
-    }
-    // setVelDiff
+    // corners, sides and corner forces in streaming order
+    const CrnQ c0 = qcs_corner(u0[0], u0[NS - 1], u0[1], xp[0], ex[0], ex[NS - 1], zx, zuc, el[NS - 1], el[0],
+                               rel[NS - 1], rel[0], ztmp1, g2, zrp);
+    CrnQ ck = c0;
+    d2 tot0 = mk(0., 0.), totk = mk(0., 0.), sfw_prev = mk(0., 0.);
     double ztmp = 0.;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int kp = nxt<NS>(k);
-        const d2 dx = sub(xp[kp], xp[k]);
-        const d2 du = sub(u0[kp], u0[k]);
-        const double lenx = el[k];
-        double dux = dot(du, dx);
-        dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
-        ztmp = smax(ztmp, dux);
-    }
-    const double zdu = a.q1 * zss + 2. * a.q2 * ztmp;
-    // per-corner outputs: corner force and corner mass at corner k (point
-    // mapsp1[sb+k]), and the work force of side k
-    const double mzz = zrp * zarea;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int km = prv<NS>(k);
-        const d2 cf = sub(tot[k], tot[km]);
-        const int sl = slot[k];
-        a.cfx[sl] = cf.x;
-        a.cfy[sl] = cf.y;
-        a.cmass[sl] = mzz * 0.5 * (mf[k] + mf[km]);
-    }
-#pragma unroll
-    for (int k = 0; k < NS; k += 2) {
-        const int sl = sw_slot<NS>(sw, k);
-        if (k + 1 < NS) {
-            const f64x2u vx = {sfw[k].x, sfw[k + 1].x}, vy = {sfw[k].y, sfw[k + 1].y};
+        const CrnQ cn = (kp == 0) ? c0
+                                  : qcs_corner(u0[kp], u0[k], u0[nxt<NS>(kp)], xp[kp], ex[kp], ex[k], zx, zuc,
+                                               el[k], el[kp], rel[k], rel[kp], ztmp1, g2, zrp);
+        // side k: pressure, TTS and viscous forces
+        const d2 ss = rot_ccw(sub(ex[k], zx));
+        const d2 sfp = scl(-zp, ss);
+        const double vfacinv = zarea / sa[k];
+        const double srho = zrp * mf[k] * vfacinv;
+        const double sdp = sstmp * (srho - zrp);
+        const d2 sft = scl(-sdp, ss);
+        const d2 f = add(scl(ck.w, add(ck.q2, scl(ck.c, ck.q1))), scl(cn.w, add(cn.q1, scl(cn.c, cn.q2))));
+        const d2 sfq = mk(f.x * rel[k], f.y * rel[k]);
+        const d2 sfw = add(sfp, sfq);                    // calcWork's sf + sf2
+        const d2 tot = add(add(sfp, sfq), sft);          // sumCrnrForce's sfp+sfq+sft
+        // setVelDiff's side k, in ascending order from 0
+        {
+            const d2 dx = sub(xp[kp], xp[k]);
+            const d2 du = sub(u0[kp], u0[k]);
+            const double lenx = el[k];
+            double dux = dot(du, dx);
+            dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
+            ztmp = smax(ztmp, dux);
+        }
+        // corner k's force (point mapsp1[sb+k]) once sides k-1 and k are known
+        if (k == 0) {
+            tot0 = tot;
+        } else {
+            const d2 cf = sub(tot, totk);
+            a.cfx[slot[k]] = cf.x;
+            a.cfy[slot[k]] = cf.y;
+        }
+        totk = tot;
+        // the work forces of a side pair together (pair-major slots)
+        if (k & 1) {
+            const int sl = sw_slot<NS>(sw, k - 1);
+            const f64x2u vx = {sfw_prev.x, sfw.x}, vy = {sfw_prev.y, sfw.y};
             __builtin_nontemporal_store(vx, reinterpret_cast<f64x2u*>(a.sfwx + sl));
             __builtin_nontemporal_store(vy, reinterpret_cast<f64x2u*>(a.sfwy + sl));
-        } else {
-            st_nt(sfw[k].x, a.sfwx + sl);
-            st_nt(sfw[k].y, a.sfwy + sl);
+        } else if (k + 1 == NS) {
+            const int sl = sw_slot<NS>(sw, k);
+            st_nt(sfw.x, a.sfwx + sl);
+            st_nt(sfw.y, a.sfwy + sl);
         }
+        sfw_prev = sfw;
+        ck = cn;
     }
+    {
+        const d2 cf = sub(tot0, totk);
+        a.cfx[slot[0]] = cf.x;
+        a.cfy[slot[0]] = cf.y;
+    }
+    const double zdu = a.q1 * zss + 2. * a.q2 * ztmp;
     // calcDtNew for this zone
     const double fuzz = 1.e-99;
     const double cdt = smax(zdu, smax(zss, fuzz));
     return zdl * a.cfl / cdt;
+}
+
+template <int NS>
+__device__ __forceinline__ double zone_pred(const FusedArgs& a, int z, int sb, SideW sw,
+                                            const int (&pp)[NS], double dt, int& bad) {
+    ZoneIn<NS> in;
+    load_zone<NS>(a, z, sb, pp, in);      // every load of the zone first
+    return zone_pred_core<NS>(a, z, sw, in, dt, bad);
 }
 
 template <int NS>
@@ -427,6 +476,208 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
         if (z >= zend) continue;
         const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS,
                                             side_w(z, zbeg, zend, sbeg, NS), pp, dt, bad);
+        // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
+        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
+    }
+    const double m = block_min(dtmin);
+    const int b = block_sum(bad);
+    if (threadIdx.x == 0) {
+        a.part_dt[poff + blockIdx.x] = m;
+        a.part_bp[poff + blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K1, zone per thread, with the next zone's inputs prefetched into LDS.
+//
+// The FP64 predictor keeps a whole zone in registers (about 240 VGPRs: two
+// waves per SIMD), so a thread has no registers to load the next zone into
+// while it works on this one, and every zone starts with a full memory round
+// trip.  Here the loads of zone i+1 go straight to LDS (global_load_lds, one
+// dword per lane and load; no VGPRs) while the thread computes zone i, and
+// the thread reads them back at the top of iteration i+1.  The LDS area is
+// per wave (dma_words dwords x 64 lanes), read back by the lane that loaded
+// it, so no barrier is needed.
+//
+// The hardware counts these loads in vmcnt, in issue order with the other
+// vector memory instructions, and the compiler does not know that the LDS
+// reads depend on them, so the waits are explicit:
+//   * before reading zone i+1's data: vmcnt(3 NS).  After the loads of zone
+//     i+1 the thread issues at least 3 NS more vector memory instructions --
+//     zone i's corner force and mass stores, 3 NS scattered stores that
+//     cannot merge and cannot move above the loads (they may alias the loads'
+//     sources) -- so vmcnt <= 3 NS means the prefetch has landed.  The first
+//     zone of a thread waits for vmcnt(0);
+//   * before the next prefetch overwrites the area: lgkmcnt(0) (the LDS
+//     reads of this zone are done).
+#define PN_DMA_MAXNS 4
+// rows of a wave's LDS area: x, u (2 dwords each) per point, 5 zone doubles,
+// smf, pcpos, then the next zone's points (mapsp1)
+template <int NS>
+constexpr int dma_words() { return 12 * NS + 10; }
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+__device__ __forceinline__ void dma_dword(const void* g, unsigned* row) {
+    __builtin_amdgcn_global_load_lds(g, (lds_void_t*)row, 4, 0, 0);
+}
+__device__ __forceinline__ void dma_double(const double* g, unsigned (*w)[64], int j) {
+    dma_dword(g, w[j]);
+    dma_dword(reinterpret_cast<const char*>(g) + 4, w[j + 1]);
+}
+// s_waitcnt with only vmcnt (expcnt and lgkmcnt left at their maximum), or
+// only lgkmcnt; each followed by a compiler barrier so that no LDS read moves
+// above it
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __asm__ __volatile__("" ::: "memory");
+}
+__device__ __forceinline__ void wait_lgkmcnt0() {
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
+    __asm__ __volatile__("" ::: "memory");
+}
+
+template <int NS>
+__device__ __forceinline__ void dma_zone(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
+                                         unsigned (*w)[64]) {
+    int j = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int p = pp[k];
+        dma_double(a.pxx + p, w, j);
+        dma_double(a.pxy + p, w, j + 2);
+        dma_double(a.pux + p, w, j + 4);
+        dma_double(a.puy + p, w, j + 6);
+        j += 8;
+    }
+    dma_double(a.zm + z, w, j);
+    dma_double(a.zr + z, w, j + 2);
+    dma_double(a.ze + z, w, j + 4);
+    dma_double(a.zvol + z, w, j + 6);
+    dma_double(a.zwrate + z, w, j + 8);
+    j += 10;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) dma_double(a.smf + sb + k, w, j + 2 * k);
+    j += 2 * NS;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) dma_dword(a.pcpos + sb + k, w[j + k]);
+}
+// the points of the zone whose sides start at sb, to the rows after the zone's
+template <int NS>
+__device__ __forceinline__ void dma_points(const FusedArgs& a, int sb, unsigned (*w)[64]) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) dma_dword(a.mapsp1 + sb + k, w[11 * NS + 10 + k]);
+}
+// The zone's inputs from this lane's column of the wave's LDS area: row pair
+// (j, j + 1) is one ds_read2st64_b32 (rows are 256 bytes apart).  The reads
+// are one asm statement that ends with lgkmcnt(0), so every output is
+// complete when the statement ends; written as asm, they carry no implied
+// wait on the prefetch (wait_vmcnt before the statement provides it).
+#define PN_RD(o, j) "ds_read2st64_b32 %" #o ", %[a] offset0:" #j " offset1:" #j "+1\n"
+template <int NS>
+__device__ __forceinline__ void read_zone(unsigned (*w)[64], int lane, ZoneIn<NS>& in, int (&pn)[NS]);
+template <>
+__device__ __forceinline__ void read_zone<4>(unsigned (*w)[64], int lane, ZoneIn<4>& in, int (&pn)[4]) {
+    const unsigned addr = (unsigned)(uintptr_t)(lds_void_t*)&w[0][lane];
+    double v[25];
+    unsigned long long sl[2], pq[2];
+    __asm__ __volatile__(
+        PN_RD(0, 0) PN_RD(1, 2) PN_RD(2, 4) PN_RD(3, 6) PN_RD(4, 8) PN_RD(5, 10) PN_RD(6, 12)
+        PN_RD(7, 14) PN_RD(8, 16) PN_RD(9, 18) PN_RD(10, 20) PN_RD(11, 22) PN_RD(12, 24) PN_RD(13, 26)
+        PN_RD(14, 28) PN_RD(15, 30) PN_RD(16, 32) PN_RD(17, 34) PN_RD(18, 36) PN_RD(19, 38) PN_RD(20, 40)
+        PN_RD(21, 42) PN_RD(22, 44) PN_RD(23, 46) PN_RD(24, 48) PN_RD(25, 50) PN_RD(26, 52)
+        PN_RD(27, 54) PN_RD(28, 56)
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+          "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9]), "=&v"(v[10]), "=&v"(v[11]), "=&v"(v[12]), "=&v"(v[13]),
+          "=&v"(v[14]), "=&v"(v[15]), "=&v"(v[16]), "=&v"(v[17]), "=&v"(v[18]), "=&v"(v[19]),
+          "=&v"(v[20]), "=&v"(v[21]), "=&v"(v[22]), "=&v"(v[23]), "=&v"(v[24]), "=&v"(sl[0]), "=&v"(sl[1]),
+          "=&v"(pq[0]), "=&v"(pq[1])
+        : [a] "v"(addr)
+        : "memory");
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        in.x[k] = mk(v[4 * k], v[4 * k + 1]);
+        in.u[k] = mk(v[4 * k + 2], v[4 * k + 3]);
+    }
+    in.zm = v[16]; in.zr = v[17]; in.ze = v[18]; in.zvol = v[19]; in.zwr = v[20];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) in.mf[k] = v[21 + k];
+    in.slot[0] = (int)(unsigned)sl[0]; in.slot[1] = (int)(unsigned)(sl[0] >> 32);
+    in.slot[2] = (int)(unsigned)sl[1]; in.slot[3] = (int)(unsigned)(sl[1] >> 32);
+    pn[0] = (int)(unsigned)pq[0]; pn[1] = (int)(unsigned)(pq[0] >> 32);
+    pn[2] = (int)(unsigned)pq[1]; pn[3] = (int)(unsigned)(pq[1] >> 32);
+}
+template <>
+__device__ __forceinline__ void read_zone<3>(unsigned (*w)[64], int lane, ZoneIn<3>& in, int (&pn)[3]) {
+    const unsigned addr = (unsigned)(uintptr_t)(lds_void_t*)&w[0][lane];
+    double v[20];
+    unsigned long long sl, pq[2];
+    __asm__ __volatile__(
+        PN_RD(0, 0) PN_RD(1, 2) PN_RD(2, 4) PN_RD(3, 6) PN_RD(4, 8) PN_RD(5, 10) PN_RD(6, 12)
+        PN_RD(7, 14) PN_RD(8, 16) PN_RD(9, 18) PN_RD(10, 20) PN_RD(11, 22) PN_RD(12, 24) PN_RD(13, 26)
+        PN_RD(14, 28) PN_RD(15, 30) PN_RD(16, 32) PN_RD(17, 34) PN_RD(18, 36) PN_RD(19, 38)
+        PN_RD(20, 40) PN_RD(21, 42) PN_RD(22, 44)
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
+          "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9]), "=&v"(v[10]), "=&v"(v[11]), "=&v"(v[12]), "=&v"(v[13]),
+          "=&v"(v[14]), "=&v"(v[15]), "=&v"(v[16]), "=&v"(v[17]), "=&v"(v[18]), "=&v"(v[19]),
+          "=&v"(sl), "=&v"(pq[0]), "=&v"(pq[1])
+        : [a] "v"(addr)
+        : "memory");
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        in.x[k] = mk(v[4 * k], v[4 * k + 1]);
+        in.u[k] = mk(v[4 * k + 2], v[4 * k + 3]);
+    }
+    in.zm = v[12]; in.zr = v[13]; in.ze = v[14]; in.zvol = v[15]; in.zwr = v[16];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in.mf[k] = v[17 + k];
+    in.slot[0] = (int)(unsigned)sl; in.slot[1] = (int)(unsigned)(sl >> 32);
+    in.slot[2] = (int)(unsigned)pq[0];
+    pn[0] = (int)(unsigned)(pq[0] >> 32); pn[1] = (int)(unsigned)pq[1]; pn[2] = (int)(unsigned)(pq[1] >> 32);
+}
+#undef PN_RD
+
+template <int NS>
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) k_pred_dma(FusedArgs a, int zbeg, int zend, int sbeg,
+                                                       int poff) {
+    static_assert(NS <= PN_DMA_MAXNS, "LDS area per wave");
+    __shared__ unsigned dbuf[PN_BLOCK / 64][dma_words<NS>()][64];
+    if (a.ctl->done) return;
+    const double dt = a.ctl->dt;
+    double dtmin = 1.e99;
+    int bad = 0;
+    const int lane = threadIdx.x & 63;
+    unsigned (*w)[64] = dbuf[threadIdx.x >> 6];
+    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
+    Units u = block_units(nunits, a.xcd);
+    // zone i (z) has its inputs in flight to LDS, and the points of zone i+1
+    // (zn) follow them there
+    int z = zbeg + u.u * PN_BLOCK + threadIdx.x;
+    int zn = zbeg + (u.u + u.step) * PN_BLOCK + threadIdx.x;
+    if (u.u < u.end && z < zend) {
+        int pp[NS];
+        load_pp<NS>(a, sbeg + (z - zbeg) * NS, pp);
+        dma_zone<NS>(a, z, sbeg + (z - zbeg) * NS, pp, w);
+    }
+    if (u.u + u.step < u.end && zn < zend) dma_points<NS>(a, sbeg + (zn - zbeg) * NS, w);
+    wait_vmcnt<0>();
+    bool first = true;
+    for (; u.u < u.end; u.u += u.step) {
+        if (!first) wait_vmcnt<3 * NS>();
+        first = false;
+        if (z >= zend) break;               // every later unit of this thread lies past zend too
+        ZoneIn<NS> in;
+        int pn[NS];
+        read_zone<NS>(w, lane, in, pn);     // ends with lgkmcnt(0): the area may be refilled
+        const int zc = z;
+        if (u.u + u.step < u.end && zn < zend) dma_zone<NS>(a, zn, sbeg + (zn - zbeg) * NS, pn, w);
+        z = zn;
+        zn = zbeg + (u.u + 2 * u.step) * PN_BLOCK + threadIdx.x;
+        if (u.u + 2 * u.step < u.end && zn < zend) dma_points<NS>(a, sbeg + (zn - zbeg) * NS, w);
+        const double zdthyd = zone_pred_core<NS>(a, zc, side_w(zc, zbeg, zend, sbeg, NS), in, dt, bad);
         // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
@@ -1463,7 +1714,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
         return;
     }
     const int p = (blockIdx.x - a.nheavy) * PN_BLOCK + threadIdx.x;
-    if (p < a.nump) point_light<T>(a, p, dt);
+    if (p < a.npown) point_light<T>(a, p, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1978,7 +2229,7 @@ FusedArgs make_args(pennant_problem* pb) {
     FusedArgs a;
     const Params& q = pb->prm;
     a.mapsp1 = pb->dm.mapsp1; a.pcbeg = pb->dm.pcbeg; a.pcpos = pb->pcpos;
-    a.smf = pb->f("smf"); a.nump = pb->dm.nump;
+    a.smf = pb->f("smf"); a.nump = pb->dm.nump; a.npown = own_points(pb);
     a.pxx = pb->f("px", 0); a.pxy = pb->f("px", 1); a.pux = pb->f("pu", 0); a.puy = pb->f("pu", 1);
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
@@ -2057,6 +2308,9 @@ void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, 
         hipLaunchKernelGGL((k_pred_mixed<NS, false>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff, run);
     else if (pb->tune.k1_layout == 1)
         hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
+    else if (pb->tune.k1_layout == 2 && NS <= PN_DMA_MAXNS)
+        hipLaunchKernelGGL(k_pred_dma<(NS <= PN_DMA_MAXNS ? NS : PN_DMA_MAXNS)>, grid, dim3(PN_BLOCK), 0, s, a,
+                           zb, ze, sb, poff);
     else
         hipLaunchKernelGGL(k_pred_zone<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
 }
@@ -2234,6 +2488,7 @@ BatchArgs make_batch_args(pennant_batch* b) {
     std::memset(&a, 0, sizeof a);
     a.mapsp1 = b->mapsp1; a.pcbeg = b->pcbeg; a.pcpos = b->pcpos; a.smf = b->smf;
     a.nump = (int)b->nump;
+    a.npown = a.nump;
     a.pxx = b->pxx; a.pxy = b->pxy; a.pux = b->pux; a.puy = b->puy;
     a.zr = b->zr; a.ze = b->ze; a.zetot = b->zetot; a.zwrate = b->zwrate; a.zvol = b->zvol;
     a.zp = b->zp; a.zm = b->zm;
@@ -2300,7 +2555,7 @@ int fused_setup(pennant_problem* pb) {
     for (double b : pb->prm.bcx) mark(b, true);
     for (double b : pb->prm.bcy) mark(b, false);
     std::vector<int> heavy;
-    for (int p = 0; p < h.nump; ++p)
+    for (int p = 0; p < own_points(pb); ++p)
         if (h.pcbeg[p + 1] - h.pcbeg[p] > PN_HEAVY) heavy.push_back(p);
     auto al = [&](size_t n, size_t sz) -> void* {
         void* p = nullptr;
@@ -2327,11 +2582,12 @@ int fused_setup(pennant_problem* pb) {
         return (int)hipErrorOutOfMemory;
     if (refresh_side_fracs(pb)) return (int)hipErrorUnknown;
     // K1 writes each corner's force and mass straight into its corner slot,
-    // where K2 reads it
+    // where K2 reads it.  (A piece's master point also lists the corners of
+    // other pieces, pcside -1: their records arrive by the corner exchange.)
     std::vector<int> pcpos((size_t)h.nums);
     for (int p = 0; p < h.nump; ++p)
         for (int k = h.pcbeg[p]; k < h.pcbeg[p + 1]; ++k)
-            pcpos[h.pcside[k]] = (int)corner_slot((int64_t)P, p, k - h.pcbeg[p], k);
+            if (h.pcside[k] >= 0) pcpos[h.pcside[k]] = (int)corner_slot((int64_t)P, p, k - h.pcbeg[p], k);
     pb->pcpos = (int*)al(S, 4);
     if (pb->pcpos &&
         hipMemcpy(pb->pcpos, pcpos.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -2408,7 +2664,7 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
         }
     }
     if (ev) (void)hipEventRecord(ev[2], s);
-    const dim3 pgrid(pb->nheavy + nblocks(h.nump));
+    const dim3 pgrid(pb->nheavy + nblocks(own_points(pb)));
     if (mixed)
         hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, s, a);
     else
@@ -2532,7 +2788,7 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     const HostMesh& h = pb->hm;
     const bool fl = (form == FORM_LEARNED);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(k_point<float>, dim3(pb->nheavy + nblocks(h.nump)), dim3(PN_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_point<float>, dim3(pb->nheavy + nblocks(own_points(pb))), dim3(PN_BLOCK), 0, s, a);
     if (ev) (void)hipEventRecord(ev[1], s);
     int poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
@@ -2625,6 +2881,174 @@ int batch_cycle(pennant_batch* b, int form) {
     }
     hipLaunchKernelGGL(k_finalize_batch, dim3(b->nprob), dim3(PN_FIN_BLOCK), 0, s, B,
                        form == FORM_FP64 ? 1 : 0);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Several pieces of one mesh on one device (src/Mesh.cc:432-617 partitions
+// the points into private, master and ghost sets; src/Hydro.cc:450-455,
+// 571-574, 766-769, 866-867, 902-906 read ghosts and reduce shared points).
+// Each piece runs the fused K1 / K2 / K3 over its own arrays.  Between them:
+//   corner exchange  each corner a piece's zones have at a ghost point is
+//                    copied from the slot the piece's K1 wrote to the slot of
+//                    the owner's corner list that holds it; the owner lists a
+//                    master point's corners from every piece in ascending
+//                    one-piece side order, so K2's ordered sum is the
+//                    one-piece run's sum, bit for bit;
+//   halo             after K2 the owners' new point state (px, pu and the
+//                    calcWork point values) goes to the ghost copies, which
+//                    K3 and the next cycle's K1 read.
+// The pieces share one control block and one array of block partials, so a
+// single finalize folds every piece's dt, dvol and bad-side partials.
+
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(PN_BLOCK) k_corner_xchg(const PieceTab* tab, const XEnt* x, int n,
+                                                          const Ctl* ctl) {
+    if (ctl->done) return;
+    const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const XEnt e = x[i];
+    const PieceTab& a = tab[e.sp];
+    const PieceTab& b = tab[e.dp];
+    if constexpr (sizeof(T) == 4) {
+        const int64_t u = 3 * (int64_t)e.src, v = 3 * (int64_t)e.dst;
+        b.fcrn[v] = a.fcrn[u];
+        b.fcrn[v + 1] = a.fcrn[u + 1];
+        b.fcrn[v + 2] = a.fcrn[u + 2];
+    } else {
+        b.cfx[e.dst] = a.cfx[e.src];
+        b.cfy[e.dst] = a.cfy[e.src];
+        b.cmass[e.dst] = a.cmass[e.src];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PN_BLOCK) k_halo(const PieceTab* tab, const XEnt* x, int n,
+                                                   const Ctl* ctl) {
+    if (ctl->done) return;
+    const int i = blockIdx.x * PN_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const XEnt e = x[i];
+    const PieceTab& a = tab[e.sp];
+    const PieceTab& b = tab[e.dp];
+    b.pxx[e.dst] = a.pxx[e.src];
+    b.pxy[e.dst] = a.pxy[e.src];
+    b.pux[e.dst] = a.pux[e.src];
+    b.puy[e.dst] = a.puy[e.src];
+    b.vs2[2 * (int64_t)e.dst] = a.vs2[2 * (int64_t)e.src];
+    b.vs2[2 * (int64_t)e.dst + 1] = a.vs2[2 * (int64_t)e.src + 1];
+    if constexpr (sizeof(T) == 8) b.pxpx[e.dst] = a.pxpx[e.src];
+}
+
+FusedArgs piece_args(pennant_pieces* pp, int q) {
+    FusedArgs a = make_args(pp->pc[q]);
+    a.ctl = pp->pc[0]->ctl;
+    a.part_dt = pp->part_d;
+    a.part_dv = pp->part_d + pp->npart;
+    a.part_bp = pp->part_i;
+    a.part_bc = pp->part_i + pp->npart;
+    // no learned pass or tuple capture in a multi-piece run
+    a.zlist = nullptr;
+    a.zcount = nullptr;
+    a.nzruns = 0;
+    a.cap_buf = nullptr;
+    return a;
+}
+
+}  // namespace
+
+int pieces_cycle(pennant_pieces* pp, int form) {
+    if (form == FORM_LEARNED || pp->pc.empty()) return PENNANT_EINVAL;
+    const bool mixed = (form != FORM_FP64);
+    const int n = (int)pp->pc.size();
+    hipStream_t s = pp->pc[0]->stream;
+    std::vector<FusedArgs> A((size_t)n);
+    for (int q = 0; q < n; ++q) A[(size_t)q] = piece_args(pp, q);
+    int poff = 0;
+    for (int q = 0; q < n; ++q) {
+        const pennant_problem* pb = pp->pc[(size_t)q];
+        for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+            const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+            const int sb = pb->hm.zsbeg[zb];
+            const int run = (int)(r / 3);
+            if (poff + pred_grid(pb, ze - zb, ns, form) > pp->npart) return PENNANT_EINVAL;
+            switch (ns) {
+                case 3: launch_pred<3>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                case 4: launch_pred<4>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                case 5: launch_pred<5>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                case 6: launch_pred<6>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                case 7: launch_pred<7>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                case 8: launch_pred<8>(pb, A[(size_t)q], zb, ze, sb, poff, form, run, s); break;
+                default: return PENNANT_EINVAL;
+            }
+            poff += pred_grid(pb, ze - zb, ns, form);
+        }
+    }
+    const int npred = poff;
+    const Ctl* ctl = pp->pc[0]->ctl;
+    if (pp->ncx > 0) {
+        if (mixed)
+            hipLaunchKernelGGL(k_corner_xchg<float>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s, pp->tab,
+                               pp->cx, pp->ncx, ctl);
+        else
+            hipLaunchKernelGGL(k_corner_xchg<double>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s, pp->tab,
+                               pp->cx, pp->ncx, ctl);
+    }
+    for (int q = 0; q < n; ++q) {
+        const pennant_problem* pb = pp->pc[(size_t)q];
+        const dim3 pgrid(pb->nheavy + nblocks(own_points(pb)));
+        if (mixed)
+            hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, s, A[(size_t)q]);
+        else
+            hipLaunchKernelGGL(k_point<double>, pgrid, dim3(PN_BLOCK), 0, s, A[(size_t)q]);
+    }
+    if (pp->nhalo > 0) {
+        if (mixed)
+            hipLaunchKernelGGL(k_halo<float>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s, pp->tab, pp->halo,
+                               pp->nhalo, ctl);
+        else
+            hipLaunchKernelGGL(k_halo<double>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s, pp->tab, pp->halo,
+                               pp->nhalo, ctl);
+    }
+    poff = 0;
+    for (int q = 0; q < n; ++q) {
+        const pennant_problem* pb = pp->pc[(size_t)q];
+        for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+            const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+            const int sb = pb->hm.zsbeg[zb];
+            if (poff + corr_grid(pb, ze - zb) > pp->npart) return PENNANT_EINVAL;
+            switch (ns) {
+                case 3: launch_corr<3>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                case 4: launch_corr<4>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                case 5: launch_corr<5>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                case 6: launch_corr<6>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                case 7: launch_corr<7>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                case 8: launch_corr<8>(pb, A[(size_t)q], zb, ze, sb, poff, form, s); break;
+                default: return PENNANT_EINVAL;
+            }
+            poff += corr_grid(pb, ze - zb);
+        }
+    }
+    const int ncorr = poff;
+    const Params& q = pp->prm;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, A[0], npred, ncorr, q.cflv, q.cstop,
+                       q.dtmax, q.dtfac, q.tstop, mixed ? 0 : 1);
+    if (mixed) pp->zre_stale = true;
+    return (int)hipGetLastError();
+}
+
+int pieces_materialize_zre(pennant_pieces* pp) {
+    if (!pp->zre_stale || pp->pc.empty()) return 0;
+    hipStream_t s = pp->pc[0]->stream;
+    for (size_t q = 0; q < pp->pc.size(); ++q) {
+        const int nz = pp->pc[q]->hm.numz;
+        if (nz > 0)
+            hipLaunchKernelGGL(k_zre, dim3(nblocks(nz)), dim3(PN_BLOCK), 0, s, piece_args(pp, (int)q), nz);
+    }
+    hipLaunchKernelGGL(k_zre_done, dim3(1), dim3(64), 0, s, pp->pc[0]->ctl, 1);
+    pp->zre_stale = false;
     return (int)hipGetLastError();
 }
 

@@ -145,6 +145,9 @@ struct pennant_problem {
     double* zpn = nullptr;           // the look-ahead predictor's zp
     bool zre_stale = false;          // mixed cycles ran since zr, ze were last materialised
     std::vector<int> zruns;          // [zbeg, zend, nsides] triples with uniform nsides
+    // a piece of a multi-piece run: the point pass updates points [0, npown)
+    // (private and master points); -1: every point
+    int npown = -1;
     int nbcs = 0;
     double bcvx[8], bcvy[8];
     double* bcv_dev = nullptr;       // device copy: bcvx then bcvy
@@ -254,4 +257,57 @@ int launch_first_dt(pennant_problem* pb);
 // px0 = px, pu0 = pu, pmaswt = 0, pf = 0
 int launch_copy_zvol0(pennant_problem* pb);
 int launch_point_begin(pennant_problem* pb);
+// Multi-piece runs (pieces.cpp, fused.hip): per-piece device pointers the
+// exchange kernels use, and one exchange entry (element src of piece sp to
+// element dst of piece dp).
+struct PieceTab {
+    double *pxx, *pxy, *pux, *puy, *vs2, *pxpx, *cfx, *cfy, *cmass;
+    float* fcrn;
+};
+struct XEnt {
+    int sp, src, dp, dst;
+};
+// deck parsing and the one-piece mesh generator (driver.cpp)
+bool read_params(const char* text, Params& q, std::string& err);
+void build_mesh(const Params& q, HostMesh& h);
+// a problem on a prebuilt mesh (driver.cpp); npown as pennant_problem::npown
+pennant_problem* create_problem(const Params& q, HostMesh&& h, int npown, std::string& err);
+// points the point pass updates
+inline int own_points(const pennant_problem* pb) {
+    return pb->npown < 0 ? pb->hm.nump : pb->npown;
+}
+}  // namespace pn
+
+// A mesh run as several pieces on one device (pennant_pieces_create).  Each
+// piece is a problem of its own over its zones, their sides and every point
+// they reach, numbered private points, master points (owned here, reached by
+// other pieces too), then ghost points (owned by another piece).  All pieces
+// share piece 0's control block and one array of block partials.
+struct pennant_pieces {
+    pn::Params prm;
+    int numpcs = 0, numpcx = 0, numpcy = 0;
+    std::vector<pennant_problem*> pc;             // the non-empty pieces, in piece order
+    std::vector<int> pid;                         // each one's piece number, py * numpcx + px
+    std::vector<std::vector<int>> zglob, sglob, pglob;   // local -> one-piece numbering
+    std::vector<int> nprv, nmstr, nghost, nrecv;  // nrecv: corners received for master points
+    int64_t numz = 0, nums = 0, nump = 0;         // the whole mesh
+    pn::PieceTab* tab = nullptr;                  // device, one per piece
+    pn::XEnt* halo = nullptr;                     // owner point -> ghost point
+    int nhalo = 0;
+    pn::XEnt* cx = nullptr;                       // corner slot at a ghost point -> owner's slot
+    int ncx = 0;
+    double* part_d = nullptr;                     // partials: [0, npart) K1's, [npart, 2 npart) K3's
+    int* part_i = nullptr;
+    int npart = 0;
+    bool zre_stale = false;                       // mixed cycles ran since zr, ze were stored
+    std::vector<void*> allocs;
+    std::string err;
+};
+
+namespace pn {
+// one cycle of every piece (fused.hip): K1 per piece, the corner exchange, K2
+// per piece, the halo, K3 per piece, one finalize
+int pieces_cycle(pennant_pieces* pp, int form);
+// zr and ze stored in every piece where mixed cycles left them derived
+int pieces_materialize_zre(pennant_pieces* pp);
 }  // namespace pn

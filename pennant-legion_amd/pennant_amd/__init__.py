@@ -179,6 +179,16 @@ DRIVER_SIGNATURES = {
                            ctypes.c_int),
     "pennant_get_tuning": ([_P, ctypes.c_char_p], ctypes.c_int),
     "pennant_lookahead_redos": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_pieces_create": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
+    "pennant_pieces_destroy": ([_P], None),
+    "pennant_pieces_error": ([_P], ctypes.c_char_p),
+    "pennant_pieces_count": ([_P], ctypes.c_int),
+    "pennant_pieces_layout": ([_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "pennant_pieces_run": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pennant_pieces_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_pieces_field_info": ([_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "pennant_pieces_get_field": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
 }
 
 
@@ -568,6 +578,73 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             lib().pennant_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+class Pieces:
+    """One mesh run as numpcs pieces on the current HIP device (the
+    reference's -n <numpcs>): private, master and ghost points per piece,
+    corners at ghost points summed by their owner in one-piece side order,
+    the owners' point state copied back to the ghosts every cycle."""
+
+    def __init__(self, deck_text: str, numpcs: int):
+        L = lib()
+        err = ctypes.create_string_buffer(512)
+        h = L.pennant_pieces_create(deck_text.encode(), int(numpcs), err, len(err))
+        if not h:
+            raise PennantError(err.value.decode())
+        self._h = ctypes.c_void_p(h)
+        self.count = int(L.pennant_pieces_count(self._h))
+
+    def _check(self, rc):
+        if rc == EBADSIDES:
+            raise PennantError(lib().pennant_pieces_error(self._h).decode())
+        if rc != 0:
+            raise PennantError(f"pennant pieces call failed ({rc})")
+        return rc
+
+    def layout(self, i: int) -> dict:
+        out = (ctypes.c_int64 * 8)()
+        if lib().pennant_pieces_layout(self._h, int(i), out) != 0:
+            raise IndexError(i)
+        keys = ("piece", "zones", "sides", "private", "master", "ghost", "received")
+        d = dict(zip(keys, (int(v) for v in out[:7])))
+        d["numpcx"], d["numpcy"] = int(out[7]) >> 16, int(out[7]) & 0xFFFF
+        return d
+
+    def run(self, ncycles: int, mode: int = MODE_FUSED) -> int:
+        rc = lib().pennant_pieces_run(self._h, int(ncycles), int(mode))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def scalars(self) -> dict:
+        out = (ctypes.c_double * 10)()
+        self._check(lib().pennant_pieces_scalars(self._h, out))
+        keys = ("time", "dt", "dtrec", "cycle", "dtnew", "dvovmax", "numsbad_pred",
+                "numsbad_corr", "dtnext", "done")
+        return dict(zip(keys, list(out)))
+
+    def get(self, name: str) -> np.ndarray:
+        n = ctypes.c_int64()
+        c = ctypes.c_int()
+        self._check(lib().pennant_pieces_field_info(self._h, name.encode(), ctypes.byref(n), ctypes.byref(c)))
+        out = np.empty(n.value * c.value, dtype=np.float64)
+        self._check(lib().pennant_pieces_get_field(self._h, name.encode(), out.ctypes.data))
+        return out.reshape(n.value, 2) if c.value == 2 else out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pennant_pieces_destroy(self._h)
             self._h = None
 
     def __enter__(self):
