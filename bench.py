@@ -114,6 +114,8 @@ def bench_ensemble(args, pa, decks):
         from pennant_amd.closure import Mlp
         from pennant_amd.train_closure import DEFAULT_OUT
         b.set_closure(Mlp.load(Path(args.closure) if args.closure else DEFAULT_OUT))
+    if args.graph is not None:
+        b.set_tuning("graph", args.graph)
     ms_tune = b.autotune(mode, cycles=4)
     log(f"autotune: max_blocks {b.get_tuning('max_blocks')} xcd {b.get_tuning('xcd')} ({ms_tune:.3f} ms / 4 cycles)")
     b.run(args.warmup, mode, raise_on_error=False)
@@ -139,12 +141,45 @@ def bench_ensemble(args, pa, decks):
     print(json.dumps(line), flush=True)
 
 
+def bench_pieces(args, pa, text, wname):
+    """The workload as --pieces pieces on the device against the one-piece
+    run: both advance --warmup cycles from the start, then --steps cycles
+    are timed (direct launches; the run in pieces launches each piece's
+    kernels and the two exchanges, so it pays launch overheads the
+    one-piece graph replay does not).  The end states are compared bit for
+    bit."""
+    import numpy as np
+    mode = {"fused": pa.MODE_FUSED, "mixed": pa.MODE_MIXED}.get(args.mode)
+    if mode is None:
+        raise SystemExit("--pieces runs the fused or mixed mode")
+    with pa.Problem(text) as one, pa.Pieces(text, args.pieces) as pp:
+        lay = [pp.layout(i) for i in range(pp.count)]
+        one.set_tuning("graph", 0)
+        one.run(args.warmup, mode)
+        pp.run(args.warmup, mode)
+        ms1, _ = one.time_cycles(args.steps, mode, phases=False)
+        msp = pp.time_cycles(args.steps, mode)
+        same = all(np.array_equal(one.get(f).view(np.uint64), pp.get(f).view(np.uint64))
+                   for f in ("px", "pu", "zr", "ze", "zp", "zetot"))
+        Z = one.numz
+    log(f"one piece: {ms1 / args.steps * 1e3:.1f} us/cycle; {pp.count} pieces: {msp / args.steps * 1e3:.1f} us/cycle; "
+        f"bit-identical: {same}")
+    line = {"metric": f"zone-cycles/s on 1 MI355X, {wname} as {pp.count} pieces", "value": round(Z * args.steps / (msp * 1e-3), 1),
+            "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(msp / args.steps, 5), "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f64" if mode == pa.MODE_FUSED else "f32+f64", "data": "synthetic",
+            "config": {"workload": wname, "pieces": pp.count, "numpcx": lay[0]["numpcx"], "numpcy": lay[0]["numpcy"],
+                       "ghost_points": sum(d["ghost"] for d in lay), "received_corners": sum(d["received"] for d in lay),
+                       "one_piece_us_per_cycle": round(ms1 / args.steps * 1e3, 2), "bit_identical": same}}
+    print(json.dumps(line), flush=True)
+
+
 PROFILE_KERNELS = {
     "fp64": {"zone_pred": "k_pred_zone<4>", "point": "k_point<double>", "zone_corr": "k_corr<4, double>"},
     "mixed": {"zone_pred": "k_pred_mixed<4, false, false>", "point": "k_point<float>",
-              "zone_corr": "k_corr<4, float>"},
+              "zone_corr": "k_corr<4, float>", "zone": "k_zone_pipe<4, false>"},
     "learned": {"zone_pred": "k_pred_mixed<4, true, false>", "point": "k_point<float>",
-                "zone_corr": "k_corr<4, float>"},
+                "zone_corr": "k_corr<4, float>", "zone": "k_zone_pipe<4, true>"},
 }
 
 
@@ -235,6 +270,9 @@ def main():
     ap.add_argument("--pipeline", type=int, default=None,
                     help="mixed forms: 1 pipelined cycle, 0 separate predictor and corrector "
                          "passes (the default)")
+    ap.add_argument("--pieces", type=int, default=0,
+                    help="N > 1: also time the workload run as N pieces on the device (the reference's "
+                         "-n N; fused and mixed modes) against the one-piece run, from the same start")
     ap.add_argument("--workload", default="sedov1440")
     ap.add_argument("--mode", default="learned", choices=["learned", "mixed", "fused", "tasks"],
                     help="learned: mixed precision with the learned QCS closure; mixed: mixed "
@@ -249,6 +287,9 @@ def main():
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
     ap.add_argument("--max-blocks", type=int, default=None, help="grid cap of the fused zone kernels")
     ap.add_argument("--xcd", type=int, default=None, help="XCD band mapping of the zone kernels (0/1)")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="0: launch every cycle's kernels directly instead of replaying 16-cycle HIP "
+                         "graphs (for tools that trace kernel launches)")
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of CPU time per oracle variant (0 disables)")
     args = ap.parse_args()
@@ -266,6 +307,9 @@ def main():
         bench_ensemble(args, pa, decks)
         return
     text, wname = workload_deck(args.workload, decks)
+    if args.pieces > 1:
+        bench_pieces(args, pa, text, wname)
+        return
     modes = {"learned": pa.MODE_LEARNED, "mixed": pa.MODE_MIXED, "fused": pa.MODE_FUSED,
              "tasks": pa.MODE_TASKS}
     mlp = None
@@ -295,6 +339,8 @@ def main():
             pb.set_tuning("xcd", args.xcd)
         if args.pipeline is not None:
             pb.set_tuning("pipeline", args.pipeline)
+        if args.graph is not None:
+            pb.set_tuning("graph", args.graph)
         e0 = sum(pb.energy())
         log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
         t0 = time.perf_counter()
@@ -377,7 +423,7 @@ def main():
     if roofline is not None:
         kb, per, names, pipelined = rdetail
         roofline["traffic"] = (profiled_traffic(args.workload, roofline["kernel"], pform)
-                               if main_r["tune"]["k1_layout"] == 0 and not pipelined else None)
+                               if main_r["tune"]["k1_layout"] == 0 else None)
         roofline["phase_us"] = {n: round(per[n] * 1e3, 2) for n in names
                                 if n != "unused" and (n != "learned_pass" or mode == pa.MODE_LEARNED)}
         roofline["cycle_alg_GBs"] = round(sum(kb.values()) / (ms_per_step * 1e-3) / 1e9, 1)

@@ -605,6 +605,13 @@ typedef struct pennant_pieces pennant_pieces;
  * rounding leaves empty are dropped) and initialises every piece.  NULL on
  * error with a message in errbuf. */
 pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* errbuf, int errlen);
+/* The partition alone, computed on the host (no device needed): out[0]
+ * numpcx, out[1] numpcy, out[2] the number n of non-empty pieces, then 7
+ * values per piece: piece number, zones, sides, private, master and ghost
+ * points, corners received.  Returns 3 + 7 n, or PENNANT_EINVAL (message in
+ * errbuf) for an invalid deck or an out array shorter than outlen needs. */
+int pennant_pieces_plan(const char* deck_text, int numpcs, int64_t* out, int64_t outlen, char* errbuf,
+                        int errlen);
 void pennant_pieces_destroy(pennant_pieces* pp);
 const char* pennant_pieces_error(pennant_pieces* pp);
 /* Number of (non-empty) pieces. */
@@ -617,6 +624,9 @@ int pennant_pieces_layout(pennant_pieces* pp, int piece, int64_t* out8);
 /* Runs up to ncycles cycles in PENNANT_MODE_FUSED or PENNANT_MODE_MIXED;
  * returns the cycles run, PENNANT_EBADSIDES or a negative HIP error. */
 int pennant_pieces_run(pennant_pieces* pp, int ncycles, int mode);
+/* Queues ncycles cycles between two events on the pieces' stream and waits;
+ * *total_ms receives their time. */
+int pennant_pieces_time_cycles(pennant_pieces* pp, int ncycles, int mode, double* total_ms);
 /* The shared control block (layout of pennant_scalars). */
 int pennant_pieces_scalars(pennant_pieces* pp, double* out10);
 /* Entries (whole mesh) and components of a field, as pennant_field_info. */

@@ -105,6 +105,18 @@ __device__ __forceinline__ int block_sum(int v) {
     return v;
 }
 
+// The block's index in a launch whose work is laid out for the 8 XCDs:
+// MI355X dispatches blocks to its XCDs round-robin by block id and each XCD
+// has its own L2, so XCD x (block ids b = x mod 8) takes the x-th contiguous
+// share of the logical blocks.  Neighbouring tiles -- which share points and
+// the side values of zones across a row -- then meet in the same L2.  A
+// bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n >> 3, r = n & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 inline int nblocks(int64_t n) { return (int)((n + PN_BLOCK - 1) / PN_BLOCK); }
 inline int nred_blocks(int64_t n) {
     const int b = nblocks(n);

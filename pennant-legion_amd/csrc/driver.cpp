@@ -706,7 +706,7 @@ int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
         return 0;
     }
     if (!strcmp(key, "k1_layout")) {
-        if (value < 0 || value > 2) return PENNANT_EINVAL;
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
         pb->tune.k1_layout = value;
         return 0;
     }
@@ -777,7 +777,7 @@ int pennant_autotune(pennant_problem* pb, int mode, int cycles, double* best_ms)
     // the mixed forms also try the pipelined cycle (it pays where the
     // volume-change limit seldom sets dt)
     const int npipe = (form == FORM_FP64 || pb->cap_buf) ? 1 : 2;
-    for (int layout = 0; layout <= (form == FORM_FP64 ? 2 : 0) && !rc; ++layout)
+    for (int layout = 0; layout <= (form == FORM_FP64 ? 1 : 0) && !rc; ++layout)
       for (int pipe = 0; pipe < npipe && !rc; ++pipe)
         for (int xcd = 0; xcd <= 1 && !rc; ++xcd)
             for (int g : grids) {

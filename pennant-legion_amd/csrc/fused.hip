@@ -63,6 +63,7 @@ struct FusedArgs {
     const double* smf;
     int nump;
     int npown;                 // K2 updates points [0, npown)
+    int la_npred;              // pipelined cycle: K2's last block folds a redone predictor's partials
     // state (in place)
     double* pxx; double* pxy; double* pux; double* puy;
     double* zr; double* ze; double* zetot; double* zwrate; double* zvol; double* zp;
@@ -476,208 +477,6 @@ __global__ void __launch_bounds__(PN_BLOCK) k_pred_zone(FusedArgs a, int zbeg, i
         if (z >= zend) continue;
         const double zdthyd = zone_pred<NS>(a, z, sbeg + (z - zbeg) * NS,
                                             side_w(z, zbeg, zend, sbeg, NS), pp, dt, bad);
-        // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
-        dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
-    }
-    const double m = block_min(dtmin);
-    const int b = block_sum(bad);
-    if (threadIdx.x == 0) {
-        a.part_dt[poff + blockIdx.x] = m;
-        a.part_bp[poff + blockIdx.x] = b;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// K1, zone per thread, with the next zone's inputs prefetched into LDS.
-//
-// The FP64 predictor keeps a whole zone in registers (about 240 VGPRs: two
-// waves per SIMD), so a thread has no registers to load the next zone into
-// while it works on this one, and every zone starts with a full memory round
-// trip.  Here the loads of zone i+1 go straight to LDS (global_load_lds, one
-// dword per lane and load; no VGPRs) while the thread computes zone i, and
-// the thread reads them back at the top of iteration i+1.  The LDS area is
-// per wave (dma_words dwords x 64 lanes), read back by the lane that loaded
-// it, so no barrier is needed.
-//
-// The hardware counts these loads in vmcnt, in issue order with the other
-// vector memory instructions, and the compiler does not know that the LDS
-// reads depend on them, so the waits are explicit:
-//   * before reading zone i+1's data: vmcnt(3 NS).  After the loads of zone
-//     i+1 the thread issues at least 3 NS more vector memory instructions --
-//     zone i's corner force and mass stores, 3 NS scattered stores that
-//     cannot merge and cannot move above the loads (they may alias the loads'
-//     sources) -- so vmcnt <= 3 NS means the prefetch has landed.  The first
-//     zone of a thread waits for vmcnt(0);
-//   * before the next prefetch overwrites the area: lgkmcnt(0) (the LDS
-//     reads of this zone are done).
-#define PN_DMA_MAXNS 4
-// rows of a wave's LDS area: x, u (2 dwords each) per point, 5 zone doubles,
-// smf, pcpos, then the next zone's points (mapsp1)
-template <int NS>
-constexpr int dma_words() { return 12 * NS + 10; }
-
-typedef __attribute__((address_space(3))) void lds_void_t;
-__device__ __forceinline__ void dma_dword(const void* g, unsigned* row) {
-    __builtin_amdgcn_global_load_lds(g, (lds_void_t*)row, 4, 0, 0);
-}
-__device__ __forceinline__ void dma_double(const double* g, unsigned (*w)[64], int j) {
-    dma_dword(g, w[j]);
-    dma_dword(reinterpret_cast<const char*>(g) + 4, w[j + 1]);
-}
-// s_waitcnt with only vmcnt (expcnt and lgkmcnt left at their maximum), or
-// only lgkmcnt; each followed by a compiler barrier so that no LDS read moves
-// above it
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-    __asm__ __volatile__("" ::: "memory");
-}
-__device__ __forceinline__ void wait_lgkmcnt0() {
-    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
-    __asm__ __volatile__("" ::: "memory");
-}
-
-template <int NS>
-__device__ __forceinline__ void dma_zone(const FusedArgs& a, int z, int sb, const int (&pp)[NS],
-                                         unsigned (*w)[64]) {
-    int j = 0;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        const int p = pp[k];
-        dma_double(a.pxx + p, w, j);
-        dma_double(a.pxy + p, w, j + 2);
-        dma_double(a.pux + p, w, j + 4);
-        dma_double(a.puy + p, w, j + 6);
-        j += 8;
-    }
-    dma_double(a.zm + z, w, j);
-    dma_double(a.zr + z, w, j + 2);
-    dma_double(a.ze + z, w, j + 4);
-    dma_double(a.zvol + z, w, j + 6);
-    dma_double(a.zwrate + z, w, j + 8);
-    j += 10;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) dma_double(a.smf + sb + k, w, j + 2 * k);
-    j += 2 * NS;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) dma_dword(a.pcpos + sb + k, w[j + k]);
-}
-// the points of the zone whose sides start at sb, to the rows after the zone's
-template <int NS>
-__device__ __forceinline__ void dma_points(const FusedArgs& a, int sb, unsigned (*w)[64]) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) dma_dword(a.mapsp1 + sb + k, w[11 * NS + 10 + k]);
-}
-// The zone's inputs from this lane's column of the wave's LDS area: row pair
-// (j, j + 1) is one ds_read2st64_b32 (rows are 256 bytes apart).  The reads
-// are one asm statement that ends with lgkmcnt(0), so every output is
-// complete when the statement ends; written as asm, they carry no implied
-// wait on the prefetch (wait_vmcnt before the statement provides it).
-#define PN_RD(o, j) "ds_read2st64_b32 %" #o ", %[a] offset0:" #j " offset1:" #j "+1\n"
-template <int NS>
-__device__ __forceinline__ void read_zone(unsigned (*w)[64], int lane, ZoneIn<NS>& in, int (&pn)[NS]);
-template <>
-__device__ __forceinline__ void read_zone<4>(unsigned (*w)[64], int lane, ZoneIn<4>& in, int (&pn)[4]) {
-    const unsigned addr = (unsigned)(uintptr_t)(lds_void_t*)&w[0][lane];
-    double v[25];
-    unsigned long long sl[2], pq[2];
-    __asm__ __volatile__(
-        PN_RD(0, 0) PN_RD(1, 2) PN_RD(2, 4) PN_RD(3, 6) PN_RD(4, 8) PN_RD(5, 10) PN_RD(6, 12)
-        PN_RD(7, 14) PN_RD(8, 16) PN_RD(9, 18) PN_RD(10, 20) PN_RD(11, 22) PN_RD(12, 24) PN_RD(13, 26)
-        PN_RD(14, 28) PN_RD(15, 30) PN_RD(16, 32) PN_RD(17, 34) PN_RD(18, 36) PN_RD(19, 38) PN_RD(20, 40)
-        PN_RD(21, 42) PN_RD(22, 44) PN_RD(23, 46) PN_RD(24, 48) PN_RD(25, 50) PN_RD(26, 52)
-        PN_RD(27, 54) PN_RD(28, 56)
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
-          "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9]), "=&v"(v[10]), "=&v"(v[11]), "=&v"(v[12]), "=&v"(v[13]),
-          "=&v"(v[14]), "=&v"(v[15]), "=&v"(v[16]), "=&v"(v[17]), "=&v"(v[18]), "=&v"(v[19]),
-          "=&v"(v[20]), "=&v"(v[21]), "=&v"(v[22]), "=&v"(v[23]), "=&v"(v[24]), "=&v"(sl[0]), "=&v"(sl[1]),
-          "=&v"(pq[0]), "=&v"(pq[1])
-        : [a] "v"(addr)
-        : "memory");
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        in.x[k] = mk(v[4 * k], v[4 * k + 1]);
-        in.u[k] = mk(v[4 * k + 2], v[4 * k + 3]);
-    }
-    in.zm = v[16]; in.zr = v[17]; in.ze = v[18]; in.zvol = v[19]; in.zwr = v[20];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) in.mf[k] = v[21 + k];
-    in.slot[0] = (int)(unsigned)sl[0]; in.slot[1] = (int)(unsigned)(sl[0] >> 32);
-    in.slot[2] = (int)(unsigned)sl[1]; in.slot[3] = (int)(unsigned)(sl[1] >> 32);
-    pn[0] = (int)(unsigned)pq[0]; pn[1] = (int)(unsigned)(pq[0] >> 32);
-    pn[2] = (int)(unsigned)pq[1]; pn[3] = (int)(unsigned)(pq[1] >> 32);
-}
-template <>
-__device__ __forceinline__ void read_zone<3>(unsigned (*w)[64], int lane, ZoneIn<3>& in, int (&pn)[3]) {
-    const unsigned addr = (unsigned)(uintptr_t)(lds_void_t*)&w[0][lane];
-    double v[20];
-    unsigned long long sl, pq[2];
-    __asm__ __volatile__(
-        PN_RD(0, 0) PN_RD(1, 2) PN_RD(2, 4) PN_RD(3, 6) PN_RD(4, 8) PN_RD(5, 10) PN_RD(6, 12)
-        PN_RD(7, 14) PN_RD(8, 16) PN_RD(9, 18) PN_RD(10, 20) PN_RD(11, 22) PN_RD(12, 24) PN_RD(13, 26)
-        PN_RD(14, 28) PN_RD(15, 30) PN_RD(16, 32) PN_RD(17, 34) PN_RD(18, 36) PN_RD(19, 38)
-        PN_RD(20, 40) PN_RD(21, 42) PN_RD(22, 44)
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]),
-          "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9]), "=&v"(v[10]), "=&v"(v[11]), "=&v"(v[12]), "=&v"(v[13]),
-          "=&v"(v[14]), "=&v"(v[15]), "=&v"(v[16]), "=&v"(v[17]), "=&v"(v[18]), "=&v"(v[19]),
-          "=&v"(sl), "=&v"(pq[0]), "=&v"(pq[1])
-        : [a] "v"(addr)
-        : "memory");
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        in.x[k] = mk(v[4 * k], v[4 * k + 1]);
-        in.u[k] = mk(v[4 * k + 2], v[4 * k + 3]);
-    }
-    in.zm = v[12]; in.zr = v[13]; in.ze = v[14]; in.zvol = v[15]; in.zwr = v[16];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) in.mf[k] = v[17 + k];
-    in.slot[0] = (int)(unsigned)sl; in.slot[1] = (int)(unsigned)(sl >> 32);
-    in.slot[2] = (int)(unsigned)pq[0];
-    pn[0] = (int)(unsigned)(pq[0] >> 32); pn[1] = (int)(unsigned)pq[1]; pn[2] = (int)(unsigned)(pq[1] >> 32);
-}
-#undef PN_RD
-
-template <int NS>
-__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) k_pred_dma(FusedArgs a, int zbeg, int zend, int sbeg,
-                                                       int poff) {
-    static_assert(NS <= PN_DMA_MAXNS, "LDS area per wave");
-    __shared__ unsigned dbuf[PN_BLOCK / 64][dma_words<NS>()][64];
-    if (a.ctl->done) return;
-    const double dt = a.ctl->dt;
-    double dtmin = 1.e99;
-    int bad = 0;
-    const int lane = threadIdx.x & 63;
-    unsigned (*w)[64] = dbuf[threadIdx.x >> 6];
-    const int nunits = (zend - zbeg + PN_BLOCK - 1) / PN_BLOCK;
-    Units u = block_units(nunits, a.xcd);
-    // zone i (z) has its inputs in flight to LDS, and the points of zone i+1
-    // (zn) follow them there
-    int z = zbeg + u.u * PN_BLOCK + threadIdx.x;
-    int zn = zbeg + (u.u + u.step) * PN_BLOCK + threadIdx.x;
-    if (u.u < u.end && z < zend) {
-        int pp[NS];
-        load_pp<NS>(a, sbeg + (z - zbeg) * NS, pp);
-        dma_zone<NS>(a, z, sbeg + (z - zbeg) * NS, pp, w);
-    }
-    if (u.u + u.step < u.end && zn < zend) dma_points<NS>(a, sbeg + (zn - zbeg) * NS, w);
-    wait_vmcnt<0>();
-    bool first = true;
-    for (; u.u < u.end; u.u += u.step) {
-        if (!first) wait_vmcnt<3 * NS>();
-        first = false;
-        if (z >= zend) break;               // every later unit of this thread lies past zend too
-        ZoneIn<NS> in;
-        int pn[NS];
-        read_zone<NS>(w, lane, in, pn);     // ends with lgkmcnt(0): the area may be refilled
-        const int zc = z;
-        if (u.u + u.step < u.end && zn < zend) dma_zone<NS>(a, zn, sbeg + (zn - zbeg) * NS, pn, w);
-        z = zn;
-        zn = zbeg + (u.u + 2 * u.step) * PN_BLOCK + threadIdx.x;
-        if (u.u + 2 * u.step < u.end && zn < zend) dma_points<NS>(a, sbeg + (zn - zbeg) * NS, w);
-        const double zdthyd = zone_pred_core<NS>(a, zc, side_w(zc, zbeg, zend, sbeg, NS), in, dt, bad);
         // the running minimum of src/Hydro.cc:1615 (a NaN never replaces it)
         dtmin = (zdthyd < dtmin) ? zdthyd : dtmin;
     }
@@ -1703,8 +1502,19 @@ __device__ __forceinline__ void point_light(const FusedArgs& a, int p, double dt
 // K2: blocks [0, nheavy) take the heavy points, the rest one point per thread.
 // Block 0 also clears the learned pass's zone counts: that pass has run
 // before K2, and the next predictor appends to them.
+template <bool REDO>
+__device__ __forceinline__ void fold_lookahead(const FusedArgs& a, int npred);
+
 template <typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_point(FusedArgs a) {
+    // The pipelined cycle's K2 has one block more: it folds the partials of
+    // the previous cycle's redone predictor, if there was one, into the
+    // look-ahead values -- the work of a separate one-block launch after the
+    // redo, here before the zone pass overwrites the partials.
+    if (a.la_npred > 0 && (int)blockIdx.x == (int)gridDim.x - 1) {
+        fold_lookahead<true>(a, a.la_npred);
+        return;
+    }
     if (a.zcount && blockIdx.x == 0)
         for (int i = threadIdx.x; i < a.nzruns * PN_LSLICES; i += PN_BLOCK) a.zcount[i * PN_LSTRIDE] = 0u;
     if (a.ctl->done) return;
@@ -2204,9 +2014,9 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_pipe(FusedArgs a, int
 
 // After a predictor of the pipelined cycle (the prologue's, or with REDO a
 // redone one): its dt limit and bad-side count become the Ctl::la_* values
-// the next finalize ends the cycle with.
+// the next finalize ends the cycle with.  One block of PN_FIN_BLOCK threads.
 template <bool REDO>
-__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_la(FusedArgs a, int npred) {
+__device__ __forceinline__ void fold_lookahead(const FusedArgs& a, int npred) {
     Ctl* c = a.ctl;
     if (c->done) return;
     if (REDO && !c->redo) return;
@@ -2224,12 +2034,16 @@ __global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_la(FusedArgs a, int n
     c->la_bp = bp;
     c->redo = 0;
 }
+template <bool REDO>
+__global__ void __launch_bounds__(PN_FIN_BLOCK) k_finalize_la(FusedArgs a, int npred) {
+    fold_lookahead<REDO>(a, npred);
+}
 
 FusedArgs make_args(pennant_problem* pb) {
     FusedArgs a;
     const Params& q = pb->prm;
     a.mapsp1 = pb->dm.mapsp1; a.pcbeg = pb->dm.pcbeg; a.pcpos = pb->pcpos;
-    a.smf = pb->f("smf"); a.nump = pb->dm.nump; a.npown = own_points(pb);
+    a.smf = pb->f("smf"); a.nump = pb->dm.nump; a.npown = own_points(pb); a.la_npred = 0;
     a.pxx = pb->f("px", 0); a.pxy = pb->f("px", 1); a.pux = pb->f("pu", 0); a.puy = pb->f("pu", 1);
     a.zr = pb->f("zr"); a.ze = pb->f("ze"); a.zetot = pb->f("zetot"); a.zwrate = pb->f("zwrate");
     a.zvol = pb->f("zvol"); a.zp = pb->f("zp"); a.zm = pb->f("zm");
@@ -2308,9 +2122,6 @@ void launch_pred(const pennant_problem* pb, const FusedArgs& a, int zb, int ze, 
         hipLaunchKernelGGL((k_pred_mixed<NS, false>), grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff, run);
     else if (pb->tune.k1_layout == 1)
         hipLaunchKernelGGL(k_pred_corner<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
-    else if (pb->tune.k1_layout == 2 && NS <= PN_DMA_MAXNS)
-        hipLaunchKernelGGL(k_pred_dma<(NS <= PN_DMA_MAXNS ? NS : PN_DMA_MAXNS)>, grid, dim3(PN_BLOCK), 0, s, a,
-                           zb, ze, sb, poff);
     else
         hipLaunchKernelGGL(k_pred_zone<NS>, grid, dim3(PN_BLOCK), 0, s, a, zb, ze, sb, poff);
 }
@@ -2489,6 +2300,7 @@ BatchArgs make_batch_args(pennant_batch* b) {
     a.mapsp1 = b->mapsp1; a.pcbeg = b->pcbeg; a.pcpos = b->pcpos; a.smf = b->smf;
     a.nump = (int)b->nump;
     a.npown = a.nump;
+    a.la_npred = 0;
     a.pxx = b->pxx; a.pxy = b->pxy; a.pux = b->pux; a.puy = b->puy;
     a.zr = b->zr; a.ze = b->ze; a.zetot = b->zetot; a.zwrate = b->zwrate; a.zvol = b->zvol;
     a.zp = b->zp; a.zm = b->zm;
@@ -2787,8 +2599,17 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     hipStream_t s = pb->stream;
     const HostMesh& h = pb->hm;
     const bool fl = (form == FORM_LEARNED);
+    // partials of one predictor pass (the zone pass's, or its redo's)
+    int nparts = 0;
+    for (size_t r = 0; r < pb->zruns.size(); r += 3)
+        nparts += pred_grid(pb, pb->zruns[r + 1] - pb->zruns[r], pb->zruns[r + 2], form);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(k_point<float>, dim3(pb->nheavy + nblocks(own_points(pb))), dim3(PN_BLOCK), 0, s, a);
+    {
+        FusedArgs a2 = a;
+        a2.la_npred = nparts;   // and one block more, for the look-ahead fold
+        hipLaunchKernelGGL(k_point<float>, dim3(pb->nheavy + nblocks(own_points(pb)) + 1), dim3(PN_BLOCK), 0, s,
+                           a2);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
     int poff = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
@@ -2813,9 +2634,11 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(k_finalize_pipe, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, poff, poff, q);
+    // the redo, when the look-ahead dt was not the cycle's dt; its partials
+    // are folded by the next cycle's K2 (every launch sequence that does not
+    // continue with this form's cycles starts again from the prologue)
     const int npred = launch_lookahead_pred<true>(pb, a, form, s);
-    if (npred != poff) return PENNANT_EINVAL;
-    hipLaunchKernelGGL(k_finalize_la<true>, dim3(1), dim3(PN_FIN_BLOCK), 0, s, a, npred);
+    if (npred != poff || npred != nparts) return PENNANT_EINVAL;
     if (ev) (void)hipEventRecord(ev[3], s);
     if (fl) {
         const int rc = launch_learned_pass(pb, a, true, s);

@@ -79,30 +79,28 @@ int fused_form_of(int mode) {
     }
 }
 
-}  // namespace
+// The partition of a mesh into pieces, computed on the host: which zones,
+// sides and points each piece holds, their local numbering and meshes, and
+// the corners that master points receive.
+struct Recv {
+    int owner, lp, c, k, gside;   // owner piece index, master point, position, CSR index, side
+};
+struct Plan {
+    int numpcx = 1, numpcy = 1;
+    std::vector<int> pid, own, idx_of_piece, slocal, spiece;
+    std::vector<std::vector<int>> zglob, sglob, pglob, plocal;
+    std::vector<int> nprv, nmstr, nghost, nrecv;
+    std::vector<Recv> recvs;
+    std::vector<HostMesh> meshes;
+};
 
-extern "C" {
-
-pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* errbuf, int errlen) {
-    pennant_pieces* pp = new pennant_pieces();
-    auto fail = [&](const std::string& msg) -> pennant_pieces* {
-        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", msg.c_str());
-        pennant_pieces_destroy(pp);
-        return nullptr;
-    };
-    std::string err;
-    if (numpcs < 1) return fail("numpcs must be at least 1");
-    if (!read_params(deck_text, pp->prm, err)) return fail(err);
-    const Params& q = pp->prm;
+bool plan_pieces(const Params& q, const HostMesh& g, int numpcs, Plan& pl, std::string& err) {
     const bool pie = (q.meshtype == "pie");
-    if (!pie && q.meshtype != "rect") return fail("meshtype " + q.meshtype + " is not supported");
-    HostMesh g;
-    build_mesh(q, g);
-    pp->numz = g.numz; pp->nums = g.nums; pp->nump = g.nump;
+    if (!pie && q.meshtype != "rect") { err = "meshtype " + q.meshtype + " is not supported"; return false; }
     const long nzx = q.nzx, nzy = q.nzy, npx = nzx + 1;
     long numpcx = 1, numpcy = 1;
     piece_grid(numpcs, nzx, nzy, numpcx, numpcy);
-    pp->numpcs = numpcs; pp->numpcx = (int)numpcx; pp->numpcy = (int)numpcy;
+    pl.numpcx = (int)numpcx; pl.numpcy = (int)numpcy;
     const long zpx = (nzx + numpcx - 1) / numpcx, zpy = (nzy + numpcy - 1) / numpcy;
 
     // zone -> piece: the block holding its (i, j) (src/GenMesh.cc:898-975)
@@ -111,7 +109,8 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
     // point -> owner: the block of the zone whose first corner it is, the
     // last row and column folded into the blocks before them; the pie
     // origin belongs to piece 0 (src/GenMesh.cc:790-873)
-    std::vector<int> own((size_t)g.nump);
+    std::vector<int>& own = pl.own;
+    own.assign((size_t)g.nump, 0);
     for (long p = 0; p < g.nump; ++p) {
         long i, j;
         if (!pie) { i = p % npx; j = p / npx; }
@@ -127,37 +126,36 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
         else owner_reaches[(size_t)p] = 1;
     }
     for (long p = 0; p < g.nump; ++p)
-        if (!owner_reaches[(size_t)p]) return fail("a point is not reached by its owner piece");
+        if (!owner_reaches[(size_t)p]) { err = "a point is not reached by its owner piece"; return false; }
 
     // per piece: zones, sides and points in local order
     const int npc_all = (int)(numpcx * numpcy);
     std::vector<std::vector<int>> zl((size_t)npc_all);
     for (int z = 0; z < g.numz; ++z) zl[(size_t)zpc[(size_t)z]].push_back(z);
-    std::vector<int> slocal((size_t)g.nums, -1), spiece((size_t)g.nums, -1);   // global side -> (piece index, local side)
-    std::vector<int> idx_of_piece((size_t)npc_all, -1);
+    pl.slocal.assign((size_t)g.nums, -1);      // global side -> local side in its piece
+    pl.spiece.assign((size_t)g.nums, -1);      // global side -> piece index
+    pl.idx_of_piece.assign((size_t)npc_all, -1);
     for (int c = 0; c < npc_all; ++c) {
         if (zl[(size_t)c].empty()) continue;            // pieces left empty by the block rounding
-        idx_of_piece[(size_t)c] = (int)pp->pid.size();
-        pp->pid.push_back(c);
+        pl.idx_of_piece[(size_t)c] = (int)pl.pid.size();
+        pl.pid.push_back(c);
     }
-    const int npc = (int)pp->pid.size();
-    pp->zglob.resize((size_t)npc); pp->sglob.resize((size_t)npc); pp->pglob.resize((size_t)npc);
-    pp->nprv.assign((size_t)npc, 0); pp->nmstr.assign((size_t)npc, 0);
-    pp->nghost.assign((size_t)npc, 0); pp->nrecv.assign((size_t)npc, 0);
-    std::vector<std::vector<int>> plocal((size_t)npc);      // per piece: global point -> local, -1
-    struct Recv { int owner, lp, c, k, gside; };           // a corner a master point receives
-    std::vector<Recv> recvs;
-    std::vector<HostMesh> meshes((size_t)npc);
+    const int npc = (int)pl.pid.size();
+    pl.zglob.resize((size_t)npc); pl.sglob.resize((size_t)npc); pl.pglob.resize((size_t)npc);
+    pl.plocal.resize((size_t)npc);
+    pl.nprv.assign((size_t)npc, 0); pl.nmstr.assign((size_t)npc, 0);
+    pl.nghost.assign((size_t)npc, 0); pl.nrecv.assign((size_t)npc, 0);
+    pl.meshes.resize((size_t)npc);
     for (int qi = 0; qi < npc; ++qi) {
-        const int c = pp->pid[(size_t)qi];
-        std::vector<int>& zg = pp->zglob[(size_t)qi];
-        std::vector<int>& sg = pp->sglob[(size_t)qi];
-        std::vector<int>& pg = pp->pglob[(size_t)qi];
+        const int c = pl.pid[(size_t)qi];
+        std::vector<int>& zg = pl.zglob[(size_t)qi];
+        std::vector<int>& sg = pl.sglob[(size_t)qi];
+        std::vector<int>& pg = pl.pglob[(size_t)qi];
         zg = zl[(size_t)c];
         for (int z : zg)
             for (int s = g.zsbeg[(size_t)z]; s < g.zsbeg[(size_t)z + 1]; ++s) {
-                slocal[(size_t)s] = (int)sg.size();
-                spiece[(size_t)s] = qi;
+                pl.slocal[(size_t)s] = (int)sg.size();
+                pl.spiece[(size_t)s] = qi;
                 sg.push_back(s);
             }
         std::vector<char> reach((size_t)g.nump, 0);
@@ -169,27 +167,30 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
             else if (shared[(size_t)p]) mst.push_back(p);
             else prv.push_back(p);
         }
-        pp->nprv[(size_t)qi] = (int)prv.size();
-        pp->nmstr[(size_t)qi] = (int)mst.size();
-        pp->nghost[(size_t)qi] = (int)gst.size();
+        pl.nprv[(size_t)qi] = (int)prv.size();
+        pl.nmstr[(size_t)qi] = (int)mst.size();
+        pl.nghost[(size_t)qi] = (int)gst.size();
         pg = prv;
         pg.insert(pg.end(), mst.begin(), mst.end());
         pg.insert(pg.end(), gst.begin(), gst.end());
-        std::vector<int>& pl = plocal[(size_t)qi];
-        pl.assign((size_t)g.nump, -1);
-        for (size_t i = 0; i < pg.size(); ++i) pl[(size_t)pg[i]] = (int)i;
+        std::vector<int>& plc = pl.plocal[(size_t)qi];
+        plc.assign((size_t)g.nump, -1);
+        for (size_t i = 0; i < pg.size(); ++i) plc[(size_t)pg[i]] = (int)i;
     }
     // local meshes; a master point's corner list spans every piece
     for (int qi = 0; qi < npc; ++qi) {
-        const int c = pp->pid[(size_t)qi];
-        const std::vector<int>& zg = pp->zglob[(size_t)qi];
-        const std::vector<int>& sg = pp->sglob[(size_t)qi];
-        const std::vector<int>& pg = pp->pglob[(size_t)qi];
-        const std::vector<int>& pl = plocal[(size_t)qi];
-        HostMesh& h = meshes[(size_t)qi];
+        const int c = pl.pid[(size_t)qi];
+        const std::vector<int>& zg = pl.zglob[(size_t)qi];
+        const std::vector<int>& sg = pl.sglob[(size_t)qi];
+        const std::vector<int>& pg = pl.pglob[(size_t)qi];
+        const std::vector<int>& plc = pl.plocal[(size_t)qi];
+        HostMesh& h = pl.meshes[(size_t)qi];
         h.numz = (int)zg.size(); h.nums = (int)sg.size(); h.nump = (int)pg.size();
         h.px.resize((size_t)h.nump); h.py.resize((size_t)h.nump);
-        for (int i = 0; i < h.nump; ++i) { h.px[(size_t)i] = g.px[(size_t)pg[(size_t)i]]; h.py[(size_t)i] = g.py[(size_t)pg[(size_t)i]]; }
+        for (int i = 0; i < h.nump; ++i) {
+            h.px[(size_t)i] = g.px[(size_t)pg[(size_t)i]];
+            h.py[(size_t)i] = g.py[(size_t)pg[(size_t)i]];
+        }
         h.znump.resize((size_t)h.numz); h.zsbeg.resize((size_t)h.numz + 1);
         std::vector<int> zloc((size_t)g.numz, -1);
         for (int i = 0; i < h.numz; ++i) zloc[(size_t)zg[(size_t)i]] = i;
@@ -206,14 +207,14 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
         h.mapss3.resize((size_t)h.nums); h.mapss4.resize((size_t)h.nums);
         for (int i = 0; i < h.nums; ++i) {
             const int s = sg[(size_t)i];
-            h.mapsp1[(size_t)i] = pl[(size_t)g.mapsp1[(size_t)s]];
-            h.mapsp2[(size_t)i] = pl[(size_t)g.mapsp2[(size_t)s]];
+            h.mapsp1[(size_t)i] = plc[(size_t)g.mapsp1[(size_t)s]];
+            h.mapsp2[(size_t)i] = plc[(size_t)g.mapsp2[(size_t)s]];
             h.mapsz[(size_t)i] = zloc[(size_t)g.mapsz[(size_t)s]];
-            h.mapss3[(size_t)i] = slocal[(size_t)g.mapss3[(size_t)s]];
-            h.mapss4[(size_t)i] = slocal[(size_t)g.mapss4[(size_t)s]];
+            h.mapss3[(size_t)i] = pl.slocal[(size_t)g.mapss3[(size_t)s]];
+            h.mapss4[(size_t)i] = pl.slocal[(size_t)g.mapss4[(size_t)s]];
         }
         h.cyclic = true;
-        const int nown = pp->nprv[(size_t)qi] + pp->nmstr[(size_t)qi];
+        const int nown = pl.nprv[(size_t)qi] + pl.nmstr[(size_t)qi];
         h.pcbeg.assign((size_t)h.nump + 1, 0);
         h.pcside.clear();
         for (int lp = 0; lp < h.nump; ++lp) {
@@ -224,11 +225,11 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
                 const int s = g.pcside[(size_t)k];          // ascending one-piece side order
                 const bool mine = (zpc[(size_t)g.mapsz[(size_t)s]] == c);
                 if (mine) {
-                    h.pcside.push_back(slocal[(size_t)s]);
-                } else if (lp >= pp->nprv[(size_t)qi] && lp < nown) {
-                    recvs.push_back(Recv{qi, lp, cpos, (int)h.pcside.size(), s});
+                    h.pcside.push_back(pl.slocal[(size_t)s]);
+                } else if (lp >= pl.nprv[(size_t)qi] && lp < nown) {
+                    pl.recvs.push_back(Recv{qi, lp, cpos, (int)h.pcside.size(), s});
                     h.pcside.push_back(-1);
-                    pp->nrecv[(size_t)qi] += 1;
+                    pl.nrecv[(size_t)qi] += 1;
                 } else {
                     continue;
                 }
@@ -236,8 +237,43 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
             }
         }
         h.pcbeg[(size_t)h.nump] = (int)h.pcside.size();
-        if (h.maxznump > PENNANT_MAX_ZSIDES) return fail("zones with more than 8 sides are not supported");
+        if (h.maxznump > PENNANT_MAX_ZSIDES) { err = "zones with more than 8 sides are not supported"; return false; }
     }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* errbuf, int errlen) {
+    pennant_pieces* pp = new pennant_pieces();
+    auto fail = [&](const std::string& msg) -> pennant_pieces* {
+        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", msg.c_str());
+        pennant_pieces_destroy(pp);
+        return nullptr;
+    };
+    std::string err;
+    if (numpcs < 1) return fail("numpcs must be at least 1");
+    if (!read_params(deck_text, pp->prm, err)) return fail(err);
+    const Params& q = pp->prm;
+    HostMesh g;
+    build_mesh(q, g);
+    pp->numz = g.numz; pp->nums = g.nums; pp->nump = g.nump;
+    Plan pl;
+    if (!plan_pieces(q, g, numpcs, pl, err)) return fail(err);
+    pp->numpcs = numpcs; pp->numpcx = pl.numpcx; pp->numpcy = pl.numpcy;
+    pp->pid = pl.pid;
+    pp->zglob = pl.zglob; pp->sglob = pl.sglob; pp->pglob = pl.pglob;
+    pp->nprv = pl.nprv; pp->nmstr = pl.nmstr; pp->nghost = pl.nghost; pp->nrecv = pl.nrecv;
+    const int npc = (int)pl.pid.size();
+    const std::vector<int>& own = pl.own;
+    const std::vector<int>& idx_of_piece = pl.idx_of_piece;
+    const std::vector<int>& slocal = pl.slocal;
+    const std::vector<int>& spiece = pl.spiece;
+    const std::vector<std::vector<int>>& plocal = pl.plocal;
+    std::vector<HostMesh>& meshes = pl.meshes;
+    const std::vector<Recv>& recvs = pl.recvs;
     // the pieces, each initialised as the one-piece run initialises its zones and points
     for (int qi = 0; qi < npc; ++qi) {
         const int nown = pp->nprv[(size_t)qi] + pp->nmstr[(size_t)qi];
@@ -299,6 +335,41 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
     return pp;
 }
 
+// The partition alone, on the host (no device): out[0] numpcx, out[1]
+// numpcy, out[2] the number of non-empty pieces n, then per piece 7 values:
+// piece number, zones, sides, private, master, ghost and received corners.
+// Returns the number of int64 values written (3 + 7 n), or PENNANT_EINVAL
+// with a message in errbuf when the deck is invalid or out is too small.
+int pennant_pieces_plan(const char* deck_text, int numpcs, int64_t* out, int64_t outlen, char* errbuf,
+                        int errlen) {
+    Params q;
+    std::string err;
+    auto fail = [&](const std::string& msg) {
+        if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", msg.c_str());
+        return PENNANT_EINVAL;
+    };
+    if (numpcs < 1) return fail("numpcs must be at least 1");
+    if (!read_params(deck_text, q, err)) return fail(err);
+    HostMesh g;
+    build_mesh(q, g);
+    Plan pl;
+    if (!plan_pieces(q, g, numpcs, pl, err)) return fail(err);
+    const int64_t n = (int64_t)pl.pid.size();
+    if (outlen < 3 + 7 * n) return fail("output array too small");
+    out[0] = pl.numpcx; out[1] = pl.numpcy; out[2] = n;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t* o = out + 3 + 7 * i;
+        o[0] = pl.pid[(size_t)i];
+        o[1] = (int64_t)pl.zglob[(size_t)i].size();
+        o[2] = (int64_t)pl.sglob[(size_t)i].size();
+        o[3] = pl.nprv[(size_t)i];
+        o[4] = pl.nmstr[(size_t)i];
+        o[5] = pl.nghost[(size_t)i];
+        o[6] = pl.nrecv[(size_t)i];
+    }
+    return (int)(3 + 7 * n);
+}
+
 void pennant_pieces_destroy(pennant_pieces* pp) {
     if (!pp) return;
     for (pennant_problem* pb : pp->pc) pennant_destroy(pb);
@@ -348,6 +419,28 @@ int pennant_pieces_run(pennant_pieces* pp, int ncycles, int mode) {
         return PENNANT_EBADSIDES;
     }
     return c.cycle - c0;
+}
+
+// ncycles cycles between two events on the pieces' stream (no host wait in
+// between); *total_ms their time.
+int pennant_pieces_time_cycles(pennant_pieces* pp, int ncycles, int mode, double* total_ms) {
+    const int form = fused_form_of(mode);
+    if (form < 0 || pp->pc.empty()) return PENNANT_EINVAL;
+    hipStream_t s = pp->pc[0]->stream;
+    int rc = 0;
+    if (form == FORM_FP64 && (rc = pieces_materialize_zre(pp))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((rc = (int)hipEventCreate(&e0)) || (rc = (int)hipEventCreate(&e1))) return rc;
+    rc = (int)hipEventRecord(e0, s);
+    for (int i = 0; i < ncycles && !rc; ++i) rc = pieces_cycle(pp, form);
+    if (!rc) rc = (int)hipEventRecord(e1, s);
+    if (!rc) rc = (int)hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (!rc) rc = (int)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (total_ms) *total_ms = ms;
+    return rc;
 }
 
 int pennant_pieces_scalars(pennant_pieces* pp, double* o) {

@@ -22,6 +22,11 @@
 //     by the whole block (contributions staged in LDS, added by one thread in
 //     ascending order), which keeps the pie origin's 1024 corners off the
 //     critical path of one lane.
+//   * The point gathers are laid out for the 8 XCDs (xcd_block): each XCD's
+//     L2 holds a contiguous share of the points, so the side values that a
+//     point's corners read again for the point across the zone (mapss3) are
+//     still there (HBM traffic 1.02x compulsory instead of 2.1x).  The zone
+//     tiles keep the plain block order, which measured faster for them.
 //   * Reductions (calcDtNew, calcDvol) use wave64 shuffles and per-block
 //     partials, then a one-block fold (reduce.h).
 //   * Vector (double2) fields come as {x, y, stride}: every kernel that takes
@@ -318,7 +323,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_calc_crnr_mass(pennant_mesh m, con
     __shared__ double part[PN_BLOCK];
     if (threadIdx.x == 0) hv.n = 0;
     __syncthreads();
-    const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
+    const int p = xcd_block() * PN_BLOCK + threadIdx.x;
     if (p < m.nump) {
         const int kb = m.pcbeg[p], ke = m.pcbeg[p + 1];
         if (ke - kb > PN_PLIGHT) {
@@ -384,7 +389,7 @@ __global__ void __launch_bounds__(PN_BLOCK) k_sum_crnr_force(pennant_mesh m, pen
     __shared__ d2 part[PN_BLOCK];
     if (threadIdx.x == 0) hv.n = 0;
     __syncthreads();
-    const int p = blockIdx.x * PN_BLOCK + threadIdx.x;
+    const int p = xcd_block() * PN_BLOCK + threadIdx.x;
     if (p < m.nump) {
         const int kb = m.pcbeg[p], ke = m.pcbeg[p + 1];
         if (ke - kb > PN_PLIGHT) {

@@ -181,11 +181,15 @@ DRIVER_SIGNATURES = {
     "pennant_lookahead_redos": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "pennant_pieces_create": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int], ctypes.c_void_p),
     "pennant_pieces_destroy": ([_P], None),
+    "pennant_pieces_plan": ([ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64,
+                             ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "pennant_pieces_error": ([_P], ctypes.c_char_p),
     "pennant_pieces_count": ([_P], ctypes.c_int),
     "pennant_pieces_layout": ([_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "pennant_pieces_run": ([_P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pennant_pieces_scalars": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "pennant_pieces_time_cycles": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                                   ctypes.c_int),
     "pennant_pieces_field_info": ([_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                    ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "pennant_pieces_get_field": ([_P, ctypes.c_char_p, _P], ctypes.c_int),
@@ -627,6 +631,12 @@ class Pieces:
             self._check(rc)
         return rc
 
+    def time_cycles(self, ncycles: int, mode: int = MODE_FUSED) -> float:
+        """Milliseconds for ncycles cycles, HIP events on the pieces' stream."""
+        ms = ctypes.c_double()
+        self._check(lib().pennant_pieces_time_cycles(self._h, int(ncycles), int(mode), ctypes.byref(ms)))
+        return ms.value
+
     def scalars(self) -> dict:
         out = (ctypes.c_double * 10)()
         self._check(lib().pennant_pieces_scalars(self._h, out))
@@ -655,3 +665,18 @@ class Pieces:
 
     def __del__(self):
         self.close()
+
+
+def pieces_plan(deck_text: str, numpcs: int) -> dict:
+    """The partition of a deck's mesh into numpcs pieces, computed on the host
+    (no device): numpcx, numpcy and per non-empty piece its number, zones,
+    sides, private, master and ghost points and received corners."""
+    n = 3 + 7 * max(int(numpcs), 1)
+    out = (ctypes.c_int64 * n)()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().pennant_pieces_plan(deck_text.encode(), int(numpcs), out, n, err, len(err))
+    if rc < 0:
+        raise PennantError(err.value.decode())
+    keys = ("piece", "zones", "sides", "private", "master", "ghost", "received")
+    pieces = [dict(zip(keys, (int(v) for v in out[3 + 7 * i: 10 + 7 * i]))) for i in range(int(out[2]))]
+    return {"numpcx": int(out[0]), "numpcy": int(out[1]), "pieces": pieces}

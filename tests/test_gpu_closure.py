@@ -9,9 +9,10 @@
   * training: deterministic, and it lowers the loss on exact QCS targets;
   * capture: the tuples a mixed cycle records against the oracle's features
     and targets of the same state;
-  * in the loop: the learned form against the exact QCS (FP64 and mixed),
-    the envelope fallback, and the gain of the trained network over the
-    closure without it (constant coefficients) and over an untrained one.
+  * in the loop: the learned form against the exact QCS in the same mixed
+    cycle and against FP64 (error and time per cycle), the envelope
+    fallback, and, as a sanity check of the network, the same closure with
+    constant coefficients or an untrained network.
 
 Tolerances: BF16 keeps 8 significant bits, so single activations can round
 differently when the FP32 accumulation order differs; outputs are compared
@@ -251,32 +252,66 @@ def test_envelope_fallback_is_exact_qcs(trained):
         assert np.array_equal(lea[f].view(np.uint64), mix[f].view(np.uint64)), f
 
 
-def test_trained_network_gains_over_no_network(trained):
-    """The same closure without the trained network -- constant coefficients
-    y0 = y1 = 1 (zkur = q1 zss + G du, linear plus quadratic viscosity
-    without the Kurapatenko blend), or an untrained network -- is much
-    further from the exact QCS than the trained one."""
+def test_learned_against_exact_qcs(trained):
+    """What the trained network gives, measured against the method it stands
+    in for -- the exact QCS coefficient (src/QCS.cc:285-296) in the same
+    mixed-precision cycle -- and against plain FP64, on Sedov 96^2 to t = 1.
+
+    The learned coefficient is an approximation of the exact one, so it
+    cannot beat it on accuracy; what it costs is reported here: its density
+    and energy errors against FP64 beside the exact-QCS mixed form's (the
+    density error must stay within 1%, the headline bound), and its time per
+    cycle beside the mixed form's (the network pass adds work; the learned
+    form is the slower of the two on every measured workload).
+
+    Sanity check of the network itself: the same closure with the network
+    replaced by constant coefficients y0 = y1 = 1 (linear plus quadratic
+    viscosity without the Kurapatenko blend) is at least 5x further from
+    FP64, and an untrained network is worse still or inverts a zone."""
     m, _, _ = trained
-    text = wd.sedov_rect(96, cstop=500)
-    _, _, ref, _ = run_form(text, pa.MODE_FUSED)
-    _, _, lea, _ = run_form(text, pa.MODE_LEARNED, m)
+    text = wd.sedov_rect(96)
+    _, s64, ref, _ = run_form(text, pa.MODE_FUSED)
+    _, smx, mix, _ = run_form(text, pa.MODE_MIXED)
+    _, sle, lea, stats = run_form(text, pa.MODE_LEARNED, m)
+    assert s64["done"] == smx["done"] == sle["done"] == 1
+    e_mix, e_lea = rel_l1(mix["zr"], ref["zr"]), rel_l1(lea["zr"], ref["zr"])
+    e_mix_e, e_lea_e = rel_l1(mix["ze"], ref["ze"]), rel_l1(lea["ze"], ref["ze"])
+    # time per cycle of the two mixed forms on the same state (Sedov 360^2, mid-run)
+    big = wd.sedov_rect(360)
+    us = {}
+    for name, mode in (("mixed", pa.MODE_MIXED), ("learned", pa.MODE_LEARNED)):
+        with pa.Problem(big) as pb:
+            if mode == pa.MODE_LEARNED:
+                pb.set_closure(m)
+            pb.run(2000, mode)
+            ms, _ = pb.time_cycles(200, mode, phases=False)
+            us[name] = ms * 1e3 / 200
+    say(f"  Sedov 96^2 to t=1, density L1 vs FP64: exact QCS (mixed) {e_mix:.2e}, learned {e_lea:.2e}; "
+        f"energy L1 {e_mix_e:.2e} / {e_lea_e:.2e}; zones learned/fallback {stats}")
+    say(f"  Sedov 360^2 cycle: exact QCS (mixed) {us['mixed']:.1f} us, learned {us['learned']:.1f} us")
+    assert stats[0] > 0
+    assert e_lea <= 1e-2
+    # sanity check: the network against no network and an untrained one
     lo, hi = m.get_envelope()
     const = Mlp(0)
     p = np.zeros(mo.NPARAM, np.float32)
     p[mo.OFF_B3:mo.NPARAM] = 1.0
     const.set_params(p)
     const.set_envelope(lo, hi)
-    _, _, cst, _ = run_form(text, pa.MODE_LEARNED, const)
+    short = wd.sedov_rect(96, cstop=500)
+    _, _, ref5, _ = run_form(short, pa.MODE_FUSED)
+    _, _, lea5, _ = run_form(short, pa.MODE_LEARNED, m)
+    _, _, cst5, _ = run_form(short, pa.MODE_LEARNED, const)
     untrained = Mlp(5)
     untrained.set_envelope(lo, hi)
     try:
-        _, _, unt, _ = run_form(text, pa.MODE_LEARNED, untrained)
-        e_u = rel_l1(unt["zr"], ref["zr"])
+        _, _, unt5, _ = run_form(short, pa.MODE_LEARNED, untrained)
+        e_u = rel_l1(unt5["zr"], ref5["zr"])
     except pa.PennantError as ex:           # random coefficients can invert zones
         say(f"  untrained network: run stopped ({ex})")
         e_u = float("inf")
-    e_l, e_c = (rel_l1(x["zr"], ref["zr"]) for x in (lea, cst))
-    say(f"  L1 zr vs exact FP64: trained {e_l:.2e}, constant {e_c:.2e}, untrained {e_u:.2e}")
+    e_l, e_c = (rel_l1(x["zr"], ref5["zr"]) for x in (lea5, cst5))
+    say(f"  500 cycles, L1 zr vs FP64: trained {e_l:.2e}, constant coefficients {e_c:.2e}, untrained {e_u:.2e}")
     assert e_l * 5 <= e_c
     assert e_l * 5 <= e_u
     const.close()

@@ -125,6 +125,19 @@ def test_pieces_stop_and_bad_input():
             pp.run(1, pa.MODE_LEARNED)
 
 
+def test_pieces_time_cycles_advances_like_run():
+    """pennant_pieces_time_cycles queues cycles between events: the state it
+    leaves equals pennant_pieces_run's."""
+    text = wd.noh_pie(16, 12, cstop=400)
+    with pa.Pieces(text, 3) as a, pa.Pieces(text, 3) as b:
+        ms = a.time_cycles(20, pa.MODE_MIXED)
+        assert ms > 0
+        assert b.run(20, pa.MODE_MIXED) == 20
+        assert a.scalars()["cycle"] == b.scalars()["cycle"] == 20
+        for f in STATE:
+            assert_bits_equal(f, a.get(f), b.get(f))
+
+
 def test_pieces_sedov_1440_eight_pieces():
     """Full size: Sedov 1440^2 (2.07M zones) as 8 pieces, 5 cycles, bit-identical."""
     text = wd.sedov_1440(cstop=10 ** 6)
