@@ -144,17 +144,17 @@ def bench_ensemble(args, pa, decks):
 def bench_pieces(args, pa, text, wname):
     """The workload as --pieces pieces on the device against the one-piece
     run: both advance --warmup cycles from the start, then --steps cycles
-    are timed (direct launches; the run in pieces launches each piece's
-    kernels and the two exchanges, so it pays launch overheads the
-    one-piece graph replay does not).  The end states are compared bit for
-    bit."""
+    are timed, both replaying 16-cycle HIP graphs (--graph 0: the one-piece
+    run launches directly; the run in pieces always replays its graph, in
+    which each phase's piece kernels are parallel branches).  The end states
+    are compared bit for bit."""
     import numpy as np
     mode = {"fused": pa.MODE_FUSED, "mixed": pa.MODE_MIXED}.get(args.mode)
     if mode is None:
         raise SystemExit("--pieces runs the fused or mixed mode")
     with pa.Problem(text) as one, pa.Pieces(text, args.pieces) as pp:
         lay = [pp.layout(i) for i in range(pp.count)]
-        one.set_tuning("graph", 0)
+        one.set_tuning("graph", 0 if args.graph == 0 else 1)
         one.run(args.warmup, mode)
         pp.run(args.warmup, mode)
         ms1, _ = one.time_cycles(args.steps, mode, phases=False)

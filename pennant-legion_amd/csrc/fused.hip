@@ -2782,16 +2782,42 @@ FusedArgs piece_args(pennant_pieces* pp, int q) {
 
 }  // namespace
 
+// Phase boundaries of the piece cycle: every piece's stream waits for piece
+// 0's stream (fork), and piece 0's stream waits for every piece (join).
+namespace {
+int pieces_fork(pennant_pieces* pp, int phase) {
+    hipStream_t s0 = pp->pc[0]->stream;
+    int rc = (int)hipEventRecord(pp->ev_fork[(size_t)phase], s0);
+    for (size_t q = 1; q < pp->pc.size() && !rc; ++q)
+        rc = (int)hipStreamWaitEvent(pp->pc[q]->stream, pp->ev_fork[(size_t)phase], 0);
+    return rc;
+}
+int pieces_join(pennant_pieces* pp, int phase) {
+    hipStream_t s0 = pp->pc[0]->stream;
+    const size_t n = pp->pc.size();
+    int rc = 0;
+    for (size_t q = 1; q < n && !rc; ++q) {
+        hipEvent_t e = pp->ev_join[(size_t)phase * n + q];
+        rc = (int)hipEventRecord(e, pp->pc[q]->stream);
+        if (!rc) rc = (int)hipStreamWaitEvent(s0, e, 0);
+    }
+    return rc;
+}
+}  // namespace
+
 int pieces_cycle(pennant_pieces* pp, int form) {
     if (form == FORM_LEARNED || pp->pc.empty()) return PENNANT_EINVAL;
     const bool mixed = (form != FORM_FP64);
     const int n = (int)pp->pc.size();
-    hipStream_t s = pp->pc[0]->stream;
+    hipStream_t s0 = pp->pc[0]->stream;
     std::vector<FusedArgs> A((size_t)n);
     for (int q = 0; q < n; ++q) A[(size_t)q] = piece_args(pp, q);
+    int rc = pieces_fork(pp, 0);
+    if (rc) return rc;
     int poff = 0;
     for (int q = 0; q < n; ++q) {
         const pennant_problem* pb = pp->pc[(size_t)q];
+        hipStream_t s = pb->stream;
         for (size_t r = 0; r < pb->zruns.size(); r += 3) {
             const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
             const int sb = pb->hm.zsbeg[zb];
@@ -2810,34 +2836,39 @@ int pieces_cycle(pennant_pieces* pp, int form) {
         }
     }
     const int npred = poff;
+    if ((rc = pieces_join(pp, 0))) return rc;
     const Ctl* ctl = pp->pc[0]->ctl;
     if (pp->ncx > 0) {
         if (mixed)
-            hipLaunchKernelGGL(k_corner_xchg<float>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s, pp->tab,
+            hipLaunchKernelGGL(k_corner_xchg<float>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s0, pp->tab,
                                pp->cx, pp->ncx, ctl);
         else
-            hipLaunchKernelGGL(k_corner_xchg<double>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s, pp->tab,
+            hipLaunchKernelGGL(k_corner_xchg<double>, dim3(nblocks(pp->ncx)), dim3(PN_BLOCK), 0, s0, pp->tab,
                                pp->cx, pp->ncx, ctl);
     }
+    if ((rc = pieces_fork(pp, 1))) return rc;
     for (int q = 0; q < n; ++q) {
         const pennant_problem* pb = pp->pc[(size_t)q];
         const dim3 pgrid(pb->nheavy + nblocks(own_points(pb)));
         if (mixed)
-            hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, s, A[(size_t)q]);
+            hipLaunchKernelGGL(k_point<float>, pgrid, dim3(PN_BLOCK), 0, pb->stream, A[(size_t)q]);
         else
-            hipLaunchKernelGGL(k_point<double>, pgrid, dim3(PN_BLOCK), 0, s, A[(size_t)q]);
+            hipLaunchKernelGGL(k_point<double>, pgrid, dim3(PN_BLOCK), 0, pb->stream, A[(size_t)q]);
     }
+    if ((rc = pieces_join(pp, 1))) return rc;
     if (pp->nhalo > 0) {
         if (mixed)
-            hipLaunchKernelGGL(k_halo<float>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s, pp->tab, pp->halo,
+            hipLaunchKernelGGL(k_halo<float>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s0, pp->tab, pp->halo,
                                pp->nhalo, ctl);
         else
-            hipLaunchKernelGGL(k_halo<double>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s, pp->tab, pp->halo,
+            hipLaunchKernelGGL(k_halo<double>, dim3(nblocks(pp->nhalo)), dim3(PN_BLOCK), 0, s0, pp->tab, pp->halo,
                                pp->nhalo, ctl);
     }
+    if ((rc = pieces_fork(pp, 2))) return rc;
     poff = 0;
     for (int q = 0; q < n; ++q) {
         const pennant_problem* pb = pp->pc[(size_t)q];
+        hipStream_t s = pb->stream;
         for (size_t r = 0; r < pb->zruns.size(); r += 3) {
             const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
             const int sb = pb->hm.zsbeg[zb];
@@ -2855,8 +2886,9 @@ int pieces_cycle(pennant_pieces* pp, int form) {
         }
     }
     const int ncorr = poff;
+    if ((rc = pieces_join(pp, 2))) return rc;
     const Params& q = pp->prm;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s, A[0], npred, ncorr, q.cflv, q.cstop,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(PN_FIN_BLOCK), 0, s0, A[0], npred, ncorr, q.cflv, q.cstop,
                        q.dtmax, q.dtfac, q.tstop, mixed ? 0 : 1);
     if (mixed) pp->zre_stale = true;
     return (int)hipGetLastError();

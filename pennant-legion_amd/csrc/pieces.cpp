@@ -71,6 +71,49 @@ int sync_ctl(pennant_pieces* pp) {
     return rc;
 }
 
+#define PN_PIECES_GRAPH_CYCLES 16
+
+// Captures PN_PIECES_GRAPH_CYCLES cycles of `form` as one HIP graph (the
+// pieces' kernels of a phase on parallel branches, kernel nodes only).
+int build_pieces_graph(pennant_pieces* pp, int form) {
+    if (pp->gexec) (void)hipGraphExecDestroy(pp->gexec);
+    pp->gexec = nullptr;
+    pp->graph_form = -1;
+    hipStream_t s0 = pp->pc[0]->stream;
+    hipGraph_t g = nullptr;
+    int rc = (int)hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+    if (rc) return rc;
+    for (int k = 0; k < PN_PIECES_GRAPH_CYCLES && !rc; ++k) rc = pieces_cycle(pp, form);
+    const int rc2 = (int)hipStreamEndCapture(s0, &g);
+    if (rc || rc2) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rc2;
+    }
+    rc = (int)hipGraphInstantiate(&pp->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (rc) { pp->gexec = nullptr; return rc; }
+    pp->graph_form = form;
+    return 0;
+}
+
+// Queues ncycles cycles of `form`: runs of PN_PIECES_GRAPH_CYCLES replay the
+// captured graph, the rest launch directly.
+int launch_pieces_cycles(pennant_pieces* pp, int ncycles, int form) {
+    int rc = 0;
+    if (form == FORM_FP64 && (rc = pieces_materialize_zre(pp))) return rc;
+    int left = ncycles;
+    if (left >= PN_PIECES_GRAPH_CYCLES) {
+        if (pp->graph_form != form || !pp->gexec) {
+            if ((rc = build_pieces_graph(pp, form))) return rc;
+        }
+        for (; left >= PN_PIECES_GRAPH_CYCLES && !rc; left -= PN_PIECES_GRAPH_CYCLES)
+            rc = (int)hipGraphLaunch(pp->gexec, pp->pc[0]->stream);
+        if (form != FORM_FP64) pp->zre_stale = true;
+    }
+    for (; left > 0 && !rc; --left) rc = pieces_cycle(pp, form);
+    return rc;
+}
+
 int fused_form_of(int mode) {
     switch (mode) {
         case PENNANT_MODE_FUSED: return FORM_FP64;
@@ -282,6 +325,13 @@ pennant_pieces* pennant_pieces_create(const char* deck_text, int numpcs, char* e
         pb->tune.pipeline = 0;
         pp->pc.push_back(pb);
     }
+    // fork and join events of the piece cycle's phases
+    pp->ev_fork.assign(3, nullptr);
+    pp->ev_join.assign((size_t)3 * npc, nullptr);
+    for (hipEvent_t& e : pp->ev_fork)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail("hipEventCreate failed");
+    for (hipEvent_t& e : pp->ev_join)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail("hipEventCreate failed");
     // corner exchange: the slot a ghost's corner is written to, and the
     // owner's slot for it (both as corner_slot places them)
     std::vector<XEnt> cx;
@@ -372,6 +422,12 @@ int pennant_pieces_plan(const char* deck_text, int numpcs, int64_t* out, int64_t
 
 void pennant_pieces_destroy(pennant_pieces* pp) {
     if (!pp) return;
+    if (!pp->pc.empty()) (void)hipStreamSynchronize(pp->pc[0]->stream);
+    if (pp->gexec) (void)hipGraphExecDestroy(pp->gexec);
+    for (hipEvent_t e : pp->ev_fork)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pp->ev_join)
+        if (e) (void)hipEventDestroy(e);
     for (pennant_problem* pb : pp->pc) pennant_destroy(pb);
     for (void* p : pp->allocs) (void)hipFree(p);
     delete pp;
@@ -402,12 +458,10 @@ int pennant_pieces_run(pennant_pieces* pp, int ncycles, int mode) {
     if (rc) return -rc;
     const Ctl& c = *pp->pc[0]->ctl_host;
     const int c0 = c.cycle;
-    if (form == FORM_FP64 && (rc = pieces_materialize_zre(pp))) return -rc;
     int left = ncycles;
     while (left > 0 && !c.done) {
         const int k = left < 64 ? left : 64;
-        for (int i = 0; i < k; ++i)
-            if ((rc = pieces_cycle(pp, form))) return rc > 0 ? -rc : rc;
+        if ((rc = launch_pieces_cycles(pp, k, form))) return rc > 0 ? -rc : rc;
         if ((rc = sync_ctl(pp))) return -rc;
         left -= k;
     }
@@ -421,18 +475,22 @@ int pennant_pieces_run(pennant_pieces* pp, int ncycles, int mode) {
     return c.cycle - c0;
 }
 
-// ncycles cycles between two events on the pieces' stream (no host wait in
-// between); *total_ms their time.
+// ncycles cycles between two events on piece 0's stream (no host wait in
+// between; runs of 16 cycles replay the captured graph); *total_ms their time.
 int pennant_pieces_time_cycles(pennant_pieces* pp, int ncycles, int mode, double* total_ms) {
     const int form = fused_form_of(mode);
     if (form < 0 || pp->pc.empty()) return PENNANT_EINVAL;
     hipStream_t s = pp->pc[0]->stream;
     int rc = 0;
     if (form == FORM_FP64 && (rc = pieces_materialize_zre(pp))) return rc;
+    // the graph is built outside the timed interval
+    if (ncycles >= PN_PIECES_GRAPH_CYCLES && (pp->graph_form != form || !pp->gexec) &&
+        (rc = build_pieces_graph(pp, form)))
+        return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if ((rc = (int)hipEventCreate(&e0)) || (rc = (int)hipEventCreate(&e1))) return rc;
     rc = (int)hipEventRecord(e0, s);
-    for (int i = 0; i < ncycles && !rc; ++i) rc = pieces_cycle(pp, form);
+    if (!rc) rc = launch_pieces_cycles(pp, ncycles, form);
     if (!rc) rc = (int)hipEventRecord(e1, s);
     if (!rc) rc = (int)hipEventSynchronize(e1);
     float ms = 0.f;

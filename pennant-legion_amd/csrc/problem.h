@@ -300,6 +300,11 @@ struct pennant_pieces {
     int* part_i = nullptr;
     int npart = 0;
     bool zre_stale = false;                       // mixed cycles ran since zr, ze were stored
+    // the pieces' kernels of a phase run on their own streams (fork and join
+    // events on piece 0's stream); runs of 16 cycles replay a captured graph
+    std::vector<hipEvent_t> ev_fork, ev_join;     // 3 forks; 3 joins per piece
+    hipGraphExec_t gexec = nullptr;
+    int graph_form = -1;
     std::vector<void*> allocs;
     std::string err;
 };

@@ -125,15 +125,28 @@ def test_pieces_stop_and_bad_input():
             pp.run(1, pa.MODE_LEARNED)
 
 
+@pytest.mark.parametrize("mode", [pa.MODE_FUSED, pa.MODE_MIXED], ids=["fp64", "mixed"])
+def test_pieces_graph_replays_between_eager_cycles(mode):
+    """Runs of 16 cycles replay a captured graph (each phase's piece kernels
+    on parallel branches); eager cycles in between: the state equals the
+    one-piece run's after every run."""
+    text = wd.noh_pie(20, 24, cstop=400)
+    with pa.Problem(text) as one, pa.Pieces(text, 6) as pp:
+        for n in (16, 5, 32, 3, 16):
+            assert one.run(n, mode) == pp.run(n, mode) == n
+            for f in ("px", "pu", "zr", "ze", "zetot"):
+                assert_bits_equal(f"{f} after {n}", pp.get(f), one.get(f))
+
+
 def test_pieces_time_cycles_advances_like_run():
-    """pennant_pieces_time_cycles queues cycles between events: the state it
-    leaves equals pennant_pieces_run's."""
+    """pennant_pieces_time_cycles queues cycles between events (a graph for
+    the first 16): the state it leaves equals pennant_pieces_run's."""
     text = wd.noh_pie(16, 12, cstop=400)
     with pa.Pieces(text, 3) as a, pa.Pieces(text, 3) as b:
-        ms = a.time_cycles(20, pa.MODE_MIXED)
+        ms = a.time_cycles(21, pa.MODE_MIXED)
         assert ms > 0
-        assert b.run(20, pa.MODE_MIXED) == 20
-        assert a.scalars()["cycle"] == b.scalars()["cycle"] == 20
+        assert b.run(21, pa.MODE_MIXED) == 21
+        assert a.scalars()["cycle"] == b.scalars()["cycle"] == 21
         for f in STATE:
             assert_bits_equal(f, a.get(f), b.get(f))
 
