@@ -363,6 +363,7 @@ def main():
         log(f"tuning {tune}")
         sc0 = pb.scalars()
         cl0 = pb.closure_stats() if mode == pa.MODE_LEARNED else None
+        rd0 = pb.lookahead_redos()
         # the timed cycles run as pennant_launch_cycles runs them (16-cycle
         # graphs); the per-kernel durations come from further cycles with
         # events between the kernels
@@ -370,12 +371,14 @@ def main():
         sc1 = pb.scalars()
         assert sc1["cycle"] == sc0["cycle"] + args.steps and not sc1["done"], sc1
         cl1 = pb.closure_stats() if mode == pa.MODE_LEARNED else None
+        rd1 = pb.lookahead_redos()
         phase_ms = None
         if mode != pa.MODE_TASKS and args.phase_steps > 0:
             _, phase_ms = pb.time_cycles(args.phase_steps, mode, phases=True)
         res = {"total_ms": total_ms, "phase_ms": phase_ms, "tune": tune, "e0": e0,
                "t_window": (sc0["time"], sc1["time"]), "cycle_window": (int(sc0["cycle"]), int(sc1["cycle"])),
                "closure_window": None if cl0 is None else (cl1[0] - cl0[0], cl1[1] - cl0[1]),
+               "redos_window": rd1 - rd0,
                "dims": (pb.numz, pb.nums, pb.nump)}
         if args.to_end:
             t0 = time.perf_counter()
@@ -384,7 +387,8 @@ def main():
             sc = pb.scalars()
             res["end"] = {"time": sc["time"], "cycles": int(sc["cycle"]), "e": sum(pb.energy()),
                           "state": {f: pb.get(f) for f in ("zr", "ze", "zp")}, "px": pb.get("px"),
-                          "closure": pb.closure_stats() if mode == pa.MODE_LEARNED else None}
+                          "closure": pb.closure_stats() if mode == pa.MODE_LEARNED else None,
+                          "redos": pb.lookahead_redos()}
             log(f"  ran on to t={sc['time']!r} (cycle {int(sc['cycle'])}) in {time.perf_counter() - t0:.1f}s")
         pb.close()
         return res
@@ -462,7 +466,8 @@ def main():
                     "plain_fp64_tuning": forms["fused"]["tune"],
                     "speedup_vs_plain_fp64": round(value / plain, 4),
                     "timed_window": {k: {"t": [round(v, 6) for v in r["t_window"]],
-                                         "cycles": list(r["cycle_window"])} for k, r in forms.items()}}
+                                         "cycles": list(r["cycle_window"]),
+                                         "lookahead_redos": r["redos_window"]} for k, r in forms.items()}}
         if "mixed" in forms and args.mode != "mixed":
             mixed_v = rate(forms["mixed"])
             log(f"mixed exact QCS: {forms['mixed']['total_ms'] / args.steps * 1e3:.1f} us/cycle, "
@@ -482,7 +487,7 @@ def main():
                     continue
                 e = r["end"]
                 err = field_errors(e["state"], ref["state"])
-                end[k] = {"cycles": e["cycles"],
+                end[k] = {"cycles": e["cycles"], "lookahead_redos": e["redos"],
                           "rel_l1": {f: float(f"{v[0]:.3e}") for f, v in err.items()},
                           "rel_linf": {f: float(f"{v[1]:.3e}") for f, v in err.items()},
                           "shock_radius": round(shock_radius(e["state"]["zr"],
