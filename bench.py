@@ -62,10 +62,21 @@ def workload_deck(name: str, decks):
     raise SystemExit(f"unknown workload {name}")
 
 
-def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False, pipelined: bool = False):
+def kernel_bytes(S: int, Z: int, P: int, mixed: bool = False, pipelined: bool = False,
+                 rect_maps: bool = False):
     """Algorithmic (compulsory) HBM bytes per launch of each fused kernel:
     every distinct input element read once, every output written once
-    (int32 maps, FP64 values; FP32 side and corner arrays in the mixed form)."""
+    (int32 maps, FP64 values; FP32 side and corner arrays in the mixed form).
+    With rect_maps the zone kernels compute the zones' points (mapsp1) and
+    corner slots (pcpos) and read neither."""
+    kb = _kernel_bytes(S, Z, P, mixed, pipelined)
+    if rect_maps:
+        maps = {"zone": 8, "zone_pred": 8, "zone_corr": 4}
+        kb = {k: v - maps.get(k, 0) * S for k, v in kb.items()}
+    return kb
+
+
+def _kernel_bytes(S, Z, P, mixed, pipelined):
     if pipelined:
         return {
             # corrector and next predictor of each zone: mapsp1 4, pcpos 4,
@@ -287,6 +298,8 @@ def main():
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
     ap.add_argument("--max-blocks", type=int, default=None, help="grid cap of the fused zone kernels")
     ap.add_argument("--xcd", type=int, default=None, help="XCD band mapping of the zone kernels (0/1)")
+    ap.add_argument("--rect-maps", type=int, default=None,
+                    help="0: the zone kernels read mapsp1 and the slot map on rect meshes too")
     ap.add_argument("--graph", type=int, default=None,
                     help="0: launch every cycle's kernels directly instead of replaying 16-cycle HIP "
                          "graphs (for tools that trace kernel launches)")
@@ -341,6 +354,8 @@ def main():
             pb.set_tuning("pipeline", args.pipeline)
         if args.graph is not None:
             pb.set_tuning("graph", args.graph)
+        if args.rect_maps is not None:
+            pb.set_tuning("rect_maps", args.rect_maps)
         e0 = sum(pb.energy())
         log(f"setup {time.perf_counter() - t0:.2f}s: zones {pb.numz} sides {pb.nums} points {pb.nump}")
         t0 = time.perf_counter()
@@ -359,7 +374,7 @@ def main():
             tune_cycles = 16
             ms = pb.autotune(mode, cycles=tune_cycles)
             log(f"autotune: {ms / tune_cycles * 1e3:.1f} us/cycle at the best shape")
-        tune = {k: pb.get_tuning(k) for k in ("k1_layout", "max_blocks", "xcd", "pipeline", "graph")}
+        tune = {k: pb.get_tuning(k) for k in ("k1_layout", "max_blocks", "xcd", "pipeline", "graph", "rect_maps")}
         log(f"tuning {tune}")
         sc0 = pb.scalars()
         cl0 = pb.closure_stats() if mode == pa.MODE_LEARNED else None
@@ -401,7 +416,7 @@ def main():
             return None, None
         Z, S, P = r["dims"]
         pipelined = mixed and r["tune"]["pipeline"] == 1
-        kb = kernel_bytes(S, Z, P, mixed, pipelined)
+        kb = kernel_bytes(S, Z, P, mixed, pipelined, r["tune"].get("rect_maps", 0) == 1)
         names = (("zone", "point", "unused", "finalize", "learned_pass") if pipelined
                  else ("zone_pred", "point", "zone_corr", "finalize", "learned_pass"))
         per = {n: r["phase_ms"][i] / args.phase_steps for i, n in enumerate(names)}

@@ -435,6 +435,11 @@ int pennant_bench_copy(size_t nbytes, int reps, double* ms_per_copy);
  *   "pipeline"   1: the mixed modes run the pipelined cycle, 0: the zone
  *                predictor and corrector as separate passes (default);
  *                pennant_autotune tries both for the mixed modes
+ *   "rect_maps"  1: on a rect mesh in the generator's numbering, the zone
+ *                kernels compute each zone's points and corner slots from
+ *                its index instead of reading mapsp1 and the slot map
+ *                (default; pennant_get_tuning reports 1 only when the mesh
+ *                qualifies), 0: they read the maps
  * pennant_set_tuning returns PENNANT_EINVAL for an unknown key or value. */
 int pennant_set_tuning(pennant_problem* pb, const char* key, int value);
 int pennant_get_tuning(pennant_problem* pb, const char* key);

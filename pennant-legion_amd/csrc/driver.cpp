@@ -720,6 +720,11 @@ int pennant_set_tuning(pennant_problem* pb, const char* key, int value) {
         pb->tune.xcd = value;
         return 0;
     }
+    if (!strcmp(key, "rect_maps")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        pb->tune.rect_maps = value;
+        return 0;
+    }
     return PENNANT_EINVAL;
 }
 
@@ -729,6 +734,8 @@ int pennant_get_tuning(pennant_problem* pb, const char* key) {
     if (!strcmp(key, "xcd")) return pb->tune.xcd;
     if (!strcmp(key, "graph")) return pb->tune.graph;
     if (!strcmp(key, "pipeline")) return pb->tune.pipeline;
+    // whether the fused kernels compute the zones' points and corner slots
+    if (!strcmp(key, "rect_maps")) return (pb->tune.rect_maps && pb->rect_nzx > 0) ? 1 : 0;
     return PENNANT_EINVAL;
 }
 

@@ -94,6 +94,11 @@ struct FusedArgs {
     long long cap;
     unsigned long long* cap_count;
     unsigned long long* cl_stats;
+    // structured rect mesh (rect_nzx > 0): points and corner slots of a zone
+    // from its index instead of mapsp1 and pcpos (rect_zone)
+    int rect_nzx;
+    unsigned long long rect_m;  // z / nzx = (z * rect_m) >> rect_sh
+    int rect_sh;
     int* zlist;                // zones for the learned pass, run r at [zbeg_r, zend_r)
     unsigned* zcount;          // per run and list slice (list_count)
     int nzruns;
@@ -139,11 +144,63 @@ __device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_lo
 template <class T>
 __device__ __forceinline__ void st_nt(T v, T* p) { __builtin_nontemporal_store(v, p); }
 
+// A structured rect mesh, as the generator numbers it (the set-up checks
+// every side against mapsp1 and pcpos before it sets a.rect_nzx): zone
+// z = zy nzx + zx has sides 4z .. 4z + 3 from points p0, p0 + 1,
+// p0 + 1 + npx, p0 + npx (npx = nzx + 1, p0 = zy npx + zx = z + zy), and the
+// corner of side k is the rank-th corner of its point in ascending side
+// order -- the number of the point's zones before z, of the four (zx-1,
+// zy-1), (zx, zy-1), (zx-1, zy), (zx, zy) around it that exist.
+struct RectZone {
+    int p0, zx, zy;
+};
+__device__ __forceinline__ RectZone rect_zone(const FusedArgs& a, int z) {
+    RectZone r;
+    r.zy = (int)(((unsigned long long)(unsigned)z * a.rect_m) >> a.rect_sh);
+    r.zx = z - r.zy * a.rect_nzx;
+    r.p0 = z + r.zy;
+    return r;
+}
+__device__ __forceinline__ int rect_point(const FusedArgs& a, const RectZone& r, int k) {
+    const int npx = a.rect_nzx + 1;
+    return k == 0 ? r.p0 : (k == 1 ? r.p0 + 1 : (k == 2 ? r.p0 + 1 + npx : r.p0 + npx));
+}
+__device__ __forceinline__ int rect_rank(const FusedArgs& a, const RectZone& r, int k) {
+    const int left = r.zx >= 1, below = r.zy >= 1;
+    if (k == 0) return (left & below) + below + left;
+    if (k == 1) return below + (below & (r.zx + 1 < a.rect_nzx));
+    return k == 2 ? 0 : left;
+}
+
 // The points of a zone: mapsp1 of its sides sb..sb+NS-1.
 template <int NS>
 __device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
+    if constexpr (NS == 4) {
+        if (a.rect_nzx > 0) {
+            const RectZone r = rect_zone(a, sb >> 2);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) pp[k] = rect_point(a, r, k);
+            return;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < NS; ++k) pp[k] = ld_nt(a.mapsp1 + sb + k);
+}
+
+// The corner-record slots of a zone's corners: pcpos of its sides (the
+// slot is corner_slot(p, rank, k); a rect mesh's ranks are below 4).
+template <int NS>
+__device__ __forceinline__ void load_slots(const FusedArgs& a, int sb, int (&slot)[NS]) {
+    if constexpr (NS == 4) {
+        if (a.rect_nzx > 0) {
+            const RectZone r = rect_zone(a, sb >> 2);
+#pragma unroll
+            for (int k = 0; k < NS; ++k) slot[k] = rect_rank(a, r, k) * a.nump + rect_point(a, r, k);
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) slot[k] = ld_nt(a.pcpos + sb + k);
 }
 
 // The learned pass's zone list of a run is kept in PN_LSLICES slices, one
@@ -283,10 +340,8 @@ __device__ __forceinline__ void load_zone(const FusedArgs& a, int z, int sb, con
     in.zvol = a.zvol[z];
     in.zwr = a.zwrate[z];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        in.mf[k] = a.smf[sb + k];
-        in.slot[k] = ld_nt(a.pcpos + sb + k);
-    }
+    for (int k = 0; k < NS; ++k) in.mf[k] = a.smf[sb + k];
+    load_slots<NS>(a, sb, in.slot);
 }
 
 template <int NS>
@@ -616,10 +671,8 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     const double zvol0 = st.zvol;
     const double zwr = st.zwrate;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        mf[k] = ld_nt(a.fsmf + sb + k);
-        slot[k] = ld_nt(a.pcpos + sb + k);
-    }
+    for (int k = 0; k < NS; ++k) mf[k] = ld_nt(a.fsmf + sb + k);
+    load_slots<NS>(a, sb, slot);
 #pragma unroll
     for (int k = 0; k < NS; ++k)
         xp[k] = mk(__builtin_fma(dth, u0[k].x, x0[k].x), __builtin_fma(dth, u0[k].y, x0[k].y));
@@ -2064,6 +2117,9 @@ FusedArgs make_args(pennant_problem* pb) {
     a.cap = pb->cap;
     a.cap_count = pb->cl_counters;
     a.cl_stats = pb->cl_counters + 1;
+    a.rect_nzx = (pb->tune.rect_maps && pb->rect_nzx > 0) ? pb->rect_nzx : 0;
+    a.rect_m = pb->rect_m;
+    a.rect_sh = pb->rect_sh;
     a.zlist = pb->zlist;
     a.zcount = pb->zcount;
     a.nzruns = (int)pb->zruns.size() / 3;
@@ -2324,6 +2380,9 @@ BatchArgs make_batch_args(pennant_batch* b) {
     a.cap = b->cap;
     a.cap_count = b->cl_counters;
     a.cl_stats = b->cl_counters + 1;
+    a.rect_nzx = 0;
+    a.rect_m = 0;
+    a.rect_sh = 0;
     B.units = b->units; B.nunits = b->nunits;
     B.punits = b->punits; B.npunits = b->npunits;
     B.heavy = b->heavy; B.nheavy = b->nheavy;
@@ -2345,6 +2404,38 @@ int refresh_side_fracs(pennant_problem* pb) {
                            (const double*)pb->f("smf"), const_cast<float*>(pb->fsmf));
     pb->smf_stale = false;
     return (int)hipGetLastError();
+}
+
+// Whether the mesh is a structured rect in the generator's numbering, so
+// that the fused kernels may compute its zones' points and corner slots
+// (rect_zone) instead of loading mapsp1 and pcpos: one run of quads, sides
+// 4z .. 4z + 3, and every side's point and slot equal to the formula's
+// (checked here, with the same integer arithmetic as the device's).
+void rect_detect(pennant_problem* pb, const std::vector<int>& pcpos) {
+    const HostMesh& h = pb->hm;
+    pb->rect_nzx = 0;
+    const long nzx = pb->prm.nzx;
+    if (pb->prm.meshtype != "rect" || own_points(pb) != h.nump || pb->zruns.size() != 3 ||
+        pb->zruns[2] != 4 || nzx < 1 || (long)h.numz % nzx != 0 || (int64_t)h.nums != 4 * (int64_t)h.numz)
+        return;
+    int sh = 0;
+    while ((1L << sh) < nzx) ++sh;
+    const unsigned long long m = ((1ull << (32 + sh)) / (unsigned long long)nzx) + 1ull;
+    const long npx = nzx + 1;
+    for (int z = 0; z < h.numz; ++z) {
+        const long zy = (long)(((unsigned long long)(unsigned)z * m) >> (32 + sh));
+        const long zx = z - zy * nzx;
+        if (zy != z / nzx || h.zsbeg[z] != 4 * z) return;
+        const long p0 = z + zy;
+        const int left = zx >= 1, below = zy >= 1;
+        const long pts[4] = {p0, p0 + 1, p0 + 1 + npx, p0 + npx};
+        const int rank[4] = {(left & below) + below + left, below + (below & (zx + 1 < nzx)), 0, left};
+        for (int k = 0; k < 4; ++k)
+            if (h.mapsp1[4 * z + k] != pts[k] || pcpos[4 * z + k] != (long)rank[k] * h.nump + pts[k]) return;
+    }
+    pb->rect_nzx = (int)nzx;
+    pb->rect_m = m;
+    pb->rect_sh = 32 + sh;
 }
 
 int fused_setup(pennant_problem* pb) {
@@ -2412,6 +2503,7 @@ int fused_setup(pennant_problem* pb) {
     if (pb->pcpos &&
         hipMemcpy(pb->pcpos, pcpos.data(), S * 4, hipMemcpyHostToDevice) != hipSuccess)
         return (int)hipErrorUnknown;
+    rect_detect(pb, pcpos);
     // partial slots: one per block of every K1 (resp. K3) launch of a cycle,
     // at the largest grid any tuning allows
     const int nruns = (int)pb->zruns.size() / 3;
@@ -2780,7 +2872,9 @@ FusedArgs piece_args(pennant_pieces* pp, int q) {
     a.part_dv = pp->part_d + pp->npart;
     a.part_bp = pp->part_i;
     a.part_bc = pp->part_i + pp->npart;
-    // no learned pass or tuple capture in a multi-piece run
+    // no learned pass or tuple capture in a multi-piece run; the pieces'
+    // maps are loaded
+    a.rect_nzx = 0;
     a.zlist = nullptr;
     a.zcount = nullptr;
     a.nzruns = 0;

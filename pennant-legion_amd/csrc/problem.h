@@ -67,6 +67,7 @@ struct Tuning {
     int xcd = 1;             // XCD-aware unit mapping of the grid-stride launches
     int graph = 1;           // batched cycles through a captured HIP graph
     int pipeline = 0;        // mixed forms: next cycle's predictor in the corrector pass
+    int rect_maps = 1;       // structured rect meshes: zone points and corner slots computed, not loaded
 };
 
 struct Params {
@@ -133,6 +134,11 @@ struct pennant_problem {
     // [0] tuples, [1] learned zones, [2] fallback zones; [3], [4] the same
     // counts of a look-ahead predictor, added to [1], [2] when its cycle ends
     unsigned long long* cl_counters = nullptr;
+    // structured rect mesh in the generator's numbering (rect_detect): nzx,
+    // else 0, and the multiplier and shift of z / nzx
+    int rect_nzx = 0;
+    unsigned long long rect_m = 0;
+    int rect_sh = 0;
     int* zlist = nullptr;            // zones for the learned pass (per run, at the run's zone range)
     unsigned* zcount = nullptr;      // per run and list slice, one 128-byte line each
     pn::Tuning tune;

@@ -149,6 +149,39 @@ def test_fused_launch_shapes_do_not_change_results(k1, maxb, xcd):
             assert_bits_equal("dtrec", pb.scalars()["dtrec"], o.scalars()["dtrec"])
 
 
+@pytest.mark.parametrize("mode,pipeline", [(pa.MODE_FUSED, 0), (pa.MODE_MIXED, 0), (pa.MODE_MIXED, 1)],
+                         ids=["fp64", "mixed", "mixed-pipelined"])
+def test_rect_maps_do_not_change_results(mode, pipeline):
+    """On a rect mesh the zone kernels compute each zone's points and corner
+    slots from its index (tuning "rect_maps", on by default) instead of
+    reading mapsp1 and the slot map: the state is the same bit for bit as
+    with the maps read, on rect meshes with one, a few and many zones per
+    row and with fixed boundaries (a pie mesh does not qualify); in FP64 it
+    is the oracle's."""
+    decks = [wd.sedov_rect(37, cstop=40), wd.leblanc_rect(64, 8, cstop=40), ref_deck("sedov"),
+             "\n".join(ln for ln in wd.sedov_rect(1, cstop=40).splitlines() if not ln.startswith("meshparams"))
+             + "\nmeshparams 1 5 1.0 1.0\n"]
+    with pa.Problem(wd.noh_pie(8, 6, cstop=4)) as pie:
+        assert pie.get_tuning("rect_maps") == 0
+    for text in decks:
+        out = []
+        for rm in (1, 0):
+            with pa.Problem(text) as pb:
+                pb.set_tuning("rect_maps", rm)
+                pb.set_tuning("pipeline", pipeline)
+                assert pb.get_tuning("rect_maps") == rm
+                n = pb.run(30, mode)
+                out.append((n, {f: pb.get(f) for f in STATE}, pb.scalars()["dtrec"]))
+        assert out[0][0] == out[1][0] > 0
+        for f in STATE:
+            assert_bits_equal(f"{f} rect_maps", out[0][1][f], out[1][1][f])
+        if mode == pa.MODE_FUSED:
+            o = Oracle(text)
+            o.run(out[0][0])
+            for f in STATE:
+                assert_bits_equal(f"{f} vs oracle", out[0][1][f], o.get(f))
+
+
 def test_sedov_xy_identical_to_oracle(tmp_path):
     text = ref_deck("sedov")
     o = Oracle(text)
