@@ -2162,14 +2162,6 @@ int pred_grid(const pennant_problem* pb, int nz, int ns, int form) {
     const int per = (form == FORM_FP64 && pb->tune.k1_layout == 1) ? (PN_BLOCK / 64) * (64 / ns) : PN_BLOCK;
     return grid_for((nz + per - 1) / per, pb->tune.max_blocks);
 }
-// Blocks of the pipelined cycle's predictor redo for a run: the redo runs
-// only when the look-ahead dt missed, and a launch of fewer blocks that
-// finds nothing to do ends sooner.
-#define PN_REDO_GRID 256
-int redo_grid(const pennant_problem* pb, int nz, int ns, int form) {
-    const int g = pred_grid(pb, nz, ns, form);
-    return g < PN_REDO_GRID ? g : PN_REDO_GRID;
-}
 int corr_grid(const pennant_problem* pb, int nz) {
     return grid_for(nblocks(nz), pb->tune.max_blocks);
 }
@@ -2625,7 +2617,7 @@ int launch_lookahead_pred(const pennant_problem* pb, const FusedArgs& a, int for
     for (size_t r = 0; r < pb->zruns.size(); r += 3) {
         const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
         const int sb = h.zsbeg[zb];
-        const dim3 grid(REDO ? redo_grid(pb, ze - zb, ns, form) : pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
+        const dim3 grid(pred_grid(pb, ze - zb, ns, form)), blk(PN_BLOCK);
         const int run = (int)(r / 3);
         const bool fl = (form == FORM_LEARNED);
 #define PN_LA(NS)                                                                                  \
@@ -2699,10 +2691,10 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     hipStream_t s = pb->stream;
     const HostMesh& h = pb->hm;
     const bool fl = (form == FORM_LEARNED);
-    // partials of the predictor redo
+    // partials of one predictor pass (the zone pass's, or its redo's)
     int nparts = 0;
     for (size_t r = 0; r < pb->zruns.size(); r += 3)
-        nparts += redo_grid(pb, pb->zruns[r + 1] - pb->zruns[r], pb->zruns[r + 2], form);
+        nparts += pred_grid(pb, pb->zruns[r + 1] - pb->zruns[r], pb->zruns[r + 2], form);
     if (ev) (void)hipEventRecord(ev[0], s);
     {
         FusedArgs a2 = a;
@@ -2738,7 +2730,7 @@ int fused_cycle_pipe(pennant_problem* pb, hipEvent_t* ev, int form) {
     // are folded by the next cycle's K2 (every launch sequence that does not
     // continue with this form's cycles starts again from the prologue)
     const int npred = launch_lookahead_pred<true>(pb, a, form, s);
-    if (npred != nparts) return PENNANT_EINVAL;
+    if (npred != poff || npred != nparts) return PENNANT_EINVAL;
     if (ev) (void)hipEventRecord(ev[3], s);
     if (fl) {
         const int rc = launch_learned_pass(pb, a, true, s);
