@@ -33,6 +33,7 @@
 //     one is instantiated for two separate arrays (stride 1), for the
 //     reference's interleaved double2 instance (stride 2, y = x + 1, 16-byte
 //     accesses) and for any other stride.
+#include <climits>
 #include <stdint.h>
 
 #include "common.h"
@@ -381,51 +382,183 @@ __device__ __forceinline__ d2 crnr_force(const pennant_cvec& sfp, const pennant_
     return sub(f, f3);
 }
 
+// One lane per point.  A wave's 64 points are neighbours, so the sides their
+// corners read (side s, and the zone's previous side s3 within maxznump - 1
+// of it) lie in one or two short runs of side indices (on a structured mesh,
+// the zones around a stretch of one point row: one stretch of each of two
+// zone rows).  The wave finds those runs from the corners' sides (wave
+// minimum and maximum, split at the middle), loads every side of them once
+// with coalesced loads into LDS as sfp + sfq + sft, and each lane then sums
+// its corners from LDS.  A wave whose corners spread wider (a stretch that
+// wraps to the next row, small meshes) reads its corners directly; the sums
+// are the same either way, in ascending order from the point's value.
+#define PN_SCF_CAP 544      // staged sides per wave (two runs of 65 quads and their reach)
+__device__ __forceinline__ int wave_imin(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_imax(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Sides sd and sd + 1 of a vector field (sd + 1 clamped to the last side);
+// two arrays are read with one 16-byte access each.
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 template <int L>
-__global__ void __launch_bounds__(PN_BLOCK) k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
+__device__ __forceinline__ void ld_pair(const pennant_cvec& v, int sd, int nums, d2& a, d2& b) {
+    if constexpr (L == VL_SOA) {
+        if (sd + 1 < nums) {
+            const f64x2u x = *reinterpret_cast<const f64x2u*>(v.x + sd);
+            const f64x2u y = *reinterpret_cast<const f64x2u*>(v.y + sd);
+            a = mk(x[0], y[0]);
+            b = mk(x[1], y[1]);
+            return;
+        }
+    }
+    a = VIO<L>::ld(v, sd);
+    b = VIO<L>::ld(v, min(sd + 1, nums - 1));
+}
+
+template <int L>
+__global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
                                                              pennant_cvec sfq, pennant_cvec sft,
                                                              pennant_vec pf) {
     __shared__ Heavy hv;
     __shared__ d2 part[PN_BLOCK];
+    __shared__ double fx[PN_BLOCK / 64][PN_SCF_CAP], fy[PN_BLOCK / 64][PN_SCF_CAP];
     if (threadIdx.x == 0) hv.n = 0;
     __syncthreads();
     const int p = xcd_block() * PN_BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int kb = 0, ke = 0;
     if (p < m.nump) {
-        const int kb = m.pcbeg[p], ke = m.pcbeg[p + 1];
-        if (ke - kb > PN_PLIGHT) {
-            hv.p[atomicAdd(&hv.n, 1)] = p;
-        } else if (ke > kb) {
-            int s[PN_PLIGHT], s3[PN_PLIGHT];
+        kb = m.pcbeg[p];
+        ke = m.pcbeg[p + 1];
+    }
+    if (ke - kb > PN_PLIGHT) hv.p[atomicAdd(&hv.n, 1)] = p;
+    const bool light = ke > kb && ke - kb <= PN_PLIGHT;
+    const d2 pf0 = light ? VIO<L>::ld(pf, p) : mk(0., 0.);
+    int s[PN_PLIGHT];
 #pragma unroll
-            for (int j = 0; j < PN_PLIGHT; ++j) s[j] = m.pcside[min(kb + j, ke - 1)];
+    for (int j = 0; j < PN_PLIGHT; ++j) s[j] = light ? m.pcside[min(kb + j, ke - 1)] : 0;
+    // the runs of sides to stage, [a0, a0 + n0) and [a1, a1 + n1): every
+    // corner's s widened by the zone's reach (s3 lies within it), rounded out
+    // to whole side pairs (a pair's second side may lie past the mesh's last)
+    const int reach = m.maxznump - 1;
+    int lo = INT_MAX, hi = -1;
+    if (light) {
 #pragma unroll
-            for (int j = 0; j < PN_PLIGHT; ++j) s3[j] = m.mapss3[s[j]];
-            d2 f[PN_PLIGHT];
+        for (int j = 0; j < PN_PLIGHT; ++j) {
+            lo = min(lo, s[j]);
+            hi = max(hi, s[j]);
+        }
+    }
+    const int smin = max(wave_imin(lo) - reach, 0) & ~1;
+    const int smax = min(wave_imax(hi) + reach, m.nums - 1) | 1;
+    int a0 = smin, n0 = 0, a1 = 0, n1 = 0;
+    if (smax >= smin) {
+        if (smax - smin < PN_SCF_CAP) {
+            n0 = smax - smin + 1;
+        } else {
+            const int mid = smin + (smax - smin) / 2;
+            int h0 = -1, l1 = INT_MAX;
+            bool across = false;
+            if (light) {
 #pragma unroll
-            for (int j = 0; j < PN_PLIGHT; ++j) f[j] = crnr_force<L>(sfp, sfq, sft, s[j], s3[j]);
-            d2 acc = VIO<L>::ld(pf, p);
+                for (int j = 0; j < PN_PLIGHT; ++j) {
+                    if (s[j] + reach <= mid) h0 = max(h0, s[j] + reach);
+                    else if (s[j] - reach > mid) l1 = min(l1, s[j] - reach);
+                    else across = true;
+                }
+            }
+            h0 = wave_imax(h0) | 1;
+            l1 = wave_imin(l1) & ~1;
+            if (!__any(across) && h0 >= smin && l1 <= smax && h0 < l1 &&
+                (h0 - smin + 1) + (smax - l1 + 1) <= PN_SCF_CAP) {
+                n0 = h0 - smin + 1;
+                a1 = l1;
+                n1 = smax - l1 + 1;
+            }
+        }
+    }
+    const int n = n0 + n1;
+    if (n > 0) {
+        // the runs' loads in two batches (each batch's loads first, then its
+        // sums into LDS); lane q takes side pair q (sides 2q, 2q + 1 of the runs)
+        constexpr int NT = (PN_SCF_CAP + 127) / 128, NB = (NT + 1) / 2;
+#pragma unroll
+        for (int t0 = 0; t0 < NT; t0 += NB) {
+            d2 fp[NB][2], fq[NB][2], ft[NB][2];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int i = min(2 * (lane + 64 * (t0 + u)), n - 2);
+                const int sd = (i < n0) ? a0 + i : a1 + (i - n0);
+                ld_pair<L>(sfp, sd, m.nums, fp[u][0], fp[u][1]);
+                ld_pair<L>(sfq, sd, m.nums, fq[u][0], fq[u][1]);
+                ld_pair<L>(sft, sd, m.nums, ft[u][0], ft[u][1]);
+            }
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int i = 2 * (lane + 64 * (t0 + u));
+                if (t0 + u < NT && i < n) {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const d2 f = add(add(fp[u][e], fq[u][e]), ft[u][e]);
+                        fx[wv][i + e] = f.x;
+                        fy[wv][i + e] = f.y;
+                    }
+                }
+            }
+        }
+    }
+    int s3[PN_PLIGHT];
+#pragma unroll
+    for (int j = 0; j < PN_PLIGHT; ++j) s3[j] = light ? m.mapss3[s[j]] : 0;
+    if (n > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (light) {
+            d2 acc = pf0;
 #pragma unroll
             for (int j = 0; j < PN_PLIGHT; ++j)
-                if (kb + j < ke) acc = add(acc, f[j]);
+                if (kb + j < ke) {
+                    const int i = (s[j] < a0 + n0) ? s[j] - a0 : n0 + s[j] - a1;
+                    const int i3 = (s3[j] < a0 + n0) ? s3[j] - a0 : n0 + s3[j] - a1;
+                    acc = add(acc, sub(mk(fx[wv][i], fy[wv][i]), mk(fx[wv][i3], fy[wv][i3])));
+                }
             VIO<L>::st(pf, p, acc);
         }
+    } else if (light) {
+        d2 f[PN_PLIGHT];
+#pragma unroll
+        for (int j = 0; j < PN_PLIGHT; ++j) f[j] = crnr_force<L>(sfp, sfq, sft, s[j], s3[j]);
+        d2 acc = pf0;
+#pragma unroll
+        for (int j = 0; j < PN_PLIGHT; ++j)
+            if (kb + j < ke) acc = add(acc, f[j]);
+        VIO<L>::st(pf, p, acc);
     }
     __syncthreads();
     const int nh = hv.n;
     for (int h = 0; h < nh; ++h) {
         const int q = hv.p[h];
-        const int kb = m.pcbeg[q], ke = m.pcbeg[q + 1];
+        const int qb = m.pcbeg[q], qe = m.pcbeg[q + 1];
         d2 acc = mk(0., 0.);
         if (threadIdx.x == 0) acc = VIO<L>::ld(pf, q);
-        for (int k0 = kb; k0 < ke; k0 += PN_BLOCK) {
+        for (int k0 = qb; k0 < qe; k0 += PN_BLOCK) {
             const int k = k0 + threadIdx.x;
-            if (k < ke) {
-                const int s = m.pcside[k];
-                part[threadIdx.x] = crnr_force<L>(sfp, sfq, sft, s, m.mapss3[s]);
+            if (k < qe) {
+                const int sd = m.pcside[k];
+                part[threadIdx.x] = crnr_force<L>(sfp, sfq, sft, sd, m.mapss3[sd]);
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                const int cnt = min(PN_BLOCK, ke - k0);
+                const int cnt = min(PN_BLOCK, qe - k0);
                 for (int i = 0; i < cnt; ++i) acc = add(acc, part[i]);
             }
             __syncthreads();
