@@ -528,8 +528,12 @@ k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
             for (int j = 0; j < PN_PLIGHT; ++j)
                 if (kb + j < ke) {
                     const int i = (s[j] < a0 + n0) ? s[j] - a0 : n0 + s[j] - a1;
-                    const int i3 = (s3[j] < a0 + n0) ? s3[j] - a0 : n0 + s3[j] - a1;
-                    acc = add(acc, sub(mk(fx[wv][i], fy[wv][i]), mk(fx[wv][i3], fy[wv][i3])));
+                    const int i3 = (s3[j] >= a0 && s3[j] < a0 + n0) ? s3[j] - a0
+                                 : ((s3[j] >= a1 && s3[j] < a1 + n1) ? n0 + s3[j] - a1 : -1);
+                    // (s3 outside the runs only if the mesh's maxznump understates
+                    // a zone: read that corner directly)
+                    acc = add(acc, (i3 >= 0) ? sub(mk(fx[wv][i], fy[wv][i]), mk(fx[wv][i3], fy[wv][i3]))
+                                             : crnr_force<L>(sfp, sfq, sft, s[j], s3[j]));
                 }
             VIO<L>::st(pf, p, acc);
         }
