@@ -309,6 +309,98 @@ struct Heavy {
     int p[PN_BLOCK];
 };
 
+// Point gathers staged per wave.  A wave's 64 points are neighbours, so the
+// sides their corners read (side s, and the zone's previous side s3 within
+// maxznump - 1 of it) lie in one or two short runs of side indices (on a
+// structured mesh, the zones around a stretch of one point row: one stretch
+// of each of two zone rows).  wave_runs finds them from the corners' sides
+// (wave minimum and maximum, widened by a zone's reach and split at the
+// middle, rounded out to whole side pairs); the corner-force gather
+// (k_sum_crnr_force) loads every side of the runs once with coalesced pair
+// loads into LDS and sums each point's corners from LDS, in the same order and
+// arithmetic as the direct gather, which a wave whose corners spread wider (a
+// stretch that wraps to the next row, small meshes) still uses.  (Staging the
+// corner-mass gather's smf and zone products the same way made it slower:
+// the zone values are a second dependent gather.)
+#define PN_SCF_CAP 544      // staged sides per wave (two runs of 65 quads and their reach)
+__device__ __forceinline__ int wave_imin(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_imax(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// The staged runs [a0, a0 + n0) and [a1, a1 + n1) (n0 = 0: none, read
+// directly); even starts and lengths (a pair's second side may lie past the
+// mesh's last).  Called by the whole wave.
+struct Runs {
+    int a0, n0, a1, n1;
+};
+__device__ __forceinline__ Runs wave_runs(const int (&s)[PN_PLIGHT], bool light, int reach, int nums) {
+    int lo = INT_MAX, hi = -1;
+    if (light) {
+#pragma unroll
+        for (int j = 0; j < PN_PLIGHT; ++j) {
+            lo = min(lo, s[j]);
+            hi = max(hi, s[j]);
+        }
+    }
+    const int smin = max(wave_imin(lo) - reach, 0) & ~1;
+    const int smax = min(wave_imax(hi) + reach, nums - 1) | 1;
+    Runs r = {smin, 0, 0, 0};
+    if (smax < smin) return r;
+    if (smax - smin < PN_SCF_CAP) {
+        r.n0 = smax - smin + 1;
+        return r;
+    }
+    const int mid = smin + (smax - smin) / 2;
+    int h0 = -1, l1 = INT_MAX;
+    bool across = false;
+    if (light) {
+#pragma unroll
+        for (int j = 0; j < PN_PLIGHT; ++j) {
+            if (s[j] + reach <= mid) h0 = max(h0, s[j] + reach);
+            else if (s[j] - reach > mid) l1 = min(l1, s[j] - reach);
+            else across = true;
+        }
+    }
+    h0 = wave_imax(h0) | 1;
+    l1 = wave_imin(l1) & ~1;
+    if (!__any(across) && h0 >= smin && l1 <= smax && h0 < l1 &&
+        (h0 - smin + 1) + (smax - l1 + 1) <= PN_SCF_CAP) {
+        r.n0 = h0 - smin + 1;
+        r.a1 = l1;
+        r.n1 = smax - l1 + 1;
+    }
+    return r;
+}
+// Slot of side x in the staged runs, or -1.
+__device__ __forceinline__ int run_slot(const Runs& r, int x) {
+    return (x >= r.a0 && x < r.a0 + r.n0) ? x - r.a0 : ((x >= r.a1 && x < r.a1 + r.n1) ? r.n0 + x - r.a1 : -1);
+}
+
+// Sides sd and sd + 1 of a vector field (sd + 1 clamped to the last side);
+// two arrays are read with one 16-byte access each.
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+template <int L>
+__device__ __forceinline__ void ld_pair(const pennant_cvec& v, int sd, int nums, d2& a, d2& b) {
+    if constexpr (L == VL_SOA) {
+        if (sd + 1 < nums) {
+            const f64x2u x = *reinterpret_cast<const f64x2u*>(v.x + sd);
+            const f64x2u y = *reinterpret_cast<const f64x2u*>(v.y + sd);
+            a = mk(x[0], y[0]);
+            b = mk(x[1], y[1]);
+            return;
+        }
+    }
+    a = VIO<L>::ld(v, sd);
+    b = VIO<L>::ld(v, min(sd + 1, nums - 1));
+}
+
 // calcCrnrMassTask, src/Hydro.cc:1083-1117, as a gather over the point's
 // corners in ascending side order: pmaswt[p] += zr*zarea*0.5*(smf[s]+smf[s3])
 __device__ __forceinline__ double crnr_mass(const double* smf, const double* zr,
@@ -382,46 +474,8 @@ __device__ __forceinline__ d2 crnr_force(const pennant_cvec& sfp, const pennant_
     return sub(f, f3);
 }
 
-// One lane per point.  A wave's 64 points are neighbours, so the sides their
-// corners read (side s, and the zone's previous side s3 within maxznump - 1
-// of it) lie in one or two short runs of side indices (on a structured mesh,
-// the zones around a stretch of one point row: one stretch of each of two
-// zone rows).  The wave finds those runs from the corners' sides (wave
-// minimum and maximum, split at the middle), loads every side of them once
-// with coalesced loads into LDS as sfp + sfq + sft, and each lane then sums
-// its corners from LDS.  A wave whose corners spread wider (a stretch that
-// wraps to the next row, small meshes) reads its corners directly; the sums
-// are the same either way, in ascending order from the point's value.
-#define PN_SCF_CAP 544      // staged sides per wave (two runs of 65 quads and their reach)
-__device__ __forceinline__ int wave_imin(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_imax(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// Sides sd and sd + 1 of a vector field (sd + 1 clamped to the last side);
-// two arrays are read with one 16-byte access each.
-typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
-template <int L>
-__device__ __forceinline__ void ld_pair(const pennant_cvec& v, int sd, int nums, d2& a, d2& b) {
-    if constexpr (L == VL_SOA) {
-        if (sd + 1 < nums) {
-            const f64x2u x = *reinterpret_cast<const f64x2u*>(v.x + sd);
-            const f64x2u y = *reinterpret_cast<const f64x2u*>(v.y + sd);
-            a = mk(x[0], y[0]);
-            b = mk(x[1], y[1]);
-            return;
-        }
-    }
-    a = VIO<L>::ld(v, sd);
-    b = VIO<L>::ld(v, min(sd + 1, nums - 1));
-}
-
+// One lane per point, the corners' forces sfp + sfq + sft at s minus the same
+// at s3 staged per wave (wave_runs) or read directly.
 template <int L>
 __global__ void __launch_bounds__(PN_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
@@ -445,46 +499,8 @@ k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
     int s[PN_PLIGHT];
 #pragma unroll
     for (int j = 0; j < PN_PLIGHT; ++j) s[j] = light ? m.pcside[min(kb + j, ke - 1)] : 0;
-    // the runs of sides to stage, [a0, a0 + n0) and [a1, a1 + n1): every
-    // corner's s widened by the zone's reach (s3 lies within it), rounded out
-    // to whole side pairs (a pair's second side may lie past the mesh's last)
-    const int reach = m.maxznump - 1;
-    int lo = INT_MAX, hi = -1;
-    if (light) {
-#pragma unroll
-        for (int j = 0; j < PN_PLIGHT; ++j) {
-            lo = min(lo, s[j]);
-            hi = max(hi, s[j]);
-        }
-    }
-    const int smin = max(wave_imin(lo) - reach, 0) & ~1;
-    const int smax = min(wave_imax(hi) + reach, m.nums - 1) | 1;
-    int a0 = smin, n0 = 0, a1 = 0, n1 = 0;
-    if (smax >= smin) {
-        if (smax - smin < PN_SCF_CAP) {
-            n0 = smax - smin + 1;
-        } else {
-            const int mid = smin + (smax - smin) / 2;
-            int h0 = -1, l1 = INT_MAX;
-            bool across = false;
-            if (light) {
-#pragma unroll
-                for (int j = 0; j < PN_PLIGHT; ++j) {
-                    if (s[j] + reach <= mid) h0 = max(h0, s[j] + reach);
-                    else if (s[j] - reach > mid) l1 = min(l1, s[j] - reach);
-                    else across = true;
-                }
-            }
-            h0 = wave_imax(h0) | 1;
-            l1 = wave_imin(l1) & ~1;
-            if (!__any(across) && h0 >= smin && l1 <= smax && h0 < l1 &&
-                (h0 - smin + 1) + (smax - l1 + 1) <= PN_SCF_CAP) {
-                n0 = h0 - smin + 1;
-                a1 = l1;
-                n1 = smax - l1 + 1;
-            }
-        }
-    }
+    const Runs rn = wave_runs(s, light, m.maxznump - 1, m.nums);
+    const int a0 = rn.a0, n0 = rn.n0, a1 = rn.a1, n1 = rn.n1;
     const int n = n0 + n1;
     if (n > 0) {
         // the runs' loads in two batches (each batch's loads first, then its
@@ -527,13 +543,11 @@ k_sum_crnr_force(pennant_mesh m, pennant_cvec sfp,
 #pragma unroll
             for (int j = 0; j < PN_PLIGHT; ++j)
                 if (kb + j < ke) {
-                    const int i = (s[j] < a0 + n0) ? s[j] - a0 : n0 + s[j] - a1;
-                    const int i3 = (s3[j] >= a0 && s3[j] < a0 + n0) ? s3[j] - a0
-                                 : ((s3[j] >= a1 && s3[j] < a1 + n1) ? n0 + s3[j] - a1 : -1);
+                    const int i = run_slot(rn, s[j]), i3 = run_slot(rn, s3[j]);
                     // (s3 outside the runs only if the mesh's maxznump understates
                     // a zone: read that corner directly)
-                    acc = add(acc, (i3 >= 0) ? sub(mk(fx[wv][i], fy[wv][i]), mk(fx[wv][i3], fy[wv][i3]))
-                                             : crnr_force<L>(sfp, sfq, sft, s[j], s3[j]));
+                    acc = add(acc, (i >= 0 && i3 >= 0) ? sub(mk(fx[wv][i], fy[wv][i]), mk(fx[wv][i3], fy[wv][i3]))
+                                                       : crnr_force<L>(sfp, sfq, sft, s[j], s3[j]));
                 }
             VIO<L>::st(pf, p, acc);
         }
