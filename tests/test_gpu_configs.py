@@ -1,5 +1,7 @@
 """The measured workloads, each run on the GPU at its full size.
 
+  * Sedov on the 90 x 90 rect mesh to t = 1 (the reference's CPU-runnable
+    case): task mode, fused FP64 and fused in pieces against the oracle;
   * Noh implosion on the 1024 x 1024 pie mesh (1,048,576 zones, triangles at
     the centre, a 1024-corner origin point): plain FP64 fused cycles against
     the CPU oracle, bit for bit;
@@ -52,6 +54,33 @@ def assert_bits_equal(name, got, want):
 def with_tstop(text: str, tstop: float) -> str:
     lines = [ln for ln in text.splitlines() if ln.split() and ln.split()[0] not in ("tstop", "cstop")]
     return "\n".join(lines + [f"tstop {tstop!r}"]) + "\n"
+
+
+def test_sedov_90_to_tstop_against_oracle():
+    """Sedov on the 90 x 90 rect mesh (8100 zones) to t = 1, the whole run
+    the reference's CPU path is quoted on (622 cycles): the task mode, the
+    plain FP64 fused form and the fused form in 4 pieces end bit-identical to
+    the oracle, cycle count, time and dt included."""
+    text = wd.sedov_90(cstop=10 ** 7)
+    o = Oracle(text)
+    n = o.run(10 ** 7)
+    so = o.scalars()
+    assert so["time"] == 1.0 and n > 600
+    runs = []
+    for mode in (pa.MODE_TASKS, pa.MODE_FUSED):
+        with pa.Problem(text) as pb:
+            got = pb.run(10 ** 7, mode)
+            runs.append((f"mode {mode}", got, pb.scalars(), {f: pb.get(f) for f in STATE}))
+    with pa.Pieces(text, 4) as pp:
+        got = pp.run(10 ** 7, pa.MODE_FUSED)
+        runs.append(("4 pieces", got, pp.scalars(), {f: pp.get(f) for f in STATE}))
+    for name, got, sc, st in runs:
+        say(f"[sedov 90 to t=1] {name}: {got} cycles, t = {sc['time']!r}")
+        assert got == n, (name, got, n)
+        for k in ("time", "dt", "dtrec"):
+            assert bits(np.array([sc[k]]))[0] == bits(np.array([so[k]]))[0], (name, k)
+        for f in STATE:
+            assert_bits_equal(f"{name} {f}", st[f], o.get(f))
 
 
 def test_noh_pie_1024_fused_against_oracle():
