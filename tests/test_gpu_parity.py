@@ -10,8 +10,10 @@ tests compare raw bits, not values within a tolerance.
     field (49 arrays) plus the reduction scalars must match after every step;
   * cycle parity: whole cycles in task mode and in fused mode against the
     oracle (rect with and without far BCs, pie with triangles, a pie origin
-    past the per-thread corner sum, two-state LeBlanc, and decks setting the
-    parameters no reference deck sets);
+    past the per-thread corner sum, two-state LeBlanc, decks setting the
+    parameters no reference deck sets, and the first cycles of every
+    reference deck but the hex one: sedovsmall, sedovbig, leblancbig and
+    nohsquare's radial inflow on a rect mesh);
   * full size: Sedov 1440^2 (2.07M zones) -- a few cycles against the oracle,
     and fused mode against task mode;
   * the C ABI called directly on caller-owned device buffers.
@@ -103,6 +105,13 @@ CYCLE_DECKS = {
                                              **dict(PARAMS, q1=0.5, q2=1.2, qgamma=1.2)), 25),
     "leblanc10x90": (lambda: ref_deck("leblanc"), 10),
     "leblanc256x16": (lambda: wd.leblanc_rect(256, 16, cstop=30), 30),
+    # the reference's other decks (hex nohpoly aside): the first 10 cycles
+    "sedovsmall9x9": (lambda: ref_deck("sedovsmall"), 10),
+    "sedovbig540": (lambda: override(ref_deck("sedovbig"), cstop=10), 10),
+    "leblancbig160x1440": (lambda: override(ref_deck("leblancbig"), cstop=10), 10),
+    # rect mesh with a radial inflow (uinitradial) and alfa 0.1; meshparams 360
+    # (one value: nzy = nzx, unit lengths)
+    "nohsquare360": (lambda: override(ref_deck("nohsquare"), cstop=10), 10),
 }
 
 
