@@ -650,37 +650,54 @@ struct ZoneState {
     double zm, zr, ze, zvol, zwrate;
 };
 
-// The mixed-precision predictor half of zone z (sides sb..sb+NS-1), from its
-// points' positions x0 and velocities u0 and its state st.  With CL !=
-// CL_EXACT it must be called by all 64 lanes of a wave together (the network
-// runs across the wave); `valid` false marks a lane that only takes part in
-// the wave-wide steps and stores nothing.  Writes zp to a.zp.
-template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z, int sb, SideW sw,
-                                                       const d2 (&x0)[NS], const d2 (&u0)[NS],
-                                                       const ZoneState& st, double dt, int& bad,
-                                                       bool valid, ClCount& cnt) {
+// The FP64 half of the mixed predictor of a zone: half-step positions,
+// centre, areas and volumes, edge lengths and characteristic length, the
+// zone EOS and the pressure and TTS side forces, with positions and
+// velocities moved into the zone's FP32 frame for the QCS half.
+template <int NS>
+struct MixedFrame {
+    float mf[NS];     // FP32 side mass fractions
+    int slot[NS];     // corner-record slots
+    f2 ul[NS];        // point velocities relative to the zone-mean velocity zuc
+    f2 xl[NS];        // half-step positions relative to the zone centre
+    f2 exl[NS];       // edge midpoints, same frame
+    float el[NS];     // edge lengths
+    f2 sfp[NS];       // pressure side forces
+    f2 sft[NS];       // TTS side forces
+    d2 zuc;
+    double zss, zrp, zarea, r;
+    float zdl;
+};
+
+// The frame of zone z (sides sb..sb+NS-1) from its points' positions x0 and
+// velocities u0 and its state st; writes zp to a.zp when store_zp.
+template <int NS>
+__device__ __forceinline__ void mixed_frame(MixedFrame<NS>& F, const FusedArgs& a, int z, int sb,
+                                            const d2 (&x0)[NS], const d2 (&u0)[NS],
+                                            const ZoneState& st, double dt, int& bad, bool valid,
+                                            bool store_zp) {
     const double dth = 0.5 * dt;
     const double n = (double)NS;
     d2 xp[NS];
-    float mf[NS];
-    int slot[NS];
+    float (&mf)[NS] = F.mf;
     const double zm = st.zm;
     const double r = st.zr;
     const double ze0 = st.ze;
     const double zvol0 = st.zvol;
     const double zwr = st.zwrate;
+    F.r = r;
 #pragma unroll
     for (int k = 0; k < NS; ++k) mf[k] = ld_nt(a.fsmf + sb + k);
-    load_slots<NS>(a, sb, slot);
+    load_slots<NS>(a, sb, F.slot);
 #pragma unroll
     for (int k = 0; k < NS; ++k)
         xp[k] = mk(__builtin_fma(dth, u0[k].x, x0[k].x), __builtin_fma(dth, u0[k].y, x0[k].y));
     // zone-mean velocity; velocities from here on live in FP32 relative to it
-    d2 zuc = mk(0., 0.);
+    d2& zuc = F.zuc;
+    zuc = mk(0., 0.);
 #pragma unroll
     for (int k = 0; k < NS; ++k) zuc = add(zuc, dvs(u0[k], n));
-    f2 ul[NS];
+    f2 (&ul)[NS] = F.ul;
 #pragma unroll
     for (int k = 0; k < NS; ++k) ul[k] = tof(sub(u0[k], zuc));
     d2 zx = mk(0., 0.);
@@ -688,7 +705,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     for (int k = 0; k < NS; ++k) zx = add(zx, dvs(xp[k], n));
     const double third = 1. / 3.;
     double sa[NS];
-    float el[NS];
+    float (&el)[NS] = F.el;
     double zarea = 0., zvolp = 0.;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -701,9 +718,11 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         if (valid && sv <= 0.) bad += 1;
         el[k] = flen(tof(e));
     }
+    F.zarea = zarea;
     // positions from here on live in FP32 relative to the zone centre (which
     // is the origin of this frame); edge midpoints follow from them
-    f2 xl[NS], exl[NS];
+    f2 (&xl)[NS] = F.xl;
+    f2 (&exl)[NS] = F.exl;
 #pragma unroll
     for (int k = 0; k < NS; ++k) xl[k] = tof(sub(xp[k], zx));
 #pragma unroll
@@ -712,8 +731,10 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     float zdl = 1.e30f;
 #pragma unroll
     for (int k = 0; k < NS; ++k) zdl = fminr(zdl, fac * (float)sa[k] * frcp(el[k]));
+    F.zdl = zdl;
     // zone EOS in FP64
     const double zrp = zm * drcp(zvolp);
+    F.zrp = zrp;
     const double gm1 = a.gamma - 1.;
     const double ssmin2 = smax(a.ssmin * a.ssmin, 1.e-99);
     const double e = smax(ze0, 0.);
@@ -722,27 +743,120 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     // pre + per pz / r^2 = (gamma - 1) e + (gamma - 1)^2 e = gamma (gamma - 1) e
     const double csqd = smax(ssmin2, a.gamma * gm1 * e);
     const double zss = __builtin_sqrt(csqd);
+    F.zss = zss;
     const double minv = drcp(zm);
     const double dv = (zvolp - zvol0) * minv;
     const double bulk = r * csqd;
     const double denom = 1. + 0.5 * per * dv;
     const double src = zwr * dth * minv;
     const double zp = pz + (per * src - r * bulk * dv) * drcp(denom);
-    if (valid && CL != CL_RECOMP) a.zp[z] = zp;
+    if (valid && store_zp) a.zp[z] = zp;
     // pressure and TTS side forces
     const float zpf = (float)zp;
     double sstmp = smax(zss, a.ssmin);
     sstmp = a.alfa * sstmp * sstmp;
-    f2 sfp[NS], sft[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const f2 ss = fmk(-exl[k].y, exl[k].x);                       // rotCCW(ex - zx)
-        sfp[k] = fscl(-zpf, ss);
+        F.sfp[k] = fscl(-zpf, ss);
         // srho - r = zrp (mf zarea - sa) / sa: the difference in FP64, the
         // division in FP32 (relative to the difference itself)
         const float drho = (float)(zrp * ((double)mf[k] * zarea - sa[k])) * frcp((float)sa[k]);
-        sft[k] = fscl(-(float)sstmp * drho, ss);
+        F.sft[k] = fscl(-(float)sstmp * drho, ss);
     }
+}
+
+// Corner k's QCS quantities in the frame F (setCornerDiv,
+// src/QCS.cc:116-242; corner k: xp0 = x[k], xp1 = ex[k], xp2 = zx,
+// xp3 = ex[k-1]; up2 = zuc is the frame's zero velocity).  evol and du are
+// the values before the compression test.
+struct MixedCrn {
+    float area, ccos, div, evol, du, minelen;
+    f2 duav;
+};
+template <int NS>
+__device__ __forceinline__ MixedCrn mixed_corner(const MixedFrame<NS>& F, int k, f2 zucf) {
+    const f2 (&ul)[NS] = F.ul;
+    const f2 (&xl)[NS] = F.xl;
+    const f2 (&exl)[NS] = F.exl;
+    const float (&el)[NS] = F.el;
+    const int km = prv<NS>(k), kp = nxt<NS>(k);
+    MixedCrn c;
+    const f2 u1 = fscl(0.5f, fadd(ul[k], ul[kp]));                 // up1 - zuc
+    const f2 u3 = fscl(0.5f, fadd(ul[km], ul[k]));                 // up3 - zuc
+    const f2 A = fmk(-xl[k].x, -xl[k].y);                           // xp2 - xp0
+    const f2 B = fmk(exl[km].x - exl[k].x, exl[km].y - exl[k].y);   // xp3 - xp1
+    const f2 v1 = fmk(exl[km].x - xl[k].x, exl[km].y - xl[k].y);    // xp3 - xp0
+    const f2 v2 = fmk(exl[k].x - xl[k].x, exl[k].y - xl[k].y);      // xp1 - xp0
+    const f2 dxx1 = fscl(0.5f, fmk(exl[k].x - xl[k].x - exl[km].x, exl[k].y - xl[k].y - exl[km].y));
+    const f2 dxx2 = fscl(0.5f, fmk(exl[km].x - xl[k].x - exl[k].x, exl[km].y - xl[k].y - exl[k].y));
+    const f2 du20 = fmk(-ul[k].x, -ul[k].y);                        // up2 - up0
+    const f2 du31 = fmk(u3.x - u1.x, u3.y - u1.y);                  // up3 - up1
+    const f2 dva = fmk(u1.x - ul[k].x - u3.x, u1.y - ul[k].y - u3.y);   // up1+up2-up0-up3
+    const f2 dvb = fmk(u3.x - ul[k].x - u1.x, u3.y - ul[k].y - u1.y);   // up2+up3-up0-up1
+    c.duav = faxpy(0.25f, fadd(fadd(ul[k], u1), u3), zucf);
+    const float cvol = 0.5f * fcross(A, B);
+    c.area = cvol;
+    const float de1 = el[km], de2 = el[k];
+    const float minelen = fminr(de1, de2);
+    c.minelen = minelen;
+    c.ccos = (minelen < 1.e-12f) ? 0.f : 4.f * fdot(v1, v2) * frcp(de1 * de2);
+    c.div = (fcross(du20, B) - fcross(du31, A)) * frcp(2.f * cvol);
+    const float dx1 = flen(dxx1), dx2 = flen(dxx2);
+    const float t1 = fabsf(fdot(dxx1, c.duav) * dx2);
+    const float t2 = fabsf(fdot(dxx2, c.duav) * dx1);
+    const float num = (t1 > t2) ? dx1 : dx2;
+    const float den = (t1 > t2) ? dx2 : dx1;
+    float evol = fsqrt(4.f * cvol * num * frcp(den));
+    c.evol = fminr(evol, 2.f * minelen);
+    c.du = fsqrt(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
+    return c;
+}
+
+// The network features of corner k (closure.h); the quotients are reciprocal
+// products (within an ulp or two: the network reads them in BF16).
+template <int NS>
+__device__ __forceinline__ void corner_features(const MixedFrame<NS>& F, int k, const MixedCrn& c,
+                                                float zssf, float zrpf, float q1f, float gq,
+                                                float (&f)[PN_MLP_NF]) {
+    const f2 (&ul)[NS] = F.ul;
+    const int km = prv<NS>(k), kp = nxt<NS>(k);
+    const float inv = frcp(zssf);
+    f[0] = slog((ul[k].x - ul[km].x) * inv);
+    f[1] = slog((ul[k].y - ul[km].y) * inv);
+    f[2] = slog((ul[kp].x - ul[k].x) * inv);
+    f[3] = slog((ul[kp].y - ul[k].y) * inv);
+    f[4] = slog(c.div * c.evol * inv);
+    f[5] = c.evol * frcp(c.minelen);
+    f[6] = zssf * frcp(zssf + flen(c.duav));
+    f[7] = zrpf * frcp((float)F.r);
+    f[8] = slog(c.du * inv);
+    const float tl = q1f * zssf, tq = gq * c.du;
+    f[9] = (tl + tq > 0.f) ? tq * frcp(tl + tq) : 0.f;
+}
+
+// The mixed-precision predictor half of zone z (sides sb..sb+NS-1), from its
+// points' positions x0 and velocities u0 and its state st.  With CL !=
+// CL_EXACT it must be called by all 64 lanes of a wave together (the network
+// runs across the wave); `valid` false marks a lane that only takes part in
+// the wave-wide steps and stores nothing.  Writes zp to a.zp.
+template <int NS, int CL>
+__device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z, int sb, SideW sw,
+                                                       const d2 (&x0)[NS], const d2 (&u0)[NS],
+                                                       const ZoneState& st, double dt, int& bad,
+                                                       bool valid, ClCount& cnt) {
+    MixedFrame<NS> F;
+    mixed_frame<NS>(F, a, z, sb, x0, u0, st, dt, bad, valid, CL != CL_RECOMP);
+    const float (&mf)[NS] = F.mf;
+    const int (&slot)[NS] = F.slot;
+    const f2 (&ul)[NS] = F.ul;
+    const f2 (&xl)[NS] = F.xl;
+    const float (&el)[NS] = F.el;
+    const f2 (&sfp)[NS] = F.sfp;
+    const f2 (&sft)[NS] = F.sft;
+    const d2 zuc = F.zuc;
+    const double zss = F.zss, zrp = F.zrp, zarea = F.zarea;
+    const float zdl = F.zdl;
     // QCS corner quantities (corner k: xp0 = x[k], xp1 = ex[k], xp2 = zx,
     // xp3 = ex[k-1]; up2 = zuc is the frame's zero velocity)
     const f2 zucf = tof(zuc);
@@ -761,35 +875,11 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
     unsigned gated = 0;       // bit k: corner k passes the gate
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const int km = prv<NS>(k), kp = nxt<NS>(k);
-        const f2 u1 = fscl(0.5f, fadd(ul[k], ul[kp]));                 // up1 - zuc
-        const f2 u3 = fscl(0.5f, fadd(ul[km], ul[k]));                 // up3 - zuc
-        const f2 A = fmk(-xl[k].x, -xl[k].y);                           // xp2 - xp0
-        const f2 B = fmk(exl[km].x - exl[k].x, exl[km].y - exl[k].y);   // xp3 - xp1
-        const f2 v1 = fmk(exl[km].x - xl[k].x, exl[km].y - xl[k].y);    // xp3 - xp0
-        const f2 v2 = fmk(exl[k].x - xl[k].x, exl[k].y - xl[k].y);      // xp1 - xp0
-        const f2 dxx1 = fscl(0.5f, fmk(exl[k].x - xl[k].x - exl[km].x, exl[k].y - xl[k].y - exl[km].y));
-        const f2 dxx2 = fscl(0.5f, fmk(exl[km].x - xl[k].x - exl[k].x, exl[km].y - xl[k].y - exl[k].y));
-        const f2 du20 = fmk(-ul[k].x, -ul[k].y);                        // up2 - up0
-        const f2 du31 = fmk(u3.x - u1.x, u3.y - u1.y);                  // up3 - up1
-        const f2 dva = fmk(u1.x - ul[k].x - u3.x, u1.y - ul[k].y - u3.y);   // up1+up2-up0-up3
-        const f2 dvb = fmk(u3.x - ul[k].x - u1.x, u3.y - ul[k].y - u1.y);   // up2+up3-up0-up1
-        const f2 duav = faxpy(0.25f, fadd(fadd(ul[k], u1), u3), zucf);
-        const float cvol = 0.5f * fcross(A, B);
-        carea[k] = cvol;
-        const float de1 = el[km], de2 = el[k];
-        const float minelen = fminr(de1, de2);
-        ccos[k] = (minelen < 1.e-12f) ? 0.f : 4.f * fdot(v1, v2) * frcp(de1 * de2);
-        const float div = (fcross(du20, B) - fcross(du31, A)) * frcp(2.f * cvol);
+        const MixedCrn c = mixed_corner<NS>(F, k, zucf);
+        const float div = c.div, evol = c.evol, du = c.du;
+        carea[k] = c.area;
+        ccos[k] = c.ccos;
         cdiv[k] = div;
-        const float dx1 = flen(dxx1), dx2 = flen(dxx2);
-        const float t1 = fabsf(fdot(dxx1, duav) * dx2);
-        const float t2 = fabsf(fdot(dxx2, duav) * dx1);
-        const float num = (t1 > t2) ? dx1 : dx2;
-        const float den = (t1 > t2) ? dx2 : dx1;
-        float evol = fsqrt(4.f * cvol * num * frcp(den));
-        evol = fminr(evol, 2.f * minelen);
-        const float du = fsqrt(fmaxr(fdot(dva, dva), fdot(dvb, dvb)));
         cevol[k] = (div < 0.f) ? evol : 0.f;
         cdu[k] = (div < 0.f) ? du : 0.f;
         if constexpr (CL == CL_FLAG || CL == CL_LEARNED || CL == CL_RECOMP) {
@@ -800,22 +890,8 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
             }
         }
         if constexpr (CL == CL_LEARNED || CL == CL_RECOMP || CL == CL_CAPTURE) {
-            // network features of this corner (closure.h); the quotients are
-            // reciprocal products (within an ulp or two: the network reads
-            // them in BF16)
-            const float inv = frcp(zssf);
             float f[PN_MLP_NF];
-            f[0] = slog((ul[k].x - ul[km].x) * inv);
-            f[1] = slog((ul[k].y - ul[km].y) * inv);
-            f[2] = slog((ul[kp].x - ul[k].x) * inv);
-            f[3] = slog((ul[kp].y - ul[k].y) * inv);
-            f[4] = slog(div * evol * inv);
-            f[5] = evol * frcp(minelen);
-            f[6] = zssf * frcp(zssf + flen(duav));
-            f[7] = zrpf * frcp((float)r);
-            f[8] = slog(du * inv);
-            const float tl = q1f * zssf, tq = gq * du;
-            f[9] = (tl + tq > 0.f) ? tq * frcp(tl + tq) : 0.f;
+            corner_features<NS>(F, k, c, zssf, zrpf, q1f, gq, f);
             if constexpr (CL == CL_LEARNED || CL == CL_RECOMP) {
                 if (gated & (1u << k)) {
 #pragma unroll
@@ -829,6 +905,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
                 fhi[k] = hi[0];
             } else {
                 // a training tuple: features and the exact coefficients
+                const float tl = q1f * zssf, tq = gq * du;
                 const uint64_t m = __ballot(valid && div < 0.f);
                 if (m) {
                     unsigned long long base = 0;
@@ -994,17 +1071,14 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
 // The predictor of zone z whose points are pp (mapsp1 of its sides); every
 // load of the zone is issued before the first use of a loaded value.  zre:
 // the control block's zre_valid, read once by the caller.
-template <int NS, int CL>
-__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, SideW sw,
-                                                  const int (&pp)[NS], double dt, int& bad,
-                                                  bool valid, ClCount& cnt, bool zre) {
-    d2 x0[NS], u0[NS];
+template <int NS>
+__device__ __forceinline__ void load_zone_mixed(const FusedArgs& a, int z, const int (&pp)[NS], bool zre,
+                                                d2 (&x0)[NS], d2 (&u0)[NS], ZoneState& st) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         x0[k] = mk(a.pxx[pp[k]], a.pxy[pp[k]]);
         u0[k] = mk(a.pux[pp[k]], a.puy[pp[k]]);
     }
-    ZoneState st;
     st.zm = a.zm[z];
     st.zvol = a.zvol[z];
     st.zwrate = a.zwrate[z];
@@ -1016,6 +1090,15 @@ __device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int
         st.zr = st.zm / st.zvol;
         st.ze = a.zetot[z] / (st.zm + 1.e-99);
     }
+}
+
+template <int NS, int CL>
+__device__ __forceinline__ double zone_pred_mixed(const FusedArgs& a, int z, int sb, SideW sw,
+                                                  const int (&pp)[NS], double dt, int& bad,
+                                                  bool valid, ClCount& cnt, bool zre) {
+    d2 x0[NS], u0[NS];
+    ZoneState st;
+    load_zone_mixed<NS>(a, z, pp, zre, x0, u0, st);
     return zone_pred_mixed_core<NS, CL>(a, z, sb, sw, x0, u0, st, dt, bad, valid, cnt);
 }
 
@@ -2522,6 +2605,37 @@ int fused_setup(pennant_problem* pb) {
     return (int)hipMemcpy(pb->pbc, bits.data(), bits.size(), hipMemcpyHostToDevice);
 }
 
+namespace {
+
+// The learned pass over the zones a flagging predictor listed; with
+// `pending` its counts go to the pending counters.
+int launch_learned_pass(const pennant_problem* pb, const FusedArgs& a, bool pending, hipStream_t s) {
+    FusedArgs b = a;
+    if (pending) b.cl_stats = pb->cl_counters + 3;
+    const HostMesh& h = pb->hm;
+    for (size_t r = 0; r < pb->zruns.size(); r += 3) {
+        const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
+        const int sb = h.zsbeg[zb];
+        // the learned pass holds 2 blocks per CU (registers): 512 blocks
+        // fill the chip without dispatching blocks that find no work
+        const int g = pred_grid(pb, ze - zb, ns, FORM_LEARNED);
+        const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
+        const int run = (int)(r / 3);
+        switch (ns) {
+            case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, b, zb, ze, sb, run); break;
+            default: return PENNANT_EINVAL;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
 int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
     if (form == FORM_LEARNED && !pb->closure) return PENNANT_EINVAL;
     const bool mixed = (form != FORM_FP64);
@@ -2548,24 +2662,8 @@ int fused_cycle(pennant_problem* pb, hipEvent_t* ev, int form) {
     const int npred = poff;
     if (ev) (void)hipEventRecord(ev[1], s);
     if (form == FORM_LEARNED) {
-        for (size_t r = 0; r < pb->zruns.size(); r += 3) {
-            const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
-            const int sb = h.zsbeg[zb];
-            // the learned pass holds 2 blocks per CU (registers): 512 blocks
-            // fill the chip without dispatching blocks that find no work
-            const int g = pred_grid(pb, ze - zb, ns, form);
-            const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
-            const int run = (int)(r / 3);
-            switch (ns) {
-                case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, a, zb, ze, sb, run); break;
-                default: return PENNANT_EINVAL;
-            }
-        }
+        const int rc = launch_learned_pass(pb, a, false, s);
+        if (rc) return rc;
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     const dim3 pgrid(pb->nheavy + nblocks(own_points(pb)));
@@ -2638,32 +2736,6 @@ int launch_lookahead_pred(const pennant_problem* pb, const FusedArgs& a, int for
     return poff;
 }
 
-// The learned pass over the zones a flagging predictor listed; with
-// `pending` its counts go to the pending counters.
-int launch_learned_pass(const pennant_problem* pb, const FusedArgs& a, bool pending, hipStream_t s) {
-    FusedArgs b = a;
-    if (pending) b.cl_stats = pb->cl_counters + 3;
-    const HostMesh& h = pb->hm;
-    for (size_t r = 0; r < pb->zruns.size(); r += 3) {
-        const int zb = pb->zruns[r], ze = pb->zruns[r + 1], ns = pb->zruns[r + 2];
-        const int sb = h.zsbeg[zb];
-        // the learned pass holds 2 blocks per CU (registers): 512 blocks
-        // fill the chip without dispatching blocks that find no work
-        const int g = pred_grid(pb, ze - zb, ns, FORM_LEARNED);
-        const dim3 grid(g < 512 ? g : 512), blk(PN_BLOCK);
-        const int run = (int)(r / 3);
-        switch (ns) {
-            case 3: hipLaunchKernelGGL(k_pred_list<3>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            case 4: hipLaunchKernelGGL(k_pred_list<4>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            case 5: hipLaunchKernelGGL(k_pred_list<5>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            case 6: hipLaunchKernelGGL(k_pred_list<6>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            case 7: hipLaunchKernelGGL(k_pred_list<7>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            case 8: hipLaunchKernelGGL(k_pred_list<8>, grid, blk, 0, s, b, zb, ze, sb, run); break;
-            default: return PENNANT_EINVAL;
-        }
-    }
-    return 0;
-}
 
 }  // namespace
 

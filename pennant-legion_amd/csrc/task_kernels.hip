@@ -143,29 +143,89 @@ __device__ __forceinline__ int zone_end(const Tile& t, const int* zsb, int zi) {
     const bool has_side = lane < t.ns;                        \
     [[maybe_unused]] const bool has_zone = lane < t.nz
 
+// Wide tiles: up to R * PN_TB sides of at most PN_TB whole zones; thread t
+// takes the tile's sides t, t + PN_TB, ..., t + (R - 1) PN_TB and issues the
+// loads of all of them before it uses any.  The tasks are chains of
+// dependent loads (zone offsets, then maps, then point values), so their
+// time is set by how many chains each wave has in flight.  R is 4 where the
+// tile's LDS arrays leave four blocks per CU at that width, else 2.
+template <int R>
+__device__ __forceinline__ Tile wide_tile_setup(const int32_t* zsbeg, int numz, int tz, int* zsb,
+                                                unsigned char* zof) {
+    Tile t;
+    t.z0 = blockIdx.x * tz;
+    t.nz = min(tz, numz - t.z0);
+    if ((int)threadIdx.x <= t.nz) zsb[threadIdx.x] = zsbeg[t.z0 + threadIdx.x];
+    if (threadIdx.x == 0 && t.nz == PN_TB) zsb[PN_TB] = zsbeg[t.z0 + PN_TB];
+    __syncthreads();
+    t.s0 = zsb[0];
+    t.ns = min(zsb[t.nz] - t.s0, R * PN_TB);
+    if ((int)threadIdx.x < t.nz) {
+        const int b = zsb[threadIdx.x] - t.s0;
+        const int e = min(zsb[threadIdx.x + 1] - t.s0, R * PN_TB);
+        for (int i = b; i < e; ++i) zof[i] = (unsigned char)threadIdx.x;
+    }
+    __syncthreads();
+    return t;
+}
+template <int R>
+__device__ __forceinline__ int wide_prev(const SideAt& c) {
+    return min(c.b + (c.k == 0 ? c.n - 1 : c.k - 1), R * PN_TB - 1);
+}
+template <int R>
+__device__ __forceinline__ int wide_next(const SideAt& c) {
+    return min(c.b + (c.k + 1 == c.n ? 0 : c.k + 1), R * PN_TB - 1);
+}
+template <int R>
+__device__ __forceinline__ int wide_zone_end(const Tile& t, const int* zsb, int zi) {
+    return min(zsb[zi + 1] - t.s0, R * PN_TB);
+}
+// Shared memory and indices of a wide-tile kernel; SIDE(j) is the tile-local
+// index of the thread's j-th side, SIDEC(j) the mesh index it loads from
+// (the tile's last side for a thread past the end, which stores nothing).
+#define PN_WIDE_SHARED(R)                                              \
+    __shared__ int zsb[PN_TB + 1];                                     \
+    __shared__ unsigned char zof[(R) * PN_TB];                         \
+    const Tile t = wide_tile_setup<R>(m.zsbeg, m.numz, tz, zsb, zof);  \
+    const int lane = threadIdx.x;                                      \
+    [[maybe_unused]] const bool has_zone = lane < t.nz
+#define SIDE(j) (lane + (j) * PN_TB)
+#define SIDEC(j) (t.s0 + min(SIDE(j), t.ns - 1))
+
 // ---------------------------------------------------------------------------
 // Mesh tasks
 
 // calcCtrsTask, src/Mesh.cc:797-837: ex = (px1 + px2) / 2 per side; zx =
-// sum of px1 * (1/n) over the zone's sides, ascending, from 0.
+// sum of px1 * (1/n) over the zone's sides, ascending, from 0.  Wide tile.
 template <int L>
 __global__ void __launch_bounds__(PN_TB) k_calc_ctrs(pennant_mesh m, int tz, pennant_cvec px,
                                                      pennant_vec ex, pennant_vec zx) {
-    PN_TILE_SHARED;
-    __shared__ d2 x1[PN_TB], xn[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const d2 p = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
-        x1[lane] = p;
-        xn[lane] = dvs(p, (double)c.n);
-    }
+    constexpr int R = 4;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 x1[R * PN_TB];
+    int p[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) p[j] = m.mapsp1[SIDEC(j)];
+    d2 x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = VIO<L>::ld(px, p[j]);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+        if (SIDE(j) < t.ns) x1[SIDE(j)] = x[j];
     __syncthreads();
-    if (has_side) VIO<L>::st(ex, t.s0 + lane, scl(0.5, add(x1[lane], x1[side_next(c)])));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            VIO<L>::st(ex, t.s0 + i, scl(0.5, add(x[j], x1[wide_next<R>(c)])));
+        }
+    }
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, e = wide_zone_end<R>(t, zsb, lane);
+        const double n = (double)(zsb[lane + 1] - zsb[lane]);
         d2 acc = mk(0., 0.);
-        for (int i = b; i < e; ++i) acc = add(acc, xn[i]);
+        for (int i = b; i < e; ++i) acc = add(acc, dvs(x1[i], n));
         VIO<L>::st(zx, t.z0 + lane, acc);
     }
 }
@@ -177,31 +237,41 @@ template <int L>
 __global__ void __launch_bounds__(PN_TB) k_calc_vols(pennant_mesh m, int tz, pennant_cvec px,
                                                      pennant_cvec zx, double* sarea, double* svol,
                                                      double* zarea, double* zvol, int* numsbad) {
-    PN_TILE_SHARED;
-    __shared__ d2 x1[PN_TB], zc[PN_TB];
-    __shared__ double sa_[PN_TB], sv_[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        x1[lane] = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
-    }
+    constexpr int R = 4;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 x1[R * PN_TB], zc[PN_TB];
+    __shared__ double sa_[R * PN_TB], sv_[R * PN_TB];
+    int p[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) p[j] = m.mapsp1[SIDEC(j)];
     if (has_zone) zc[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    d2 x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = VIO<L>::ld(px, p[j]);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+        if (SIDE(j) < t.ns) x1[SIDE(j)] = x[j];
     __syncthreads();
     int count = 0;
-    if (has_side) {
-        const double third = 1. / 3.;
-        const d2 p1 = x1[lane], p2 = x1[side_next(c)], z = zc[c.zi];
-        const double sa = 0.5 * cross(sub(p2, p1), sub(z, p1));
-        const double sv = third * sa * (p1.x + p2.x + z.x);
-        sarea[t.s0 + lane] = sa;
-        svol[t.s0 + lane] = sv;
-        sa_[lane] = sa;
-        sv_[lane] = sv;
-        if (sv <= 0.) count = 1;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            const double third = 1. / 3.;
+            const d2 p1 = x[j], p2 = x1[wide_next<R>(c)], z = zc[c.zi];
+            const double sa = 0.5 * cross(sub(p2, p1), sub(z, p1));
+            const double sv = third * sa * (p1.x + p2.x + z.x);
+            sarea[t.s0 + i] = sa;
+            svol[t.s0 + i] = sv;
+            sa_[i] = sa;
+            sv_[i] = sv;
+            if (sv <= 0.) count += 1;
+        }
     }
     __syncthreads();
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, e = wide_zone_end<R>(t, zsb, lane);
         double za = 0., zv = 0.;
         for (int i = b; i < e; ++i) {
             za += sa_[i];
@@ -234,29 +304,53 @@ __global__ void __launch_bounds__(PN_TB) k_calc_side_fracs(pennant_mesh m, int t
 template <int L>
 __global__ void __launch_bounds__(PN_TB) k_calc_surf_vecs(pennant_mesh m, int tz, pennant_cvec ex,
                                                           pennant_cvec zx, pennant_vec ssurf) {
-    PN_TILE_SHARED;
+    constexpr int R = 4;
+    PN_WIDE_SHARED(R);
     __shared__ d2 zc[PN_TB];
     if (has_zone) zc[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    d2 e[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) e[j] = VIO<L>::ld(ex, SIDEC(j));
     __syncthreads();
-    if (has_side) {
-        const SideAt c = side_at(t, zsb, zof, lane);
-        VIO<L>::st(ssurf, t.s0 + lane, rot_ccw(sub(VIO<L>::ld(ex, t.s0 + lane), zc[c.zi])));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            VIO<L>::st(ssurf, t.s0 + i, rot_ccw(sub(e[j], zc[c.zi])));
+        }
     }
 }
 
-// calcEdgeLenTask, src/Mesh.cc:1151-1179: elen = |px2 - px1|
+// calcEdgeLenTask, src/Mesh.cc:1151-1188: elen = |px[mapsp2] - px[mapsp1]|
+// per side.  Sides are independent, so there is no zone tile: every lane
+// takes PN_EL sides a block apart and issues all their map loads, then all
+// their point loads, before it uses any (the task is a two-step dependent
+// gather, so its time is the number of such chains a wave has in flight).
+#define PN_EL 4
 template <int L>
-__global__ void __launch_bounds__(PN_TB) k_calc_edge_len(pennant_mesh m, int tz, pennant_cvec px,
-                                                         double* elen) {
-    PN_TILE_SHARED;
-    __shared__ d2 x1[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        x1[lane] = VIO<L>::ld(px, m.mapsp1[t.s0 + lane]);
+__global__ void __launch_bounds__(PN_BLOCK) k_calc_edge_len(int nums, const int32_t* __restrict__ mapsp1,
+                                                            const int32_t* __restrict__ mapsp2,
+                                                            pennant_cvec px, double* elen) {
+    const int s0 = blockIdx.x * (PN_BLOCK * PN_EL) + (int)threadIdx.x;
+    int p1[PN_EL], p2[PN_EL];
+#pragma unroll
+    for (int j = 0; j < PN_EL; ++j) {
+        const int s = min(s0 + j * PN_BLOCK, nums - 1);
+        p1[j] = __builtin_nontemporal_load(mapsp1 + s);
+        p2[j] = __builtin_nontemporal_load(mapsp2 + s);
     }
-    __syncthreads();
-    if (has_side) elen[t.s0 + lane] = len(sub(x1[side_next(c)], x1[lane]));
+    d2 a[PN_EL], b[PN_EL];
+#pragma unroll
+    for (int j = 0; j < PN_EL; ++j) {
+        a[j] = VIO<L>::ld(px, p1[j]);
+        b[j] = VIO<L>::ld(px, p2[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < PN_EL; ++j) {
+        const int s = s0 + j * PN_BLOCK;
+        if (s < nums) __builtin_nontemporal_store(len(sub(b[j], a[j])), elen + s);
+    }
 }
 
 // calcCharLenTask, src/Mesh.cc:1216-1244: zdl = min(1e99, min over the
@@ -264,16 +358,27 @@ __global__ void __launch_bounds__(PN_TB) k_calc_edge_len(pennant_mesh m, int tz,
 __global__ void __launch_bounds__(PN_TB) k_calc_char_len(pennant_mesh m, int tz,
                                                          const double* sarea, const double* elen,
                                                          double* zdl) {
-    PN_TILE_SHARED;
-    __shared__ double sdl[PN_TB];
-    if (has_side) {
-        const SideAt c = side_at(t, zsb, zof, lane);
-        const double fac = (c.n == 3 ? 3. : 4.);
-        sdl[lane] = fac * sarea[t.s0 + lane] / elen[t.s0 + lane];
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
+    __shared__ double sdl[R * PN_TB];
+    double sa[R], el[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        sa[j] = sarea[SIDEC(j)];
+        el[j] = elen[SIDEC(j)];
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            const double fac = (c.n == 3 ? 3. : 4.);
+            sdl[i] = fac * sa[j] / el[j];
+        }
     }
     __syncthreads();
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, e = wide_zone_end<R>(t, zsb, lane);
         double dl = 1.e99;
         for (int i = b; i < e; ++i) dl = smin(dl, sdl[i]);
         zdl[t.z0 + lane] = dl;
@@ -620,30 +725,53 @@ __global__ void __launch_bounds__(PN_TB) k_calc_work(pennant_mesh m, int tz, pen
                                                      pennant_cvec sfq, pennant_cvec pu,
                                                      pennant_cvec pu0, pennant_cvec pxp,
                                                      double* zw, double* zetot, const double* dtp) {
-    PN_TILE_SHARED;
-    __shared__ d2 vs[PN_TB];
-    __shared__ double xx[PN_TB], dw[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const int p1 = m.mapsp1[t.s0 + lane];
-        vs[lane] = add(VIO<L>::ld(pu0, p1), VIO<L>::ld(pu, p1));
-        xx[lane] = VIO<L>::ldx(pxp, p1);
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 vs[R * PN_TB];
+    __shared__ double xx[R * PN_TB], dw[R * PN_TB];
+    int p[R];
+    d2 sp[R], sq[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        p[j] = m.mapsp1[SIDEC(j)];
+        sp[j] = VIO<L>::ld(sfp, SIDEC(j));
+        sq[j] = VIO<L>::ld(sfq, SIDEC(j));
+    }
+    const double et0 = has_zone ? zetot[t.z0 + lane] : 0.;
+    d2 v[R];
+    double xv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        v[j] = add(VIO<L>::ld(pu0, p[j]), VIO<L>::ld(pu, p[j]));
+        xv[j] = VIO<L>::ldx(pxp, p[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            vs[i] = v[j];
+            xx[i] = xv[j];
+        }
     }
     __syncthreads();
-    if (has_side) {
-        const double dth = 0.5 * (*dtp);
-        const int s = t.s0 + lane, n2 = side_next(c);
-        const d2 sftot = add(VIO<L>::ld(sfp, s), VIO<L>::ld(sfq, s));
-        const double sd1 = dot(sftot, vs[lane]);
-        const double sd2 = dot(neg(sftot), vs[n2]);
-        dw[lane] = -dth * (sd1 * xx[lane] + sd2 * xx[n2]);
+    const double dth = 0.5 * (*dtp);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            const int n2 = wide_next<R>(c);
+            const d2 sftot = add(sp[j], sq[j]);
+            const double sd1 = dot(sftot, v[j]);
+            const double sd2 = dot(neg(sftot), vs[n2]);
+            dw[i] = -dth * (sd1 * xv[j] + sd2 * xx[n2]);
+        }
     }
     __syncthreads();
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, e = wide_zone_end<R>(t, zsb, lane);
         const int z = t.z0 + lane;
-        double w = 0., et = zetot[z];
+        double w = 0., et = et0;
         for (int i = b; i < e; ++i) {
             et += dw[i];
             w += dw[i];
@@ -741,13 +869,21 @@ template <int L>
 __global__ void __launch_bounds__(PN_TB) k_calc_force_pgas(pennant_mesh m, int tz,
                                                            pennant_cvec ssurf, const double* zp,
                                                            pennant_vec sf) {
-    PN_TILE_SHARED;
+    constexpr int R = 4;
+    PN_WIDE_SHARED(R);
     __shared__ double zpl[PN_TB];
     if (has_zone) zpl[lane] = zp[t.z0 + lane];
+    d2 ss[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ss[j] = VIO<L>::ld(ssurf, SIDEC(j));
     __syncthreads();
-    if (has_side) {
-        const SideAt c = side_at(t, zsb, zof, lane);
-        VIO<L>::st(sf, t.s0 + lane, scl(-zpl[c.zi], VIO<L>::ld(ssurf, t.s0 + lane)));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            VIO<L>::st(sf, t.s0 + i, scl(-zpl[c.zi], ss[j]));
+        }
     }
 }
 
@@ -759,24 +895,36 @@ __global__ void __launch_bounds__(PN_TB) k_calc_force_tts(pennant_mesh m, int tz
                                                           const double* zr, const double* zss,
                                                           pennant_vec sf, double alfa,
                                                           double ssmin) {
-    PN_TILE_SHARED;
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
     __shared__ double za[PN_TB], zrl[PN_TB], zsl[PN_TB];
     if (has_zone) {
         za[lane] = zarea[t.z0 + lane];
         zrl[lane] = zr[t.z0 + lane];
         zsl[lane] = zss[t.z0 + lane];
     }
+    double sa[R], mf[R];
+    d2 ss[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        sa[j] = sarea[SIDEC(j)];
+        mf[j] = smf[SIDEC(j)];
+        ss[j] = VIO<L>::ld(ssurf, SIDEC(j));
+    }
     __syncthreads();
-    if (has_side) {
-        const SideAt c = side_at(t, zsb, zof, lane);
-        const int s = t.s0 + lane;
-        const double vfacinv = za[c.zi] / sarea[s];
-        const double r = zrl[c.zi];
-        const double srho = r * smf[s] * vfacinv;
-        double sstmp = smax(zsl[c.zi], ssmin);
-        sstmp = alfa * sstmp * sstmp;
-        const double sdp = sstmp * (srho - r);
-        VIO<L>::st(sf, s, scl(-sdp, VIO<L>::ld(ssurf, s)));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            const double vfacinv = za[c.zi] / sa[j];
+            const double r = zrl[c.zi];
+            const double srho = r * mf[j] * vfacinv;
+            double sstmp = smax(zsl[c.zi], ssmin);
+            sstmp = alfa * sstmp * sstmp;
+            const double sdp = sstmp * (srho - r);
+            VIO<L>::st(sf, t.s0 + i, scl(-sdp, ss[j]));
+        }
     }
 }
 
@@ -796,72 +944,91 @@ __global__ void __launch_bounds__(PN_TB) k_set_corner_div(pennant_mesh m, int tz
                                                           pennant_vec zuc, double* carea,
                                                           double* ccos, double* cdiv,
                                                           double* cevol, double* cdu) {
-    PN_TILE_SHARED;
-    __shared__ d2 u1[PN_TB], x1[PN_TB], e1[PN_TB], un[PN_TB], zxl[PN_TB], zul[PN_TB];
-    __shared__ double el[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const int s = t.s0 + lane;
-        const int p1 = m.mapsp1[s];
-        const d2 u = VIO<L>::ld(pu, p1);
-        u1[lane] = u;
-        un[lane] = dvs(u, (double)c.n);
-        x1[lane] = VIO<L>::ld(px, p1);
-        e1[lane] = VIO<L>::ld(ex, s);
-        el[lane] = elen[s];
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 u1[R * PN_TB], e1[R * PN_TB], zxl[PN_TB], zul[PN_TB];
+    __shared__ double el[R * PN_TB];
+    int p[R];
+    d2 e[R];
+    double l[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        p[j] = m.mapsp1[SIDEC(j)];
+        e[j] = VIO<L>::ld(ex, SIDEC(j));
+        l[j] = elen[SIDEC(j)];
     }
     if (has_zone) zxl[lane] = VIO<L>::ld(zx, t.z0 + lane);
+    d2 u[R], x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        u[j] = VIO<L>::ld(pu, p[j]);
+        x[j] = VIO<L>::ld(px, p[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            u1[i] = u[j];
+            e1[i] = e[j];
+            el[i] = l[j];
+        }
+    }
     __syncthreads();
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, ez = wide_zone_end<R>(t, zsb, lane);
+        const double n = (double)(zsb[lane + 1] - zsb[lane]);
         d2 acc = mk(0., 0.);
-        for (int i = b; i < e; ++i) acc = add(acc, un[i]);
+        for (int i = b; i < ez; ++i) acc = add(acc, dvs(u1[i], n));
         zul[lane] = acc;
         VIO<L>::st(zuc, t.z0 + lane, acc);
     }
     __syncthreads();
-    if (!has_side) return;
-    const int ip = side_prev(c), in = side_next(c);
-    const int cc = t.s0 + lane;
-    const d2 up0 = u1[lane];
-    const d2 xp0 = x1[lane];
-    const d2 up1 = scl(0.5, add(up0, u1[in]));
-    const d2 xp1 = e1[lane];
-    const d2 up2 = zul[c.zi];
-    const d2 xp2 = zxl[c.zi];
-    const d2 up3 = scl(0.5, add(u1[ip], up0));
-    const d2 xp3 = e1[ip];
-    const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
-    carea[cc] = cvolume;
-    const d2 v1 = sub(xp3, xp0);
-    const d2 v2 = sub(xp1, xp0);
-    const double de1 = el[ip];
-    const double de2 = el[lane];
-    const double minelen = smin(de1, de2);
-    ccos[cc] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
-    const double div = (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) /
-                       (2.0 * cvolume);
-    cdiv[cc] = div;
-    const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
-    const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
-    const double dx1 = len(dxx1);
-    const double dx2 = len(dxx2);
-    const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
-    const double test1 = fabs(dot(dxx1, duav) * dx2);
-    const double test2 = fabs(dot(dxx2, duav) * dx1);
-    const double num = (test1 > test2 ? dx1 : dx2);
-    const double den = (test1 > test2 ? dx2 : dx1);
-    const double r = num / den;
-    double evol = __builtin_sqrt(4.0 * cvolume * r);
-    evol = smin(evol, 2.0 * minelen);
-    const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
-    const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
-    double du = __builtin_sqrt(smax(dv1, dv2));
-    evol = (div < 0.0 ? evol : 0.);
-    du = (div < 0.0 ? du : 0.);
-    cevol[cc] = evol;
-    cdu[cc] = du;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i >= t.ns) continue;
+        const SideAt c = side_at(t, zsb, zof, i);
+        const int ip = wide_prev<R>(c), in = wide_next<R>(c);
+        const int cc = t.s0 + i;
+        const d2 up0 = u[j];
+        const d2 xp0 = x[j];
+        const d2 up1 = scl(0.5, add(up0, u1[in]));
+        const d2 xp1 = e[j];
+        const d2 up2 = zul[c.zi];
+        const d2 xp2 = zxl[c.zi];
+        const d2 up3 = scl(0.5, add(u1[ip], up0));
+        const d2 xp3 = e1[ip];
+        const double cvolume = 0.5 * cross(sub(xp2, xp0), sub(xp3, xp1));
+        carea[cc] = cvolume;
+        const d2 v1 = sub(xp3, xp0);
+        const d2 v2 = sub(xp1, xp0);
+        const double de1 = el[ip];
+        const double de2 = l[j];
+        const double minelen = smin(de1, de2);
+        ccos[cc] = ((minelen < 1.e-12) ? 0. : 4. * dot(v1, v2) / (de1 * de2));
+        const double div =
+            (cross(sub(up2, up0), sub(xp3, xp1)) - cross(sub(up3, up1), sub(xp2, xp0))) / (2.0 * cvolume);
+        cdiv[cc] = div;
+        const d2 dxx1 = scl(0.5, sub(sub(add(xp1, xp2), xp0), xp3));
+        const d2 dxx2 = scl(0.5, sub(sub(add(xp2, xp3), xp0), xp1));
+        const double dx1 = len(dxx1);
+        const double dx2 = len(dxx2);
+        const d2 duav = scl(0.25, add(add(add(up0, up1), up2), up3));
+        const double test1 = fabs(dot(dxx1, duav) * dx2);
+        const double test2 = fabs(dot(dxx2, duav) * dx1);
+        const double num = (test1 > test2 ? dx1 : dx2);
+        const double den = (test1 > test2 ? dx2 : dx1);
+        const double r = num / den;
+        double evol = __builtin_sqrt(4.0 * cvolume * r);
+        evol = smin(evol, 2.0 * minelen);
+        const double dv1 = len2(sub(sub(add(up1, up2), up0), up3));
+        const double dv2 = len2(sub(sub(add(up2, up3), up0), up1));
+        double du = __builtin_sqrt(smax(dv1, dv2));
+        evol = (div < 0.0 ? evol : 0.);
+        du = (div < 0.0 ? du : 0.);
+        cevol[cc] = evol;
+        cdu[cc] = du;
+    }
 }
 
 // QCS::setQCnForceTask, src/QCS.cc:246-327: crmu, and the viscous edge
@@ -875,34 +1042,53 @@ __global__ void __launch_bounds__(PN_TB) k_set_qcn_force(pennant_mesh m, int tz,
                                                          double* crmu, pennant_vec cqe1,
                                                          pennant_vec cqe2, double qgamma,
                                                          double q1, double q2) {
-    PN_TILE_SHARED;
-    __shared__ d2 u1[PN_TB];
-    __shared__ double el[PN_TB], zrl[PN_TB], zsl[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const int s = t.s0 + lane;
-        u1[lane] = VIO<L>::ld(pu, m.mapsp1[s]);
-        el[lane] = elen[s];
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 u1[R * PN_TB];
+    __shared__ double el[R * PN_TB], zrl[PN_TB], zsl[PN_TB];
+    int p[R];
+    double l[R], du[R], ev[R], dv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        p[j] = m.mapsp1[SIDEC(j)];
+        l[j] = elen[SIDEC(j)];
+        du[j] = cdu[SIDEC(j)];
+        ev[j] = cevol[SIDEC(j)];
+        dv[j] = cdiv[SIDEC(j)];
     }
     if (has_zone) {
         zrl[lane] = zrp[t.z0 + lane];
         zsl[lane] = zss[t.z0 + lane];
     }
+    d2 u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) u[j] = VIO<L>::ld(pu, p[j]);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            u1[i] = u[j];
+            el[i] = l[j];
+        }
+    }
     __syncthreads();
-    if (!has_side) return;
-    const int cc = t.s0 + lane;
-    const int ip = side_prev(c), in = side_next(c);
     const double gammap1 = qgamma + 1.0;
-    const double ztmp2 = q2 * 0.25 * gammap1 * cdu[cc];
-    const double ztmp1 = q1 * zsl[c.zi];
-    const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
-    const double mu = zkur * zrl[c.zi] * cevol[cc];
-    const double cm = ((cdiv[cc] > 0.0) ? 0. : mu);
-    crmu[cc] = cm;
-    const d2 u = u1[lane];
-    VIO<L>::st(cqe1, cc, dvs(scl(cm, sub(u, u1[ip])), el[ip]));
-    VIO<L>::st(cqe2, cc, dvs(scl(cm, sub(u1[in], u)), el[lane]));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i >= t.ns) continue;
+        const SideAt c = side_at(t, zsb, zof, i);
+        const int cc = t.s0 + i;
+        const int ip = wide_prev<R>(c), in = wide_next<R>(c);
+        const double ztmp2 = q2 * 0.25 * gammap1 * du[j];
+        const double ztmp1 = q1 * zsl[c.zi];
+        const double zkur = ztmp2 + __builtin_sqrt(ztmp2 * ztmp2 + ztmp1 * ztmp1);
+        const double mu = zkur * zrl[c.zi] * ev[j];
+        const double cm = ((dv[j] > 0.0) ? 0. : mu);
+        crmu[cc] = cm;
+        VIO<L>::st(cqe1, cc, dvs(scl(cm, sub(u[j], u1[ip])), el[ip]));
+        VIO<L>::st(cqe2, cc, dvs(scl(cm, sub(u1[in], u[j])), l[j]));
+    }
 }
 
 // QCS::setForceTask, src/QCS.cc:331-380: each lane clamps its corner's ccos
@@ -913,35 +1099,51 @@ __global__ void __launch_bounds__(PN_TB) k_set_force_qcs(pennant_mesh m, int tz,
                                                          pennant_cvec cqe1, pennant_cvec cqe2,
                                                          const double* elen, double* ccos,
                                                          double* cw, pennant_vec sfq) {
-    PN_TILE_SHARED;
-    __shared__ d2 q1l[PN_TB], q2l[PN_TB];
-    __shared__ double cwl[PN_TB], ccl[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const int s = t.s0 + lane;
-        const double cc = ccos[s];
-        const double csin2 = 1.0 - cc * cc;
-        const double w = ((csin2 < 1.e-4) ? 0. : carea[s] / csin2);
-        const double cl = ((csin2 < 1.e-4) ? 0. : cc);
-        cw[s] = w;
-        ccos[s] = cl;
-        cwl[lane] = w;
-        ccl[lane] = cl;
-        q1l[lane] = VIO<L>::ld(cqe1, s);
-        q2l[lane] = VIO<L>::ld(cqe2, s);
+    constexpr int R = 1;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 q1l[R * PN_TB], q2l[R * PN_TB];
+    __shared__ double cwl[R * PN_TB], ccl[R * PN_TB];
+    double cc0[R], ca[R], el[R];
+    d2 qa[R], qb[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        cc0[j] = ccos[SIDEC(j)];
+        ca[j] = carea[SIDEC(j)];
+        qa[j] = VIO<L>::ld(cqe1, SIDEC(j));
+        qb[j] = VIO<L>::ld(cqe2, SIDEC(j));
+        el[j] = elen[SIDEC(j)];
+    }
+    double w[R], cl[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const double csin2 = 1.0 - cc0[j] * cc0[j];
+        w[j] = ((csin2 < 1.e-4) ? 0. : ca[j] / csin2);
+        cl[j] = ((csin2 < 1.e-4) ? 0. : cc0[j]);
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            cw[t.s0 + i] = w[j];
+            ccos[t.s0 + i] = cl[j];
+            cwl[i] = w[j];
+            ccl[i] = cl[j];
+            q1l[i] = qa[j];
+            q2l[i] = qb[j];
+        }
     }
     __syncthreads();
-    if (!has_side) return;
-    const int s = t.s0 + lane, i2 = side_next(c);
-    const double el = elen[s];
-    const double cw1 = cwl[lane], cw2 = cwl[i2];
-    const double ccos1 = ccl[lane], ccos2 = ccl[i2];
-    const d2 cqe11 = q1l[lane], cqe12 = q1l[i2];
-    const d2 cqe21 = q2l[lane], cqe22 = q2l[i2];
-    const d2 f = add(scl(cw1, add(cqe21, scl(ccos1, cqe11))),
-                     scl(cw2, add(cqe12, scl(ccos2, cqe22))));
-    VIO<L>::st(sfq, s, dvs(f, el));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i >= t.ns) continue;
+        const SideAt c = side_at(t, zsb, zof, i);
+        const int i2 = wide_next<R>(c);
+        const double cw1 = w[j], cw2 = cwl[i2];
+        const double ccos1 = cl[j], ccos2 = ccl[i2];
+        const d2 cqe11 = qa[j], cqe12 = q1l[i2];
+        const d2 cqe21 = qb[j], cqe22 = q2l[i2];
+        const d2 f = add(scl(cw1, add(cqe21, scl(ccos1, cqe11))),
+                         scl(cw2, add(cqe12, scl(ccos2, cqe22))));
+        VIO<L>::st(sfq, t.s0 + i, dvs(f, el[j]));
+    }
 }
 
 // QCS::setVelDiffTask, src/QCS.cc:384-448: ztmp = max(0, max over the
@@ -952,34 +1154,55 @@ __global__ void __launch_bounds__(PN_TB) k_set_vel_diff(pennant_mesh m, int tz, 
                                                         const double* zss, pennant_cvec px,
                                                         pennant_cvec pu, double* ztmp, double* zdu,
                                                         double q1, double q2) {
-    PN_TILE_SHARED;
-    __shared__ d2 x1[PN_TB], u1[PN_TB];
-    __shared__ double dux_[PN_TB];
-    SideAt c;
-    if (has_side) {
-        c = side_at(t, zsb, zof, lane);
-        const int p1 = m.mapsp1[t.s0 + lane];
-        x1[lane] = VIO<L>::ld(px, p1);
-        u1[lane] = VIO<L>::ld(pu, p1);
+    constexpr int R = 2;
+    PN_WIDE_SHARED(R);
+    __shared__ d2 x1[R * PN_TB], u1[R * PN_TB];
+    __shared__ double dux_[R * PN_TB];
+    int p[R];
+    double l[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        p[j] = m.mapsp1[SIDEC(j)];
+        l[j] = elen[SIDEC(j)];
+    }
+    const double zs = has_zone ? zss[t.z0 + lane] : 0.;
+    d2 x[R], u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        x[j] = VIO<L>::ld(px, p[j]);
+        u[j] = VIO<L>::ld(pu, p[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            x1[i] = x[j];
+            u1[i] = u[j];
+        }
     }
     __syncthreads();
-    if (has_side) {
-        const int i2 = side_next(c);
-        const d2 dx = sub(x1[i2], x1[lane]);
-        const d2 du = sub(u1[i2], u1[lane]);
-        const double lenx = elen[t.s0 + lane];
-        double dux = dot(du, dx);
-        dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
-        dux_[lane] = dux;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            const int i2 = wide_next<R>(c);
+            const d2 dx = sub(x1[i2], x[j]);
+            const d2 du = sub(u1[i2], u[j]);
+            const double lenx = l[j];
+            double dux = dot(du, dx);
+            dux = (lenx > 0. ? fabs(dux) / lenx : 0.);
+            dux_[i] = dux;
+        }
     }
     __syncthreads();
     if (has_zone) {
-        const int b = zsb[lane] - t.s0, e = zone_end(t, zsb, lane);
+        const int b = zsb[lane] - t.s0, e = wide_zone_end<R>(t, zsb, lane);
         double tm = 0.;
         for (int i = b; i < e; ++i) tm = smax(tm, dux_[i]);
         const int z = t.z0 + lane;
         ztmp[z] = tm;
-        zdu[z] = q1 * zss[z] + 2. * q2 * tm;
+        zdu[z] = q1 * zs + 2. * q2 * tm;
     }
 }
 
@@ -1025,6 +1248,13 @@ inline int tile_zones(const pennant_mesh* m) {
     return PN_TB / m->maxznump;
 }
 inline int tile_blocks(const pennant_mesh* m, int tz) { return (m->numz + tz - 1) / tz; }
+// zones of a wide tile of r * PN_TB sides: as many whole zones as it holds,
+// at most one per thread
+inline int wide_tile_zones(const pennant_mesh* m, int r) {
+    if (m->maxznump < 3 || m->maxznump > PENNANT_MAX_ZSIDES) return 0;
+    const int tz = r * PN_TB / m->maxznump;
+    return tz < PN_TB ? tz : PN_TB;
+}
 
 #define PN_LAUNCH_L(lay, kern, grid, block, st, ...)                                               \
     do {                                                                                           \
@@ -1037,6 +1267,13 @@ inline int tile_blocks(const pennant_mesh* m, int tz) { return (m->numz + tz - 1
 
 // Common preamble of a tile-kernel entry point: nothing to do for an empty
 // mesh, EINVAL for a descriptor that cannot be tiled.
+#define PN_WIDE_ENTRY(m, tz, R)                       \
+    if (!(m)) return PENNANT_EINVAL;                  \
+    if ((m)->numz <= 0) return launch_status();       \
+    const int tz = wide_tile_zones(m, R);             \
+    if (tz == 0) return PENNANT_EINVAL;               \
+    const dim3 tgrid(tile_blocks(m, tz)), tblk(PN_TB)
+
 #define PN_TILE_ENTRY(m, tz)                          \
     if (!(m)) return PENNANT_EINVAL;                  \
     if ((m)->numz <= 0) return launch_status();       \
@@ -1057,7 +1294,7 @@ size_t pennant_hip_reduce_workspace_bytes(void) {
 
 int pennant_hip_calc_ctrs(const pennant_mesh* m, pennant_cvec px, pennant_vec ex, pennant_vec zx,
                           hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 4);
     Layout l;
     px = l.take(px); ex = l.take(ex); zx = l.take(zx);
     if (l.bad) return PENNANT_EINVAL;
@@ -1071,7 +1308,7 @@ int pennant_hip_calc_vols(const pennant_mesh* m, pennant_cvec px, pennant_cvec z
     (void)ws;
     if (!m || !numsbad) return PENNANT_EINVAL;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, numsbad);
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 4);
     Layout l;
     px = l.take(px); zx = l.take(zx);
     if (l.bad) return PENNANT_EINVAL;
@@ -1089,7 +1326,7 @@ int pennant_hip_calc_side_fracs(const pennant_mesh* m, const double* sarea, cons
 
 int pennant_hip_calc_surf_vecs(const pennant_mesh* m, pennant_cvec ex, pennant_cvec zx,
                                pennant_vec ssurf, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 4);
     Layout l;
     ex = l.take(ex); zx = l.take(zx); ssurf = l.take(ssurf);
     if (l.bad) return PENNANT_EINVAL;
@@ -1099,17 +1336,20 @@ int pennant_hip_calc_surf_vecs(const pennant_mesh* m, pennant_cvec ex, pennant_c
 
 int pennant_hip_calc_edge_len(const pennant_mesh* m, pennant_cvec px, double* elen,
                               hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    if (!m || !m->mapsp1 || !m->mapsp2) return PENNANT_EINVAL;
+    if (m->nums <= 0) return launch_status();
     Layout l;
     px = l.take(px);
     if (l.bad) return PENNANT_EINVAL;
-    PN_LAUNCH_L(l.kind(), k_calc_edge_len, tgrid, tblk, st, *m, tz, px, elen);
+    const dim3 grid((unsigned)((m->nums + PN_BLOCK * PN_EL - 1) / (PN_BLOCK * PN_EL)));
+    PN_LAUNCH_L(l.kind(), k_calc_edge_len, grid, dim3(PN_BLOCK), st, m->nums, m->mapsp1, m->mapsp2, px,
+                elen);
     return launch_status();
 }
 
 int pennant_hip_calc_char_len(const pennant_mesh* m, const double* sarea, const double* elen,
                               double* zdl, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     hipLaunchKernelGGL(k_calc_char_len, tgrid, tblk, 0, st, *m, tz, sarea, elen, zdl);
     return launch_status();
 }
@@ -1184,7 +1424,7 @@ int pennant_hip_adv_pos_full(const pennant_mesh* m, pennant_cvec px0, pennant_cv
 int pennant_hip_calc_work(const pennant_mesh* m, pennant_cvec sfp, pennant_cvec sfq,
                           pennant_cvec pu, pennant_cvec pu0, pennant_cvec pxp, double* zw,
                           double* zetot, const double* dt, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     Layout l;
     sfp = l.take(sfp); sfq = l.take(sfq); pu = l.take(pu); pu0 = l.take(pu0); pxp = l.take(pxp);
     if (l.bad) return PENNANT_EINVAL;
@@ -1261,7 +1501,7 @@ int pennant_hip_calc_state_half(const pennant_mesh* m, const double* zr, const d
 
 int pennant_hip_calc_force_pgas(const pennant_mesh* m, pennant_cvec ssurf, const double* zp,
                                 pennant_vec sf, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 4);
     Layout l;
     ssurf = l.take(ssurf); sf = l.take(sf);
     if (l.bad) return PENNANT_EINVAL;
@@ -1273,7 +1513,7 @@ int pennant_hip_calc_force_tts(const pennant_mesh* m, const double* sarea, const
                                pennant_cvec ssurf, const double* zarea, const double* zr,
                                const double* zss, pennant_vec sf, double alfa, double ssmin,
                                hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     Layout l;
     ssurf = l.take(ssurf); sf = l.take(sf);
     if (l.bad) return PENNANT_EINVAL;
@@ -1286,7 +1526,7 @@ int pennant_hip_set_corner_div(const pennant_mesh* m, pennant_cvec ex, const dou
                                pennant_cvec zx, pennant_cvec px, pennant_cvec pu,
                                pennant_vec zuc, double* carea, double* ccos, double* cdiv,
                                double* cevol, double* cdu, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     Layout l;
     ex = l.take(ex); zx = l.take(zx); px = l.take(px); pu = l.take(pu); zuc = l.take(zuc);
     if (l.bad) return PENNANT_EINVAL;
@@ -1300,7 +1540,7 @@ int pennant_hip_set_qcn_force(const pennant_mesh* m, const double* elen, const d
                               const double* zss, pennant_cvec pu, double* crmu,
                               pennant_vec cqe1, pennant_vec cqe2, double qgamma, double q1,
                               double q2, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     Layout l;
     pu = l.take(pu); cqe1 = l.take(cqe1); cqe2 = l.take(cqe2);
     if (l.bad) return PENNANT_EINVAL;
@@ -1312,7 +1552,7 @@ int pennant_hip_set_qcn_force(const pennant_mesh* m, const double* elen, const d
 int pennant_hip_set_force_qcs(const pennant_mesh* m, const double* carea, pennant_cvec cqe1,
                               pennant_cvec cqe2, const double* elen, double* ccos, double* cw,
                               pennant_vec sfq, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 1);
     Layout l;
     cqe1 = l.take(cqe1); cqe2 = l.take(cqe2); sfq = l.take(sfq);
     if (l.bad) return PENNANT_EINVAL;
@@ -1324,7 +1564,7 @@ int pennant_hip_set_force_qcs(const pennant_mesh* m, const double* carea, pennan
 int pennant_hip_set_vel_diff(const pennant_mesh* m, const double* elen, const double* zss,
                              pennant_cvec px, pennant_cvec pu, double* ztmp, double* zdu,
                              double q1, double q2, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 2);
     Layout l;
     px = l.take(px); pu = l.take(pu);
     if (l.bad) return PENNANT_EINVAL;
