@@ -24,6 +24,11 @@ Prints ONE JSON line on stdout:
               errors, shock radius at tstop; the 1% density gate
   energy      total energy (internal + kinetic) at t = 0 and at tstop, and
               its relative drift, per form
+  drop_in_tasks  the task-by-task FP64 cycle (one kernel launch per
+              reference task ID, through the C ABI's per-task entry points) on
+              the same workload from t = 0: us per cycle and the reference
+              decomposition's bytes per cycle (688 S + 536 Z + 512 P) over that
+              time, against 8 TB/s
   cpu_baseline  the CPU oracle (a serial C++ restatement of the reference's
               CPU task bodies, and its OpenMP variant) on a bounded sample of
               the same workload, on this host (lscpu model, sockets, cores)
@@ -293,6 +298,8 @@ def main():
     ap.add_argument("--closure", default=None,
                     help="trained closure (.npz from pennant_amd.train_closure); default: the "
                          "package's closure_ensemble.npz")
+    ap.add_argument("--tasks-steps", type=int, default=20,
+                    help="cycles of the task-by-task FP64 form timed for drop_in_tasks (0: skip)")
     ap.add_argument("--copy-mb", type=int, default=512,
                     help="size of the streaming-copy calibration (0 disables)")
     ap.add_argument("--k1", type=int, default=None, help="fused K1 layout: 0 zone/thread, 1 corner/lane")
@@ -526,6 +533,26 @@ def main():
     else:
         energy = None
 
+    # the drop-in path: one launch per reference task ID, from t = 0 (its
+    # kernels move the same bytes at any time of the run)
+    tasks = None
+    if args.tasks_steps > 0 and mode != pa.MODE_TASKS:
+        with pa.Problem(text) as pt:
+            pt.run(2, pa.MODE_TASKS)
+            sct0 = pt.scalars()
+            tms, _ = pt.time_cycles(args.tasks_steps, pa.MODE_TASKS, phases=False)
+            sct1 = pt.scalars()
+        us = tms / args.tasks_steps * 1e3
+        tb = 688 * S + 536 * Z + 512 * P
+        gbs = tb / (us * 1e-6) / 1e9
+        log(f"drop-in tasks fp64: {us:.1f} us/cycle, {tb / 1e9:.3f} GB of task bytes per cycle, "
+            f"{gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.3f} of peak)")
+        tasks = {"form": "plain-fp64-tasks", "us_per_cycle": round(us, 2),
+                 "zone_cycles_per_s": round(Z / (us * 1e-6), 1),
+                 "task_bytes_per_cycle": tb, "effective_GBs": round(gbs, 1),
+                 "peak_GBs": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4),
+                 "t_window": [round(sct0["time"], 9), round(sct1["time"], 9)]}
+
     cpu = None
     if args.cpu_budget > 0:
         res, threads = cpu_baseline(text, Z, args.cpu_budget)
@@ -553,6 +580,8 @@ def main():
     }
     if roofline_fp64 is not None:
         line["roofline_fp64"] = roofline_fp64
+    if tasks is not None:
+        line["drop_in_tasks"] = tasks
     if accuracy is not None:
         line["accuracy"] = accuracy
     if energy is not None:
