@@ -5,11 +5,13 @@
 //
 //   * Zone tiles.  A task that couples a side to its zone or to the
 //     neighbouring sides of its zone runs one block per tile of whole zones
-//     (PN_TB / maxznump zones, so a tile has at most PN_TB sides): one lane
-//     per side reads and writes the side arrays fully coalesced, gathers its
-//     point values once into LDS, and finds the previous / next side of its
-//     zone and the zone's values there (the reference reads mapss3/mapss4
-//     and gathers zone values per side).  Zone sums, minima and maxima are
+//     (at most PN_TB zones and R * PN_TB sides, R = 1, 2 or 4 per task): each
+//     thread takes R sides PN_TB apart and issues the loads of all of them
+//     before it uses any, the side arrays are read and written fully
+//     coalesced, point values are gathered once into LDS, and a side finds
+//     the previous / next side of its zone and the zone's values there (the
+//     reference reads mapss3/mapss4 and gathers zone values per side).
+//     Zone sums, minima and maxima are
 //     formed by one lane per zone reading the staged side values in
 //     ascending side order from the same starting value as the CPU body, so
 //     they equal the serial result bit for bit (the reference's GPU variants
@@ -41,7 +43,7 @@
 
 using namespace pn;
 
-#define PN_TB 256          // threads of a tile block; a tile holds at most PN_TB sides
+#define PN_TB 256          // threads of a tile block; a tile holds at most PN_TB zones
 #define PN_PLIGHT 4        // corners a point-gather lane sums by itself
 
 namespace {
@@ -86,25 +88,6 @@ struct Tile {
     int s0, ns;   // their sides s0 .. s0 + ns - 1
 };
 
-// Reads the tile's zone offsets into zsb (nz + 1 entries, relative to the
-// mesh) and marks each side with its zone's index in the tile (zof).
-__device__ __forceinline__ Tile tile_setup(const int32_t* zsbeg, int numz, int tz, int* zsb,
-                                           unsigned char* zof) {
-    Tile t;
-    t.z0 = blockIdx.x * tz;
-    t.nz = min(tz, numz - t.z0);
-    if ((int)threadIdx.x <= t.nz) zsb[threadIdx.x] = zsbeg[t.z0 + threadIdx.x];
-    __syncthreads();
-    t.s0 = zsb[0];
-    t.ns = min(zsb[t.nz] - t.s0, PN_TB);
-    if ((int)threadIdx.x < t.nz) {
-        const int b = zsb[threadIdx.x] - t.s0;
-        const int e = min(zsb[threadIdx.x + 1] - t.s0, PN_TB);
-        for (int i = b; i < e; ++i) zof[i] = (unsigned char)threadIdx.x;
-    }
-    __syncthreads();
-    return t;
-}
 
 // A side's place in its zone: tile-local index i of the side, zone index zi
 // in the tile, the zone's first side b (tile-local), its side count n and
@@ -122,26 +105,6 @@ __device__ __forceinline__ SideAt side_at(const Tile& t, const int* zsb, const u
     c.k = i - c.b;
     return c;
 }
-// previous and next side of the zone (mapss3, mapss4), tile-local
-__device__ __forceinline__ int side_prev(const SideAt& c) {
-    return min(c.b + (c.k == 0 ? c.n - 1 : c.k - 1), PN_TB - 1);
-}
-__device__ __forceinline__ int side_next(const SideAt& c) {
-    return min(c.b + (c.k + 1 == c.n ? 0 : c.k + 1), PN_TB - 1);
-}
-// the side range of tile zone `zi`, tile-local
-__device__ __forceinline__ int zone_end(const Tile& t, const int* zsb, int zi) {
-    return min(zsb[zi + 1] - t.s0, PN_TB);
-}
-
-// Shared memory and indices every tile kernel declares.
-#define PN_TILE_SHARED                                        \
-    __shared__ int zsb[PN_TB + 1];                            \
-    __shared__ unsigned char zof[PN_TB];                      \
-    const Tile t = tile_setup(m.zsbeg, m.numz, tz, zsb, zof); \
-    const int lane = threadIdx.x;                             \
-    const bool has_side = lane < t.ns;                        \
-    [[maybe_unused]] const bool has_zone = lane < t.nz
 
 // Wide tiles: up to R * PN_TB sides of at most PN_TB whole zones; thread t
 // takes the tile's sides t, t + PN_TB, ..., t + (R - 1) PN_TB and issues the
@@ -290,13 +253,21 @@ __global__ void k_zero_int(int32_t* out) { *out = 0; }
 __global__ void __launch_bounds__(PN_TB) k_calc_side_fracs(pennant_mesh m, int tz,
                                                            const double* sarea,
                                                            const double* zarea, double* smf) {
-    PN_TILE_SHARED;
+    constexpr int R = 4;
+    PN_WIDE_SHARED(R);
     __shared__ double za[PN_TB];
     if (has_zone) za[lane] = zarea[t.z0 + lane];
+    double sa[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) sa[j] = sarea[SIDEC(j)];
     __syncthreads();
-    if (has_side) {
-        const SideAt c = side_at(t, zsb, zof, lane);
-        smf[t.s0 + lane] = sarea[t.s0 + lane] / za[c.zi];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int i = SIDE(j);
+        if (i < t.ns) {
+            const SideAt c = side_at(t, zsb, zof, i);
+            smf[t.s0 + i] = sa[j] / za[c.zi];
+        }
     }
 }
 
@@ -1243,10 +1214,6 @@ struct Layout {
 
 // zones per tile block for the mesh; 0 when the mesh descriptor cannot be
 // tiled (zones of fewer than 3 or more than PENNANT_MAX_ZSIDES sides)
-inline int tile_zones(const pennant_mesh* m) {
-    if (m->maxznump < 3 || m->maxznump > PENNANT_MAX_ZSIDES) return 0;
-    return PN_TB / m->maxznump;
-}
 inline int tile_blocks(const pennant_mesh* m, int tz) { return (m->numz + tz - 1) / tz; }
 // zones of a wide tile of r * PN_TB sides: as many whole zones as it holds,
 // at most one per thread
@@ -1271,13 +1238,6 @@ inline int wide_tile_zones(const pennant_mesh* m, int r) {
     if (!(m)) return PENNANT_EINVAL;                  \
     if ((m)->numz <= 0) return launch_status();       \
     const int tz = wide_tile_zones(m, R);             \
-    if (tz == 0) return PENNANT_EINVAL;               \
-    const dim3 tgrid(tile_blocks(m, tz)), tblk(PN_TB)
-
-#define PN_TILE_ENTRY(m, tz)                          \
-    if (!(m)) return PENNANT_EINVAL;                  \
-    if ((m)->numz <= 0) return launch_status();       \
-    const int tz = tile_zones(m);                     \
     if (tz == 0) return PENNANT_EINVAL;               \
     const dim3 tgrid(tile_blocks(m, tz)), tblk(PN_TB)
 
@@ -1319,7 +1279,7 @@ int pennant_hip_calc_vols(const pennant_mesh* m, pennant_cvec px, pennant_cvec z
 
 int pennant_hip_calc_side_fracs(const pennant_mesh* m, const double* sarea, const double* zarea,
                                 double* smf, hipStream_t st) {
-    PN_TILE_ENTRY(m, tz);
+    PN_WIDE_ENTRY(m, tz, 4);
     hipLaunchKernelGGL(k_calc_side_fracs, tgrid, tblk, 0, st, *m, tz, sarea, zarea, smf);
     return launch_status();
 }
