@@ -104,16 +104,47 @@ __device__ __forceinline__ bf16x8 relu_chunk(const f32x16& a, int base) {
     return as_bf16x8(v);
 }
 
+// Where a lane reads its weight fragments: straight from the packed weights
+// (global memory or an LDS copy), or, for the hidden layer, from registers
+// loaded once (MlpRegs), for a wave that evaluates the network several times
+// in a row.  Both give the same operands, so the same results.
+struct MlpPackSrc {
+    const ClosurePack* __restrict__ w;
+    int lane;
+    __device__ __forceinline__ bf16x8 w1(int mb) const { return w->w1[mb][lane]; }
+    __device__ __forceinline__ bf16x8 w2(int mb, int c) const { return w->w2[mb][c][lane]; }
+    __device__ __forceinline__ bf16x8 w3(int c) const { return w->w3[c][lane]; }
+    __device__ __forceinline__ float b2(int mb, int r) const { return w->b2c[mb][lane >> 5][r]; }
+    __device__ __forceinline__ float b3(int o) const { return w->b3[o]; }
+};
+struct MlpRegs {
+    const ClosurePack* __restrict__ w;
+    int lane;
+    bf16x8 w2_[2][4];     // the hidden layer's fragments (8 of the 14 MFMAs)
+    __device__ __forceinline__ bf16x8 w1(int mb) const { return w->w1[mb][lane]; }
+    __device__ __forceinline__ bf16x8 w2(int mb, int c) const { return w2_[mb][c]; }
+    __device__ __forceinline__ bf16x8 w3(int c) const { return w->w3[c][lane]; }
+    __device__ __forceinline__ float b2(int mb, int r) const { return w->b2c[mb][lane >> 5][r]; }
+    __device__ __forceinline__ float b3(int o) const { return w->b3[o]; }
+};
+__device__ __forceinline__ void mlp_load_regs(const ClosurePack* __restrict__ w, int lane, MlpRegs& r) {
+    r.w = w;
+    r.lane = lane;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r.w2_[mb][c] = w->w2[mb][c][lane];
+}
+
 // One group of 32 corners (the B operand columns): returns y0, y1 in lanes
 // 0-31 (column lane), unspecified in lanes 32-63.  Wave-uniform control flow.
-__device__ __forceinline__ void mlp_group(const ClosurePack* __restrict__ w, bf16x8 x, int lane,
-                                          float& y0, float& y1) {
-    const int h = lane >> 5;
+template <class W>
+__device__ __forceinline__ void mlp_group(const W& w, bf16x8 x, float& y0, float& y1) {
     f32x16 a1[2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
         f32x16 z = {};
-        a1[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->w1[mb][lane], x, z, 0, 0, 0);
+        a1[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w1(mb), x, z, 0, 0, 0);
     }
     bf16x8 h1[4];
 #pragma unroll
@@ -122,10 +153,10 @@ __device__ __forceinline__ void mlp_group(const ClosurePack* __restrict__ w, bf1
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) a2[mb][r] = w->b2c[mb][h][r];
+        for (int r = 0; r < 16; ++r) a2[mb][r] = w.b2(mb, r);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            a2[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->w2[mb][c][lane], h1[c], a2[mb], 0, 0, 0);
+            a2[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w2(mb, c), h1[c], a2[mb], 0, 0, 0);
     }
     bf16x8 h2[4];
 #pragma unroll
@@ -133,18 +164,72 @@ __device__ __forceinline__ void mlp_group(const ClosurePack* __restrict__ w, bf1
     f32x16 a3 = {};
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-        a3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->w3[c][lane], h2[c], a3, 0, 0, 0);
-    y0 = a3[0] + w->b3[0];
-    y1 = a3[1] + w->b3[1];
+        a3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w.w3(c), h2[c], a3, 0, 0, 0);
+    y0 = a3[0] + w.b3(0);
+    y1 = a3[1] + w.b3(1);
+}
+
+// Both groups of a wave in one instruction stream, so that the MFMAs of one
+// group issue while the other's wait on their inputs (the same operations
+// as two mlp_group calls, so the same results).
+template <class W>
+__device__ __forceinline__ void mlp_group2(const W& w, bf16x8 xa, bf16x8 xb, float& a0, float& a1,
+                                           float& b0, float& b1) {
+    f32x16 pa[2], pb[2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        f32x16 z = {};
+        const bf16x8 w1 = w.w1(mb);
+        pa[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xa, z, 0, 0, 0);
+        pb[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, xb, z, 0, 0, 0);
+    }
+    bf16x8 ha[4], hb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        ha[c] = relu_chunk(pa[c >> 1], 8 * (c & 1));
+        hb[c] = relu_chunk(pb[c >> 1], 8 * (c & 1));
+    }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            pa[mb][r] = w.b2(mb, r);
+            pb[mb][r] = pa[mb][r];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bf16x8 w2 = w.w2(mb, c);
+            pa[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, ha[c], pa[mb], 0, 0, 0);
+            pb[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2, hb[c], pb[mb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        ha[c] = relu_chunk(pa[c >> 1], 8 * (c & 1));
+        hb[c] = relu_chunk(pb[c >> 1], 8 * (c & 1));
+    }
+    f32x16 qa = {}, qb = {};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const bf16x8 w3 = w.w3(c);
+        qa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3, ha[c], qa, 0, 0, 0);
+        qb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3, hb[c], qb, 0, 0, 0);
+    }
+    a0 = qa[0] + w.b3(0);
+    a1 = qa[1] + w.b3(1);
+    b0 = qb[0] + w.b3(0);
+    b1 = qb[1] + w.b3(1);
 }
 
 // Evaluates the network on one corner per lane (64 corners).  lo/hi are the
 // lane's packed input slots 0-7 and 8-15.  needA / needB (wave-uniform) say
 // whether any lane of the low / high half needs a result; a half that needs
 // none skips its MFMA group.  Each lane receives its own corner's outputs.
-__device__ __forceinline__ void mlp_wave(const ClosurePack* __restrict__ w, u32x4 lo, u32x4 hi,
-                                         bool needA, bool needB, float& y0, float& y1) {
-    const int lane = threadIdx.x & 63;
+// With PAIR, a wave that needs both groups runs them as one stream
+// (mlp_group2).
+template <class W, bool PAIR = false>
+__device__ __forceinline__ void mlp_wave_w(const W& w, u32x4 lo, u32x4 hi, bool needA, bool needB,
+                                           float& y0, float& y1) {
     // group A: columns are the corners of lanes 0-31 (slots 0-7 from the low
     // lanes, slots 8-15 from the high lanes); group B: the corners of lanes
     // 32-63.  One swap of the upper half of `lo` with the lower half of `hi`
@@ -157,8 +242,12 @@ __device__ __forceinline__ void mlp_wave(const ClosurePack* __restrict__ w, u32x
         opB[i] = r[1];
     }
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    if (needA) mlp_group(w, as_bf16x8(opA), lane, a0, a1);
-    if (needB) mlp_group(w, as_bf16x8(opB), lane, b0, b1);
+    if (PAIR && needA && needB) {
+        mlp_group2(w, as_bf16x8(opA), as_bf16x8(opB), a0, a1, b0, b1);
+    } else {
+        if (needA) mlp_group(w, as_bf16x8(opA), a0, a1);
+        if (needB) mlp_group(w, as_bf16x8(opB), b0, b1);
+    }
     // lanes 0-31 keep group A's results; lanes 32-63 take group B's results
     // from lanes 0-31
     const auto s0 = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a0),
@@ -167,6 +256,10 @@ __device__ __forceinline__ void mlp_wave(const ClosurePack* __restrict__ w, u32x
                                                      __builtin_bit_cast(unsigned, b1), false, false);
     y0 = __builtin_bit_cast(float, (unsigned)s0[0]);
     y1 = __builtin_bit_cast(float, (unsigned)s1[0]);
+}
+__device__ __forceinline__ void mlp_wave(const ClosurePack* __restrict__ w, u32x4 lo, u32x4 hi,
+                                         bool needA, bool needB, float& y0, float& y1) {
+    mlp_wave_w(MlpPackSrc{w, (int)(threadIdx.x & 63)}, lo, hi, needA, needB, y0, y1);
 }
 
 // Packs 10 features into the two input halves (slot 10 = 1 for the bias).

@@ -969,6 +969,10 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
         // the compiler from moving them across
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        // the lane's weight fragments, read once for all of the wave's
+        // evaluations
+        MlpRegs wr;
+        if (total > 0) mlp_load_regs(a.pack, lane, wr);
         for (int e = 0; e < total; e += 64) {
             const int sl = e + lane;
             const bool act = sl < total;
@@ -981,7 +985,7 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
             hi[2] = 0u;
             hi[3] = 0u;
             float y0, y1;
-            mlp_wave(a.pack, lo, hi, (uint32_t)am != 0u, (am >> 32) != 0u, y0, y1);
+            mlp_wave_w<MlpRegs, true>(wr, lo, hi, (uint32_t)am != 0u, (am >> 32) != 0u, y0, y1);
             if (act) {
                 xw[sl][0] = __float_as_uint(y0);
                 xw[sl][1] = __float_as_uint(y1);
