@@ -109,7 +109,8 @@ int pennant_hip_calc_side_fracs(const pennant_mesh* m, const double* sarea,
 int pennant_hip_calc_surf_vecs(const pennant_mesh* m, pennant_cvec ex, pennant_cvec zx,
                                pennant_vec ssurf, hipStream_t stream);
 
-/* TID_CALCEDGELEN: elen = |px2 - px1|.
+/* TID_CALCEDGELEN: elen = |px[mapsp2] - px[mapsp1]| per side (reads mapsp1
+ * and mapsp2; EINVAL if either is NULL).
  * Replaces src/Mesh.cu:212-264 (src/Mesh.cc:1151-1179). */
 int pennant_hip_calc_edge_len(const pennant_mesh* m, pennant_cvec px, double* elen,
                               hipStream_t stream);
