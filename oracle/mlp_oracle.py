@@ -2,7 +2,7 @@
 
 A numpy restatement of the closure network (forward, loss, backward) and of
 its per-corner features and training targets.  Only tests/ may import it, as
-the checker of the GPU kernels; the product never does.
+the reference the GPU kernels are compared with; the product never does.
 
 Network (csrc/closure.h): 10 features -> 64 -> 64 -> 2, relu, BF16 weights
 and activations, FP32 accumulation.  With `rounding=True` the oracle rounds

@@ -2,8 +2,9 @@
 //
 // A serial CPU restatement of pennant-legion's LOC_PROC task bodies, run as
 // one piece (`-n 1`).  Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load this library, and only as the checker (or as the
-// timed CPU baseline); the product path never links or calls it.
+// cpu_baseline leg may load this library, and only as the reference the GPU
+// results are compared with (or as the timed CPU baseline); the product path
+// never links or calls it.
 //
 // Build: `make -C oracle` (g++ -O2 -ffp-contract=off, no -march=native), which
 // produces oracle/liboracle_pennant.so.  Every routine cites the reference

@@ -17,8 +17,8 @@
 // side order; the corners at ghost points are sent to the owner's list by the
 // corner exchange, and the owners' new point state goes back to the ghosts by
 // the halo (pieces_cycle in fused.hip).  The reference leaves the fold order of
-// shared-point reductions to Legion; this build fixes it to the one-piece
-// order, so a run in pieces is bit-identical to the one-piece run.
+// shared-point reductions to Legion; here it is the one-piece order, so a
+// run in pieces is bit-identical to the one-piece run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -1,6 +1,6 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle_pennant.so).
 
-Test infrastructure only: the oracle is the checker the HIP path is compared
+Test infrastructure only: the oracle is what the HIP path is compared
 against, never part of the product.
 """
 from __future__ import annotations
