@@ -1019,7 +1019,8 @@ __device__ __forceinline__ double zone_pred_mixed_core(const FusedArgs& a, int z
                 hi[2] = 0u;
                 hi[3] = 0u;
                 float y0, y1;
-                mlp_wave(a.pack, flo[k], hi, (uint32_t)m != 0u, (m >> 32) != 0u, y0, y1);
+                mlp_wave_w<MlpPackSrc, true>(MlpPackSrc{a.pack, (int)(threadIdx.x & 63)}, flo[k], hi,
+                                             (uint32_t)m != 0u, (m >> 32) != 0u, y0, y1);
                 if (need) crmu[k] = (q1f * y0 * zssf + gq * y1 * cdu[k]) * zrpf * cevol[k];
             }
         }
@@ -2341,11 +2342,21 @@ template <int FORM, int CL>
 __global__ void __launch_bounds__(PN_BLOCK)
 __attribute__((amdgpu_waves_per_eu((CL == CL_LEARNED) ? 2 : ((FORM == FORM_MIXED && CL == CL_EXACT) ? 4 : 1))))
 k_pred_batch(BatchArgs B) {
+    // the learned form reads the network's weights from an LDS copy of the
+    // block's own
+    constexpr int NQ = (CL == CL_LEARNED) ? (int)(sizeof(ClosurePack) / 16) : 1;
+    __shared__ uint4 wq[NQ];
+    if constexpr (CL == CL_LEARNED) {
+        const uint4* src = reinterpret_cast<const uint4*>(B.a.pack);
+        for (int i = threadIdx.x; i < NQ; i += PN_BLOCK) wq[i] = src[i];
+        __syncthreads();
+    }
     for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
         const BUnit U = B.units[w.u];
         Ctl* c = B.ctls + U.pid;
         if (c->done) continue;
         FusedArgs a = B.a;
+        if constexpr (CL == CL_LEARNED) a.pack = reinterpret_cast<const ClosurePack*>(wq);
         apply_prob(a, B.probs[U.pid]);
         a.ctl = c;                 // the problem's zr/ze flag for the predictor
         const double dt = c->dt;
