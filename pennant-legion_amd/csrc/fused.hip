@@ -2351,15 +2351,32 @@ k_pred_batch(BatchArgs B) {
         for (int i = threadIdx.x; i < NQ; i += PN_BLOCK) wq[i] = src[i];
         __syncthreads();
     }
-    for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
-        const BUnit U = B.units[w.u];
+    // a unit's header (its work unit and its problem's done flag and dt) is
+    // loaded one unit ahead, so a unit's zone loads do not wait behind it
+    Units w = block_units(B.nunits, B.a.xcd);
+    BUnit Un;
+    int dn = 1;
+    double dtn = 0.;
+    if (w.u < w.end) {
+        Un = B.units[w.u];
+        dn = B.ctls[Un.pid].done;
+        dtn = B.ctls[Un.pid].dt;
+    }
+    for (; w.u < w.end; w.u += w.step) {
+        const BUnit U = Un;
+        const int done = dn;
+        const double dt = dtn;
+        if (w.u + w.step < w.end) {
+            Un = B.units[w.u + w.step];
+            dn = B.ctls[Un.pid].done;
+            dtn = B.ctls[Un.pid].dt;
+        }
+        if (done) continue;
         Ctl* c = B.ctls + U.pid;
-        if (c->done) continue;
         FusedArgs a = B.a;
         if constexpr (CL == CL_LEARNED) a.pack = reinterpret_cast<const ClosurePack*>(wq);
         apply_prob(a, B.probs[U.pid]);
         a.ctl = c;                 // the problem's zr/ze flag for the predictor
-        const double dt = c->dt;
         double dtmin = 1.e99;
         int bad = 0;
         ClCount cnt;
@@ -2404,11 +2421,27 @@ __global__ void __launch_bounds__(PN_BLOCK) k_point_batch(BatchArgs B) {
 
 template <typename T>
 __global__ void __launch_bounds__(PN_BLOCK) k_corr_batch(BatchArgs B) {
-    for (Units w = block_units(B.nunits, B.a.xcd); w.u < w.end; w.u += w.step) {
-        const BUnit U = B.units[w.u];
-        const Ctl* c = B.ctls + U.pid;
-        if (c->done) continue;
-        const double dt = c->dt;
+    // a unit's header (its work unit and its problem's done flag and dt) is
+    // loaded one unit ahead, so a unit's zone loads do not wait behind it
+    Units w = block_units(B.nunits, B.a.xcd);
+    BUnit Un;
+    int dn = 1;
+    double dtn = 0.;
+    if (w.u < w.end) {
+        Un = B.units[w.u];
+        dn = B.ctls[Un.pid].done;
+        dtn = B.ctls[Un.pid].dt;
+    }
+    for (; w.u < w.end; w.u += w.step) {
+        const BUnit U = Un;
+        const int done = dn;
+        const double dt = dtn;
+        if (w.u + w.step < w.end) {
+            Un = B.units[w.u + w.step];
+            dn = B.ctls[Un.pid].done;
+            dtn = B.ctls[Un.pid].dt;
+        }
+        if (done) continue;
         double dvov = 1.e-99;
         int bad = 0;
         const int z = U.zb + (int)threadIdx.x;
