@@ -576,7 +576,9 @@ int pennant_batch_closure_stats(pennant_batch* b, int64_t* out2);
 /* Training-tuple capture over all problems (see pennant_set_capture). */
 int pennant_batch_set_capture(pennant_batch* b, float* dev_buf, int64_t cap);
 int pennant_batch_capture_count(pennant_batch* b, int64_t* count);
-/* "max_blocks" (1..65535, default 2048), "xcd" (0/1), "graph" (0/1). */
+/* "max_blocks" (1..65535, default 2048), "xcd" (0/1), "graph" (0/1),
+ * "rect_maps" (0/1, default 1: the zone kernels compute the points and corner
+ * slots of the batch's rect problems, as pennant_set_tuning's "rect_maps"). */
 int pennant_batch_set_tuning(pennant_batch* b, const char* key, int value);
 int pennant_batch_get_tuning(pennant_batch* b, const char* key);
 /* Times `cycles` cycles at each grid shape (grid cap x XCD mapping) from the

@@ -213,6 +213,12 @@ pennant_batch* pennant_batch_create(const char* const* decks, int n, char* errbu
         q.q1 = pr.q1; q.q2 = pr.q2; q.cfl = pr.cfl; q.cflv = pr.cflv; q.dtmax = pr.dtmax;
         q.dtfac = pr.dtfac; q.tstop = pr.tstop; q.cstop = pr.cstop; q.nbcs = pb->nbcs;
         for (int k = 0; k < 8; ++k) { q.bcvx[k] = pb->bcvx[k]; q.bcvy[k] = pb->bcvy[k]; }
+        // the single-problem set-up's rect detection (every side checked)
+        q.rect_nzx = pb->rect_nzx;
+        q.rect_m = pb->rect_m;
+        q.rect_sh = pb->rect_sh;
+        q.so = (int)so;
+        q.po = (int)po;
         q.ubeg = (int)units.size();
         for (size_t r = 0; r < pb->zruns.size(); r += 3) {
             const int zb = pb->zruns[r], ze = pb->zruns[r + 1], nsd = pb->zruns[r + 2];
@@ -423,6 +429,9 @@ int pennant_batch_set_tuning(pennant_batch* b, const char* key, int value) {
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return PENNANT_EINVAL;
         b->tune.graph = value;
+    } else if (!strcmp(key, "rect_maps")) {
+        if (value != 0 && value != 1) return PENNANT_EINVAL;
+        b->tune.rect_maps = value;
     } else {
         return PENNANT_EINVAL;
     }
@@ -434,6 +443,7 @@ int pennant_batch_get_tuning(pennant_batch* b, const char* key) {
     if (!strcmp(key, "max_blocks")) return b->tune.max_blocks;
     if (!strcmp(key, "xcd")) return b->tune.xcd;
     if (!strcmp(key, "graph")) return b->tune.graph;
+    if (!strcmp(key, "rect_maps")) return b->tune.rect_maps;
     return PENNANT_EINVAL;
 }
 

@@ -99,6 +99,7 @@ struct FusedArgs {
     int rect_nzx;
     unsigned long long rect_m;  // z / nzx = (z * rect_m) >> rect_sh
     int rect_sh;
+    int rect_so, rect_po;       // the mesh's first side and point in the arrays (a batch's arena)
     int* zlist;                // zones for the learned pass, run r at [zbeg_r, zend_r)
     unsigned* zcount;          // per run and list slice (list_count)
     int nzruns;
@@ -177,9 +178,9 @@ template <int NS>
 __device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS]) {
     if constexpr (NS == 4) {
         if (a.rect_nzx > 0) {
-            const RectZone r = rect_zone(a, sb >> 2);
+            const RectZone r = rect_zone(a, (sb - a.rect_so) >> 2);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) pp[k] = rect_point(a, r, k);
+            for (int k = 0; k < NS; ++k) pp[k] = a.rect_po + rect_point(a, r, k);
             return;
         }
     }
@@ -193,9 +194,9 @@ template <int NS>
 __device__ __forceinline__ void load_slots(const FusedArgs& a, int sb, int (&slot)[NS]) {
     if constexpr (NS == 4) {
         if (a.rect_nzx > 0) {
-            const RectZone r = rect_zone(a, sb >> 2);
+            const RectZone r = rect_zone(a, (sb - a.rect_so) >> 2);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) slot[k] = rect_rank(a, r, k) * a.nump + rect_point(a, r, k);
+            for (int k = 0; k < NS; ++k) slot[k] = rect_rank(a, r, k) * a.nump + a.rect_po + rect_point(a, r, k);
             return;
         }
     }
@@ -2208,6 +2209,8 @@ FusedArgs make_args(pennant_problem* pb) {
     a.rect_nzx = (pb->tune.rect_maps && pb->rect_nzx > 0) ? pb->rect_nzx : 0;
     a.rect_m = pb->rect_m;
     a.rect_sh = pb->rect_sh;
+    a.rect_so = 0;
+    a.rect_po = 0;
     a.zlist = pb->zlist;
     a.zcount = pb->zcount;
     a.nzruns = (int)pb->zruns.size() / 3;
@@ -2295,6 +2298,7 @@ struct BatchArgs {
     int nheavy;
     const BProb* probs;
     Ctl* ctls;
+    int rect_maps;                 // rect problems: points and corner slots computed
 };
 
 __device__ __forceinline__ void apply_prob(FusedArgs& a, const BProb& q) {
@@ -2302,6 +2306,15 @@ __device__ __forceinline__ void apply_prob(FusedArgs& a, const BProb& q) {
     a.q1 = q.q1; a.q2 = q.q2; a.cfl = q.cfl;
     a.nbcs = q.nbcs;
     a.bcv = q.bcvx;            // q.bcvy follows q.bcvx in memory
+}
+// A rect problem's zone kernels compute its zones' points and corner slots
+// (offset to the problem's place in the arena) when `on`.
+__device__ __forceinline__ void apply_rect(FusedArgs& a, const BProb& q, int on) {
+    a.rect_nzx = on ? q.rect_nzx : 0;
+    a.rect_m = q.rect_m;
+    a.rect_sh = q.rect_sh;
+    a.rect_so = q.so;
+    a.rect_po = q.po;
 }
 
 // A batch unit's side work force slots: the unit's zones are a PN_BLOCK
@@ -2376,6 +2389,7 @@ k_pred_batch(BatchArgs B) {
         FusedArgs a = B.a;
         if constexpr (CL == CL_LEARNED) a.pack = reinterpret_cast<const ClosurePack*>(wq);
         apply_prob(a, B.probs[U.pid]);
+        apply_rect(a, B.probs[U.pid], B.rect_maps);
         a.ctl = c;                 // the problem's zr/ze flag for the predictor
         double dtmin = 1.e99;
         int bad = 0;
@@ -2446,8 +2460,13 @@ __global__ void __launch_bounds__(PN_BLOCK) k_corr_batch(BatchArgs B) {
         int bad = 0;
         const int z = U.zb + (int)threadIdx.x;
         if (z < U.ze) {
-            if (U.ns == 4) corr_zone<4, T>(B.a, z, U.sb + (z - U.zb) * 4, unit_w(U, z), dt, bad, dvov);
-            else corr_zone<3, T>(B.a, z, U.sb + (z - U.zb) * 3, unit_w(U, z), dt, bad, dvov);
+            if (U.ns == 4) {
+                FusedArgs a = B.a;
+                apply_rect(a, B.probs[U.pid], B.rect_maps);
+                corr_zone<4, T>(a, z, U.sb + (z - U.zb) * 4, unit_w(U, z), dt, bad, dvov);
+            } else {
+                corr_zone<3, T>(B.a, z, U.sb + (z - U.zb) * 3, unit_w(U, z), dt, bad, dvov);
+            }
         }
         const double m = block_max(dvov);
         const int nb = block_sum(bad);
@@ -2506,6 +2525,9 @@ BatchArgs make_batch_args(pennant_batch* b) {
     a.rect_nzx = 0;
     a.rect_m = 0;
     a.rect_sh = 0;
+    a.rect_so = 0;
+    a.rect_po = 0;
+    B.rect_maps = b->tune.rect_maps;
     B.units = b->units; B.nunits = b->nunits;
     B.punits = b->punits; B.npunits = b->npunits;
     B.heavy = b->heavy; B.nheavy = b->nheavy;

@@ -180,6 +180,10 @@ struct BProb {
     double gamma, ssmin, alfa, qgamma, q1, q2, cfl, cflv, dtmax, dtfac, tstop;
     double bcvx[8], bcvy[8];
     int cstop, nbcs, ubeg, uend;
+    // rect problems (rect_nzx > 0): z / nzx = (z * rect_m) >> rect_sh for the
+    // problem's own zone z; so, po: its first side and point in the arena
+    unsigned long long rect_m;
+    int rect_nzx, rect_sh, so, po;
 };
 }  // namespace pn
 
