@@ -151,7 +151,9 @@ __device__ __forceinline__ void st_nt(T v, T* p) { __builtin_nontemporal_store(v
 // p0 + 1 + npx, p0 + npx (npx = nzx + 1, p0 = zy npx + zx = z + zy), and the
 // corner of side k is the rank-th corner of its point in ascending side
 // order -- the number of the point's zones before z, of the four (zx-1,
-// zy-1), (zx, zy-1), (zx-1, zy), (zx, zy) around it that exist.
+// zy-1), (zx, zy-1), (zx-1, zy), (zx, zy) around it that exist. In a batch's
+// arena z counts from the mesh's first side a.rect_so / 4 and p0 is offset
+// by its first point a.rect_po (both 0 for a single problem).
 struct RectZone {
     int p0, zx, zy;
 };
@@ -159,7 +161,7 @@ __device__ __forceinline__ RectZone rect_zone(const FusedArgs& a, int z) {
     RectZone r;
     r.zy = (int)(((unsigned long long)(unsigned)z * a.rect_m) >> a.rect_sh);
     r.zx = z - r.zy * a.rect_nzx;
-    r.p0 = z + r.zy;
+    r.p0 = z + r.zy + a.rect_po;
     return r;
 }
 __device__ __forceinline__ int rect_point(const FusedArgs& a, const RectZone& r, int k) {
@@ -180,7 +182,7 @@ __device__ __forceinline__ void load_pp(const FusedArgs& a, int sb, int (&pp)[NS
         if (a.rect_nzx > 0) {
             const RectZone r = rect_zone(a, (sb - a.rect_so) >> 2);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) pp[k] = a.rect_po + rect_point(a, r, k);
+            for (int k = 0; k < NS; ++k) pp[k] = rect_point(a, r, k);
             return;
         }
     }
@@ -196,7 +198,7 @@ __device__ __forceinline__ void load_slots(const FusedArgs& a, int sb, int (&slo
         if (a.rect_nzx > 0) {
             const RectZone r = rect_zone(a, (sb - a.rect_so) >> 2);
 #pragma unroll
-            for (int k = 0; k < NS; ++k) slot[k] = rect_rank(a, r, k) * a.nump + a.rect_po + rect_point(a, r, k);
+            for (int k = 0; k < NS; ++k) slot[k] = rect_rank(a, r, k) * a.nump + rect_point(a, r, k);
             return;
         }
     }
