@@ -4,8 +4,8 @@
     own single-problem run (and so to the CPU oracle), with problems of
     different mesh types, sizes, parameters and stop conditions;
   * the mixed form of a batch matches each problem's own mixed run;
-  * graph replay, grid shapes and the autotuner do not change results, and
-    the autotuner does not advance the problems;
+  * graph replay, grid shapes, computed rect maps and the autotuner do not
+    change results, and the autotuner does not advance the problems;
   * a batch records training tuples and runs the learned closure.
 """
 import numpy as np
@@ -77,6 +77,25 @@ def test_batch_mixed_bit_identical_to_single_mixed_runs():
             for f in STATE:
                 assert np.array_equal(bits(b.get(i, f)), bits(want[f])), (i, f)
             assert rel_l1(b.get(i, "zr"), want["zr"]) == 0.0
+
+
+@pytest.mark.parametrize("mode", [pa.MODE_FUSED, pa.MODE_MIXED])
+def test_batch_rect_maps_do_not_change_results(mode):
+    """Rect problems placed after pie problems in the arena (non-zero first
+    side and point) give the same bits with their points and corner slots
+    computed from the zone index as with the maps read."""
+    decks = mixed_decks()
+    out = []
+    for rect in (1, 0):
+        with pa.Batch(decks) as b:
+            b.set_tuning("rect_maps", rect)
+            b.run(100, mode)
+            out.append([(b.scalars(i)["cycle"], [bits(b.get(i, f)) for f in STATE])
+                        for i in range(len(decks))])
+    for i, (on, off) in enumerate(zip(*out)):
+        assert on[0] == off[0], i
+        for f, x, y in zip(STATE, on[1], off[1]):
+            assert np.array_equal(x, y), (i, f)
 
 
 @pytest.mark.parametrize("maxb,xcd", [(1, 0), (7, 1), (64, 1), (65535, 0)])
